@@ -1,0 +1,672 @@
+"""Pins the CPU oracle against the reference's own known-answer tests.
+
+Each test restates the inputs and expected values of one reference test
+(GoogleTest, src/*_test.cpp) and checks the oracle's float build and/or its
+FP64 build against them. Values are transcribed from the reference tests as
+data; citations give file:line.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+LIVE, BAD, MISSING = 0, 1, 2
+
+
+# ---------------------------------------------------------------------------
+# AddInterval (blend_test.cpp:19-99)
+# ---------------------------------------------------------------------------
+def test_add_interval_first():
+    assert O.add_intervals([(100, 200)]) == [(100, 200)]
+
+
+def test_add_interval_two_disjoint():
+    assert O.add_intervals([(100, 200), (300, 400)]) == [(100, 200), (300, 400)]
+
+
+def test_add_interval_two_overlapping():
+    assert O.add_intervals([(100, 200), (150, 250)]) == [(100, 250)]
+
+
+def test_add_interval_three_overlapping():
+    ivs = [(50, 75), (100, 200), (250, 350), (400, 500), (175, 275)]
+    assert O.add_intervals(ivs) == [(50, 75), (100, 350), (400, 500)]
+
+
+def test_add_interval_adjacent_merges():
+    # adjacency merges too (blend.c:24-27, 59-60)
+    assert O.add_intervals([(0, 10), (10, 20)]) == [(0, 20)]
+    assert O.add_intervals([(10, 20), (0, 10)]) == [(0, 20)]
+
+
+def _interval_union_bruteforce(ivs):
+    covered = set()
+    for a, b in ivs:
+        covered.update(range(a, b))
+    # maximal runs where adjacency also merges
+    runs = []
+    for t in sorted(covered):
+        if runs and runs[-1][1] == t:
+            runs[-1][1] = t + 1
+        else:
+            runs.append([t, t + 1])
+    return sorted(tuple(r) for r in runs)
+
+
+def test_add_interval_matches_union_bruteforce():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        n = int(rng.integers(1, 12))
+        ivs = []
+        for _ in range(n):
+            a = int(rng.integers(0, 60))
+            ivs.append((a, a + int(rng.integers(1, 12))))
+        got = O.add_intervals(ivs)
+        assert sorted(got) == _interval_union_bruteforce(ivs)
+        # pairwise non-overlapping and non-adjacent
+        s = sorted(got)
+        for (a0, b0), (a1, b1) in zip(s, s[1:]):
+            assert b0 < a1
+
+
+# ---------------------------------------------------------------------------
+# GetUniqueChannels / GetBlendConfig / GetBlendParams (blend_test.cpp:101-309)
+# ---------------------------------------------------------------------------
+def test_unique_channels_two_patches():
+    ch = [np.array([123, 456, 0, 123]), np.array([123, -789])]
+    tt = [np.array([LIVE, BAD, MISSING, LIVE]), np.array([LIVE, LIVE])]
+    cfg = O.BlendConfig([4, 2], [10, 10], [np.zeros(4), np.zeros(2)], ch, tt)
+    assert cfg.unique_channels() == [123, 456, -789]
+    c0 = cfg.coords(0)[:, 0]
+    c1 = cfg.coords(1)[:, 0]
+    assert list(c0) == [0, 1, -1, 0]
+    assert list(c1) == [0, 2]
+
+
+def test_blend_config_two_channels():
+    cfg = O.BlendConfig([1, 1], [100, 50], [[25], [35]], [[1], [0]], [[LIVE], [LIVE]])
+    assert cfg.n_channels == 2
+    assert cfg.unique_channels() == [1, 0]
+    iv = cfg.intervals()
+    assert iv.tolist() == [[0, 25, 125], [1, 35, 85]]
+
+
+def test_blend_params_two_live():
+    # config with one interval [100, 250) on channel 0 (blend_test.cpp:212-254)
+    cfg = O.BlendConfig([1], [150], [[100]], [[0]], [[LIVE]])
+    bc = cfg.coords_for(50, [100, 200], [0, 0], [LIVE, LIVE])
+    assert bc.tolist() == [[0, 0, 0, 0, 50], [0, 0, 100, 0, 50]]
+
+
+def test_blend_params_partial_overlap():
+    cfg = O.BlendConfig([1], [100], [[50]], [[0]], [[LIVE]])
+    bc = cfg.coords_for(100, [100], [0], [LIVE])
+    assert bc.tolist() == [[0, 0, 50, 0, 50]]
+
+
+def test_blend_params_missing():
+    cfg = O.BlendConfig([1], [100], [[50]], [[0]], [[LIVE]])
+    bc = cfg.coords_for(50, [-1], [-1], [MISSING])
+    assert bc[0, 0] == -1
+
+
+# ---------------------------------------------------------------------------
+# Blend.ForwardAndAdjoint (blend_test.cpp:311-558)
+# ---------------------------------------------------------------------------
+def _fwd_adj_case(double):
+    shot = [np.array([-5, -3, -1]), np.array([-10, -4, -2, 0])]
+    ch = [np.array([123, 123, 123]), np.array([456, 123, 123, 123])]
+    tt = [np.array([MISSING, LIVE, LIVE]), np.array([LIVE, LIVE, BAD, MISSING])]
+    data = [np.arange(1, 10, dtype=float), np.arange(50, 66, dtype=float)]
+    cfg = O.BlendConfig([3, 4], [3, 4], shot, ch, tt, double=double)
+    return cfg, data
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_blend_forward_and_adjoint(double):
+    cfg, data = _fwd_adj_case(double)
+    assert cfg.n_channels == 2
+    assert cfg.unique_channels() == [123, 456]
+    assert cfg.intervals().tolist() == [[0, -4, 2], [1, -10, -6]]
+    assert cfg.mutes().tolist() == [[0, 0, 2, 4]]
+    assert cfg.coords(0).tolist()[1:] == [[0, 0, 1, 0, 3], [0, 0, 3, 0, 3]]
+    assert cfg.coords(0)[0, 0] == -1
+    assert cfg.coords(1).tolist()[:3] == [[1, 0, 0, 0, 4], [0, 0, 0, 0, 4], [0, 0, 2, 0, 4]]
+    assert cfg.coords(1)[3, 0] == -1
+    # record layout: channel 0 interval (6 samples) then channel 1 (4)
+    over = cfg.forward(data, mode=2)
+    assert over.tolist() == [54, 55, 58, 59, 60, 61, 50, 51, 52, 53]
+    s = cfg.forward(data, mode=0)
+    assert s.tolist() == [54, 59, 119, 129, 68, 70, 50, 51, 52, 53]
+    m = cfg.mute(s)
+    assert m.tolist() == [54, 59, 0, 0, 0, 0, 50, 51, 52, 53]
+    a0 = cfg.adjoint(m, 0)
+    a1 = cfg.adjoint(m, 1)
+    assert a0.tolist() == [[0, 0, 0], [59, 0, 0], [0, 0, 0]]
+    assert a1.tolist() == [[50, 51, 52, 53], [54, 59, 0, 0], [0, 0, 0, 0], [0, 0, 0, 0]]
+
+
+# ---------------------------------------------------------------------------
+# Blend.MeanForward (blend_test.cpp:560-691)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("double", [False, True])
+def test_blend_mean_forward(double):
+    w = [0.5 * math.sqrt(2 - math.sqrt(2)), 0.5 * math.sqrt(2), 0.5 * math.sqrt(2 + math.sqrt(2))]
+    shot = [np.array([-5, -3]), np.array([-10, -4])]
+    ch = [np.array([123, 123]), np.array([456, 123])]
+    tt = [np.array([LIVE, LIVE]), np.array([LIVE, LIVE])]
+    data = [np.arange(1, 21, dtype=float), np.arange(50, 72, dtype=float)]
+    cfg = O.BlendConfig([2, 2], [10, 11], shot, ch, tt, double=double)
+    assert cfg.n_channels == 2 and len(cfg.mutes()) == 0
+    assert cfg.intervals().tolist() == [[0, -5, 7], [1, -10, 1]]
+    cnt = [w[0], w[1] + w[0], w[2] + w[0] + w[1], 1 + w[1] + w[2], 1 + w[2] + 1, 3, 3,
+           w[2] + 2, w[1] + 2, w[0] + 2 * w[2], 2 * w[1], 2 * w[0]]
+    val = [1 * w[0], 2 * w[1] + 61 * w[0], 3 * w[2] + 11 * w[0] + 62 * w[1],
+           4 + 12 * w[1] + 63 * w[2], 5 + 13 * w[2] + 64, 6 + 14 + 65, 7 + 15 + 66,
+           8 * w[2] + 16 + 67, 9 * w[1] + 17 + 68, 10 * w[0] + (18 + 69) * w[2],
+           (19 + 70) * w[1], (20 + 71) * w[0]]
+    rec = cfg.forward(data, mode=1, taper_length=3)
+    tol = 1e-12 if double else 2e-6
+    np.testing.assert_allclose(rec[:12], np.array(val) / np.array(cnt), rtol=tol)
+    np.testing.assert_allclose(rec[12:], data[1][:11], rtol=tol)
+
+
+# ---------------------------------------------------------------------------
+# Blend.DotTest (blend_test.cpp:693-849), own random draws
+# ---------------------------------------------------------------------------
+def test_blend_dot_fp64():
+    rng = np.random.default_rng(11)
+    for _ in range(8):
+        np_ = int(rng.integers(1, 5))
+        ntr = [int(rng.integers(1, 64)) for _ in range(np_)]
+        nt = [int(rng.integers(1, 64)) for _ in range(np_)]
+        smax = max(1, sum(a * b for a, b in zip(ntr, nt)) // 3)
+        cmax = max(1, sum(ntr) // 10)
+        shot = [rng.integers(0, smax, n) for n in ntr]
+        ch = [rng.integers(0, cmax, n) for n in ntr]
+        tt = [rng.integers(0, 3, n) for n in ntr]
+        cfg = O.BlendConfig(ntr, nt, shot, ch, tt, double=True)
+        x = [rng.random(a * b) for a, b in zip(ntr, nt)]
+        yp = cfg.mute(rng.random(cfg.record_len))
+        y = cfg.mute(cfg.forward(x, mode=0))
+        xp = [cfg.adjoint(yp, p).reshape(-1) for p in range(np_)]
+        sx = sum(float(np.dot(a, b)) for a, b in zip(x, xp))
+        sy = float(np.dot(y, yp))
+        assert abs(sx - sy) < 1e-12 * max(1.0, abs(sy))
+
+
+# ---------------------------------------------------------------------------
+# FK.SetTaperWindow (fk_test.cpp:19-175)
+# ---------------------------------------------------------------------------
+TAPER_KAT = {
+    (2, 0, 0): [1, 1], (2, 1, 1): [0, 1], (2, 0, 1): [1, 1], (2, 1, 0): [0, 1],
+    (3, 0, 0): [1, 1, 1], (3, 1, 1): [0, 1, 1], (3, 0, 1): [1, 1, 1], (3, 1, 0): [0, 1, 1],
+    (4, 0, 0): [1, 1, 1, 1], (4, 1, 1): [0, .5, 1, .5], (4, 0, 1): [1, 1, 1, .5],
+    (4, 1, 0): [0, .5, 1, 1],
+    (5, 0, 0): [1] * 5, (5, 1, 1): [0, .5, 1, 1, .5], (5, 0, 1): [1, 1, 1, 1, .5],
+    (5, 1, 0): [0, .5, 1, 1, 1],
+}
+
+
+@pytest.mark.parametrize("key", sorted(TAPER_KAT))
+def test_taper_window(key):
+    n, left, right = key
+    got = O.taper_window(n, left, right)
+    np.testing.assert_allclose(got, TAPER_KAT[key], atol=1e-7)
+
+
+def test_taper_partition_of_unity():
+    # interior Hann tapers of 50%-overlapping windows sum to one
+    for W in [4, 8, 16, 32, 256]:
+        t = O.taper_window(W, 1, 1, double=True)
+        h = W // 2
+        np.testing.assert_allclose(t[h:] + t[:h], 1.0, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# FK.ForwardOnes and configuration (fk_test.cpp:177-545) via the model path:
+# a DC-only model gives a tapered constant (no blend involved).
+# ---------------------------------------------------------------------------
+def _ceil125(w):
+    return int(math.ceil(w * 1.25))
+
+
+def _single_window_model(shape, wshape, double):
+    nd = len(shape)
+    ntr = int(np.prod(shape[:-1]))
+    return O.Model([0], [nd], [wshape], [[0] * (nd - 1)], [shape],
+                   [np.arange(ntr) * 10000], [np.zeros(ntr)], [np.zeros(ntr)],
+                   double=double)
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_fk_forward_ones_2d(double):
+    # fk_test.cpp window 0 has use_taper {1,0} in space; here a lone patch
+    # whose window covers the whole space dim gets no space taper, and the
+    # time dim is tapered both sides. Window (6, 8), n_fk = 16.
+    m = _single_window_model([6, 8], [6, 8], double)
+    assert m.n_windows == 1
+    wi = m.windows[0]
+    assert wi["P"] == [_ceil125(6), _ceil125(8)]
+    assert wi["Ft"] == _ceil125(8) // 2 + 1
+    assert wi["nfk"] == 16 and wi["ntw"] == 3
+    scaler = 1.0 / math.sqrt(_ceil125(6) * _ceil125(8))
+    assert abs(wi["scaler"] - scaler) < 1e-7
+    x = np.zeros(m.window_size(0), np.complex128)
+    for tw in range(wi["ntw"]):
+        x[tw * wi["npad_tr"] * wi["Ft"]] = 1.0
+    y = m.window_fk_forward(0, x)
+    tt = [0.5 * (1 - math.cos(2 * math.pi * i / 8)) for i in range(4)]
+    tol = 1e-12 if double else 1e-6
+    for tr in range(6):
+        for t in range(4):
+            assert abs(y[tr, t] - scaler * tt[t]) < tol
+        for t in range(4, 12):
+            assert abs(y[tr, t] - scaler) < tol
+        for t in range(12, 16):
+            t_end = 0.5 * (1 - math.cos(2 * math.pi * (t + 4 - 12) / 8))
+            assert abs(y[tr, t] - scaler * t_end) < tol
+        assert np.all(y[tr, 16:] == 0)
+
+
+def test_fk_forward_ones_3d_space_taper():
+    # fk_test.cpp window 2: shape (2,3,4), use_taper {1,1,0,0}: the first
+    # trace of dim 0 is zero, the rest follow the time taper. Reproduced with
+    # three patches in a row along dim 0: the middle one is tapered on both
+    # sides of dim 0.
+    nd = 3
+    shapes = [[2, 3, 12]] * 3
+    ntr = 6
+    m = O.Model([0, 0, 0], [nd], [[2, 3, 4]], [[0, 0], [1, 0], [2, 0]], shapes,
+                [np.arange(ntr) * 1000 + 100000 * p for p in range(3)],
+                [np.zeros(ntr)] * 3, [np.zeros(ntr)] * 3, double=True)
+    wi = m.windows[1]
+    assert wi["use_taper"] == [1, 1, 0, 0]
+    assert wi["P"] == [_ceil125(2), _ceil125(3), _ceil125(4)]
+    scaler = 1 / math.sqrt(_ceil125(2) * _ceil125(3) * _ceil125(4))
+    x = np.zeros(m.window_size(1), np.complex128)
+    for tw in range(wi["ntw"]):
+        x[tw * wi["npad_tr"] * wi["Ft"]] = 1.0
+    y = m.window_fk_forward(1, x)
+    nfk = wi["nfk"]
+    assert np.all(np.abs(y[:3, :nfk]) < 1e-14)
+    tt = [0.5 * (1 - math.cos(2 * math.pi * i / 4)) for i in range(2)]
+    for tr in range(3, 6):
+        np.testing.assert_allclose(y[tr, :2], [scaler * v for v in tt], atol=1e-14)
+        np.testing.assert_allclose(y[tr, 2:nfk - 2], scaler, atol=1e-14)
+        np.testing.assert_allclose(y[tr, nfk - 2:nfk], [scaler * 0.5 * (1 - math.cos(2 * math.pi * (i + 2) / 4)) for i in range(2)], atol=1e-14)
+
+
+def test_fk_config_matches_formulas():
+    # fk.c:75-169: P = ceil(1.25 W), Ft = P_t/2+1, ntw = n_fk/(W_t/2) - 1
+    for W in [2, 4, 6, 8, 10, 14, 16, 30, 32, 64, 256, 512]:
+        assert _ceil125(W) == int(np.ceil(np.float32(W) * np.float32(1.25)))
+    m = _single_window_model([16, 100], [8, 32], False)
+    for wi in m.windows:
+        assert wi["P"] == [_ceil125(8), _ceil125(32)]
+        assert wi["ntw"] == wi["nfk"] // 16 - 1
+
+
+# ---------------------------------------------------------------------------
+# FK against the DFT definition (numpy) and FK.DotTest (fk_test.cpp:547-660)
+# ---------------------------------------------------------------------------
+def _fk_forward_numpy(m, w, x):
+    """Restatement with numpy.fft: y = crop(taper * scaler * c2r_nd(x))."""
+    wi = m.windows[w]
+    P = wi["P"]
+    nd = wi["ndim"]
+    W = wi["W"]
+    half = W[-1] // 2
+    xs = x.reshape([wi["ntw"]] + P[:-1] + [wi["Ft"]])
+    # FFTW c2r: DFTs along leading dims, then c2r along the last, ignoring
+    # imag parts of DC/Nyquist — for Hermitian-consistent input this equals
+    # irfftn * N.
+    y = np.fft.irfftn(xs, s=P, axes=list(range(1, nd + 1))) * np.prod(P)
+    out = np.zeros((wi["ntr"], wi["nconv"]))
+    tapers = [m.taper(w, d).astype(np.float64) for d in range(nd)]
+    for t in range(wi["ntr"]):
+        idx = np.unravel_index(t, W[:-1])
+        sc = wi["scaler"] * np.prod([tapers[d][idx[d]] for d in range(nd - 1)])
+        for tw in range(wi["ntw"]):
+            out[t, tw * half:tw * half + W[-1]] += y[(tw,) + tuple(idx)][:W[-1]] * sc * tapers[-1]
+    return out
+
+
+def _random_geometry_model(rng, double=True, n_dims=None, wavelets=False):
+    nd = n_dims or int(rng.integers(2, 5))
+    shape, wshape = [], []
+    for d in range(nd):
+        L = int(rng.integers(4, 12)) if d < nd - 1 else int(rng.integers(8, 24))
+        W = int(rng.integers(2, L + 1))
+        if W != L or d == nd - 1:
+            W = max(2, (W // 2) * 2)
+        shape.append(L)
+        wshape.append(W)
+    ntr = int(np.prod(shape[:-1]))
+    shot = rng.integers(0, ntr * shape[-1] // 3 + 1, ntr)
+    ch = rng.integers(0, 1 + ntr // 10, ntr)
+    tt = rng.integers(0, 3, ntr)
+    tt[0] = LIVE
+    kw = {}
+    if wavelets:
+        lw = int(rng.integers(1, 5))
+        nw = int(rng.integers(1, 4))
+        kw = dict(wavelet_lengths=[lw] * nw, wavelet_idxs=[rng.integers(0, nw, ntr)],
+                  wavelets=[rng.random(lw) - 0.5 for _ in range(nw)])
+    return O.Model([0], [nd], [wshape], [[0] * (nd - 1)], [shape], [shot], [ch], [tt],
+                   double=double, **kw), shape
+
+
+def test_fk_forward_matches_numpy_dft():
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        m, _ = _random_geometry_model(rng)
+        for w in range(m.n_windows):
+            wi = m.windows[w]
+            P = wi["P"]
+            real = rng.random([wi["ntw"]] + P) - 0.5
+            x = np.fft.rfftn(real, axes=list(range(1, wi["ndim"] + 1))).reshape(-1)
+            got = m.window_fk_forward(w, x)
+            ref = _fk_forward_numpy(m, w, x)
+            assert np.abs(got - ref).max() <= 1e-11 * max(1, np.abs(ref).max())
+
+
+def test_fk_adjoint_matches_numpy_dft():
+    rng = np.random.default_rng(6)
+    for _ in range(10):
+        m, _ = _random_geometry_model(rng)
+        for w in range(m.n_windows):
+            wi = m.windows[w]
+            P, W, nd = wi["P"], wi["W"], wi["ndim"]
+            half = W[-1] // 2
+            tr = rng.random((wi["ntr"], wi["nconv"])) - 0.5
+            tapers = [m.taper(w, d).astype(np.float64) for d in range(nd)]
+            z = np.zeros([wi["ntw"]] + P)
+            for t in range(wi["ntr"]):
+                idx = np.unravel_index(t, W[:-1])
+                sc = wi["scaler"] * np.prod([tapers[d][idx[d]] for d in range(nd - 1)])
+                for tw in range(wi["ntw"]):
+                    z[(tw,) + tuple(idx)][:W[-1]] = tr[t, tw * half:tw * half + W[-1]] * sc * tapers[-1]
+            g = np.fft.rfftn(z, axes=list(range(1, nd + 1)))
+            g[..., 1:1 + (P[-1] - 1) // 2] *= 2
+            got = m.window_fk_adjoint(w, tr)
+            assert np.abs(got - g.reshape(-1)).max() <= 1e-11 * max(1, np.abs(g).max())
+
+
+def test_fk_dot_fp64():
+    rng = np.random.default_rng(7)
+    for _ in range(10):
+        m, _ = _random_geometry_model(rng)
+        for w in range(m.n_windows):
+            wi = m.windows[w]
+            real = rng.random([wi["ntw"]] + wi["P"]) - 0.5
+            x = np.fft.rfftn(real, axes=list(range(1, wi["ndim"] + 1))).reshape(-1)
+            yp = rng.random((wi["ntr"], wi["nconv"])) - 0.5
+            y = m.window_fk_forward(w, x)
+            xp = m.window_fk_adjoint(w, yp)
+            sx = float(np.sum(x.real * xp.real + x.imag * xp.imag))
+            sy = float(np.sum(y * yp))
+            assert abs(sx - sy) < 5e-11
+
+
+# ---------------------------------------------------------------------------
+# Wavelet.Forward (wavelet_test.cpp:19-130): delta wavelets shift traces
+# ---------------------------------------------------------------------------
+def test_wavelet_delta_shift():
+    for n in [5, 6, 7, 4767]:
+        for s in range(3):
+            wav = np.zeros(3)
+            wav[s] = 1
+            spec = O.wavelet_spectrum(wav, n, double=True)
+            x = np.zeros(n)
+            x[: min(4, n - 3)] = np.arange(1, 1 + min(4, n - 3))
+            X = np.fft.rfft(x) * spec
+            y = np.fft.irfft(X, n) * n
+            np.testing.assert_allclose(y, np.roll(x, s), atol=1e-10)
+
+
+def test_wavelet_spectrum_phase_only():
+    rng = np.random.default_rng(2)
+    for n in [16, 21, 4767]:
+        wav = rng.random(10) - 0.5
+        spec = O.wavelet_spectrum(wav, n, double=True)
+        ref = np.fft.rfft(np.concatenate([wav, np.zeros(n - 10)]))
+        ph = np.angle(ref)
+        np.testing.assert_allclose(spec, np.exp(1j * ph) / n, atol=1e-14)
+
+
+def test_wavelet_window_forward_kat():
+    # wavelet_test.cpp:83-129 through a model window: live traces with delta
+    # wavelets at 0, 1, 2 are shifted by 0, 1, 2 samples.
+    nd = 2
+    ntr = 3
+    shape = [ntr, 4]
+    m = O.Model([0], [nd], [[ntr, 2]], [[0]], [shape], [np.arange(ntr) * 100],
+                [np.zeros(ntr)], [np.zeros(ntr)], wavelet_lengths=[3, 3, 3],
+                wavelet_idxs=[np.array([0, 1, 2])],
+                wavelets=[np.array([1., 0, 0]), np.array([0., 1, 0]), np.array([0., 0, 1])],
+                double=True)
+    wi = m.windows[0]
+    n = wi["nconv"]
+    buf = np.zeros((ntr, n))
+    for t in range(ntr):
+        buf[t, :4] = 100 + 4 * t + np.arange(4)
+    out = m.window_wavelet(0, buf, 1.0)
+    for t in range(ntr):
+        np.testing.assert_allclose(out[t], np.roll(buf[t], t), atol=1e-9)
+
+
+def test_wavelet_dot_fp64():
+    rng = np.random.default_rng(9)
+    for _ in range(6):
+        m, _ = _random_geometry_model(rng, wavelets=True)
+        for w in range(m.n_windows):
+            wi = m.windows[w]
+            n = wi["nconv"]
+            nfk = wi["nfk"]
+            x = np.zeros((wi["ntr"], n))
+            x[:, :nfk] = rng.random((wi["ntr"], nfk)) - 0.5
+            yp = rng.random((wi["ntr"], n)) - 0.5
+            y = m.window_wavelet(w, x, 1.0)
+            xp = m.window_wavelet(w, yp, -1.0)
+            assert abs(float(np.sum(x * xp)) - float(np.sum(y * yp))) < 1e-12 * max(1, np.abs(y).sum())
+
+
+# ---------------------------------------------------------------------------
+# Model.SetSpaceWindows (model_test.cpp:19-270)
+# ---------------------------------------------------------------------------
+def _windows_model(volumes, n_dims, coords, shapes, wshapes):
+    ntr = [int(np.prod(s[:-1])) for s in shapes]
+    return O.Model(volumes, n_dims, wshapes, coords, shapes,
+                   [np.arange(n) * 1000 + 100000 * p for p, n in enumerate(ntr)],
+                   [np.zeros(n) for n in ntr], [np.zeros(n) for n in ntr])
+
+
+def test_set_space_windows_kat():
+    m = _windows_model([0, 1, 2, 3, 3], [2, 3, 3, 3],
+                       [[0], [0, 0], [0, 0], [0, 0], [1, 1]],
+                       [[4, 8], [3, 5, 7], [3, 7, 7], [4, 2, 4], [6, 2, 4]],
+                       [[2, 8], [3, 2, 6], [2, 4, 6], [4, 2, 4]])
+    assert m.n_windows == 14
+    exp = [
+        # patch, n_traces, coords, shape, use_taper
+        (0, 2, [0], [2], [0, 1]), (0, 2, [1], [2], [1, 1]), (0, 2, [2], [2], [1, 0]),
+        (1, 6, [0, 0], [3, 2], [0, 0, 0, 1]), (1, 6, [0, 1], [3, 2], [0, 0, 1, 1]),
+        (1, 6, [0, 2], [3, 2], [0, 0, 1, 1]), (1, 6, [0, 3], [3, 2], [0, 0, 1, 0]),
+        (2, 10, [0, 0], [2, 5], [0, 1, 0, 1]), (2, 8, [0, 3], [2, 4], [0, 1, 1, 0]),
+        (2, 10, [1, 0], [2, 5], [1, 0, 0, 1]), (2, 8, [1, 3], [2, 4], [1, 0, 1, 0]),
+        (3, 8, [0, 0], [4, 2], [0, 1, 0, 1]), (4, 8, [0, 0], [4, 2], [1, 1, 1, 0]),
+        (4, 8, [2, 0], [4, 2], [1, 0, 1, 0]),
+    ]
+    for w, (p, ntr, c, s, ut) in enumerate(exp):
+        wi = m.windows[w]
+        assert (wi["patch"], wi["ntr"], wi["coords"], wi["shape"], wi["use_taper"]) == (p, ntr, c, s, ut), w
+
+
+def test_set_space_windows_larger_kat():
+    m = _windows_model([0], [2], [[0]], [[78, 8]], [[32, 8]])
+    assert m.n_windows == 3
+    got = [(wi["ntr"], wi["coords"], wi["shape"], wi["use_taper"]) for wi in m.windows]
+    assert got == [(37, [0], [37], [0, 1]), (37, [21], [37], [1, 1]), (36, [42], [36], [1, 0])]
+
+
+def test_window_arrays_kat():
+    # model_test.cpp:338-434
+    m = _windows_model([0], [3], [[0, 0]], [[3, 7, 7]], [[2, 4, 6]])
+    assert m.n_windows == 4
+    exp = [[0, 1, 2, 3, 4, 7, 8, 9, 10, 11], [3, 4, 5, 6, 10, 11, 12, 13],
+           [7, 8, 9, 10, 11, 14, 15, 16, 17, 18], [10, 11, 12, 13, 17, 18, 19, 20]]
+    for w in range(4):
+        assert m.window_traces(w, [3, 7, 7]).tolist() == exp[w]
+
+
+# ---------------------------------------------------------------------------
+# Model.GetStepsize (model_test.cpp:436-492)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("spacing,expected", [(16 + 8, 0.5), (16 + 7, 0.25)])
+def test_get_stepsize_kat(spacing, expected):
+    m = O.Model([0], [2], [[4, 8]], [[0]], [[16, 16]], [np.arange(16) * spacing],
+                [np.zeros(16)], [np.zeros(16)])
+    assert m.stepsize() == expected
+
+
+# ---------------------------------------------------------------------------
+# Model.Forward (model_test.cpp:494-596)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("double", [False, True])
+def test_model_forward_kat(double):
+    volumes = [0, 1, 2, 0]
+    n_dims = [2, 3, 2, 2]
+    shapes = [[4, 4], [2, 3, 5], [4, 4], [4, 4]]
+    wsh = [[2, 4], [2, 2, 4], [4, 4]]
+    coords = [[0], [0, 0], [0], [1]]
+    shot = [[1, 15, 51, 21], [7, 27, 35, 43, 100, 57], [1, 2, 3, 4], [21, 65, 71, 77]]
+    ch = [[1] * 4, [1, 1, 1, 1, 2, 1], [1] * 4, [1] * 4]
+    tt = [[LIVE] * 4, [LIVE] * 6, [MISSING] * 4, [LIVE] * 4]
+    widx = [[0] * 4, [0] * 6, [0] * 4, [0] * 4]
+    m = O.Model(volumes, n_dims, wsh, coords, shapes, shot, ch, tt,
+                wavelet_lengths=[1], wavelet_idxs=widx, wavelets=[np.array([1.0])],
+                double=double)
+    assert m.n_windows == 8
+    assert [wi["patch"] for wi in m.windows] == [0, 0, 0, 1, 1, 3, 3, 3]
+    assert m.intervals().tolist() == [[0, -1, 83], [1, 98, 108]]
+    x = m.zeros_model()
+    for w, wi in enumerate(m.windows):
+        for tw in range(wi["ntw"]):
+            x[wi["model_off"] + tw * wi["npad_tr"] * wi["Ft"]] = math.sqrt(wi["npad"])
+    rec = m.forward(x)
+    r0, r1 = rec[:84], rec[84:]
+    tol = 1e-12 if double else 2e-6
+    assert abs(r0[0]) < tol and abs(r0[1] - 0.5) < tol and abs(r0[83] - 0.5) < tol
+    np.testing.assert_allclose(r0[2:83], 1.0, atol=tol)
+    assert abs(r1[0]) < tol and abs(r1[1] - 0.5) < tol and abs(r1[9] - 0.5) < tol
+    np.testing.assert_allclose(r1[2:9], 1.0, atol=tol)
+
+
+# ---------------------------------------------------------------------------
+# Model.DotTest (model_test.cpp:598-835), own random draws
+# ---------------------------------------------------------------------------
+def _random_deblend_problem(rng, wavelets):
+    np_ = int(rng.integers(1, 4))
+    n_dims, shapes, wsh, ntr, wl, widx, wavs = [], [], [], [], [], [], []
+    for p in range(np_):
+        nd = int(rng.integers(2, 5))
+        pre = int(rng.integers(2, 9))
+        conv = pre + int(rng.integers(0, 4))
+        sh, ws = [], []
+        for d in range(nd):
+            L = conv if d == nd - 1 else int(rng.integers(2, 9))
+            W = int(rng.integers(2, L + 1))
+            if W != L or d == nd - 1:
+                W = max(2, (W // 2) * 2)
+            sh.append(L)
+            ws.append(W)
+        n_dims.append(nd)
+        shapes.append(sh)
+        wsh.append(ws)
+        ntr.append(int(np.prod(sh[:-1])))
+        lw = conv - pre + 1
+        base = len(wavs)
+        for _ in range(int(rng.integers(1, max(2, ntr[-1])))):
+            wavs.append(rng.random(lw) - 0.5)
+            wl.append(lw)
+        widx.append(rng.integers(base, len(wavs), ntr[-1]))
+    total = sum(n * s[-1] for n, s in zip(ntr, shapes))
+    shot = [rng.integers(0, max(1, total // 3), n) for n in ntr]
+    ch = [rng.integers(0, 1 + sum(ntr) // 10, n) for n in ntr]
+    tt = [rng.integers(0, 3, n) for n in ntr]
+    kw = dict(wavelet_lengths=wl, wavelet_idxs=widx, wavelets=wavs) if wavelets else {}
+    return dict(volumes=list(range(np_)), n_dims=n_dims, window_shapes=wsh,
+                coords=[[0] * (nd - 1) for nd in n_dims], shapes=shapes, shottimes=shot,
+                channels=ch, trace_types=tt, **kw)
+
+
+@pytest.mark.parametrize("wavelets", [False, True])
+def test_model_dot_fp64(wavelets):
+    rng = np.random.default_rng(21 + wavelets)
+    done = 0
+    while done < 5:
+        prob = _random_deblend_problem(rng, wavelets)
+        m = O.Model(double=True, **prob)
+        if m.n_windows == 0 or m.record_len == 0:
+            continue
+        x = m.zeros_model()
+        for w, wi in enumerate(m.windows):
+            real = rng.random([wi["ntw"]] + wi["P"]) - 0.5
+            xw = np.fft.rfftn(real, axes=list(range(1, wi["ndim"] + 1))).reshape(-1)
+            x[wi["model_off"]:wi["model_off"] + len(xw)] = xw
+        n_ch = m.L.n_channels(m.h)
+        yp = rng.random(m.record_len)
+        # mute yp through a forward of zero + mute: use the oracle's mute via
+        # blend of the model's own config is not exposed, so mute by the
+        # returned mute table
+        mut = m.mutes()
+        iv = m.intervals()
+        offs = np.concatenate([[0], np.cumsum(iv[:, 2] - iv[:, 1])])
+        first_iv = {}
+        for k, (c, a, b) in enumerate(iv):
+            first_iv.setdefault(int(c), k)
+        for c, i, s, n in mut:
+            o = offs[first_iv[int(c)] + int(i)] + s
+            yp[o:o + n] = 0
+        y = m.forward(x)
+        xp = m.adjoint_update(yp, -1.0, m.zeros_model())
+        sx = float(np.sum(x.real * xp.real + x.imag * xp.imag))
+        sy = float(np.dot(y, yp))
+        assert abs(sx - sy) < 1e-12 * max(1.0, abs(sy)), (sx, sy, n_ch)
+        done += 1
+
+
+# ---------------------------------------------------------------------------
+# AGDBlend API known answers (test.cpp:30-430)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("double", [False, True])
+def test_agd_blend_two_volumes_one_channel(double):
+    out = O.blend([1, 1], [3, 3], [[25], [27]], [[123], [123]], [[LIVE], [LIVE]],
+                  [np.array([1., 2, 3]), np.array([4., 5, 6])], 0, 0, [1, 1], [3, 3],
+                  [[25], [27]], [[123], [123]], [[LIVE], [LIVE]], double=double)
+    assert out[0].tolist() == [[1, 2, 7]] and out[1].tolist() == [[7, 5, 6]]
+
+
+@pytest.mark.parametrize("tin,tout", [(MISSING, MISSING), (MISSING, LIVE), (LIVE, MISSING)])
+def test_agd_blend_missing_cases(tin, tout):
+    out = O.blend([1], [3], [[25]], [[123]], [[tin]], [np.array([1., 2, 3])], 0, 0,
+                  [1], [3], [[25]], [[123]], [[tout]])
+    assert out[0].tolist() == [[0, 0, 0]]
+
+
+def test_agd_blend_one_volume_to_two_with_mute():
+    out = O.blend([5], [3], [[25, 27, 29, 31, -10]], [[123, 123, 123, 123, -789]],
+                  [[MISSING, LIVE, LIVE, BAD, LIVE]], [np.arange(1, 16, dtype=float)], 0, 0,
+                  [3, 2], [2, 3], [[24, 26, -10], [28, 30]], [[123, 123, -789], [123, 123]],
+                  [[MISSING, LIVE, LIVE], [LIVE, LIVE]])
+    assert out[0].tolist() == [[0, 0], [0, 4], [13, 14]]
+    assert out[1].tolist() == [[5, 13, 8], [8, 0, 0]]
+
+
+def test_agd_blend_rejects_bad_output_trace():
+    with pytest.raises(RuntimeError):
+        O.blend([1], [3], [[25]], [[123]], [[LIVE]], [np.array([1., 2, 3])], 0, 0,
+                [1], [3], [[25]], [[123]], [[BAD]])
