@@ -1,0 +1,127 @@
+/*
+ * Thin C-ABI between the C host (agd_host.c) and the HIP kernels
+ * (agd_kernels.hip). Plain pointers and sizes only; device pointers are
+ * opaque to the host code. All launches go to the library's own stream.
+ * Every function returns 0 on success and 1 on a HIP error (message on
+ * stderr).
+ */
+#ifndef AGD_GPU_H
+#define AGD_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "agd_layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device-resident tables and buffers of one deblending problem. Filled by
+ * the host; kernels read it by value. */
+typedef struct {
+  /* geometry */
+  int n_win;
+  AgdWin const *win;          /* [n_win] */
+  AgdFftPlan const *plans;    /* FFT plans */
+  float const *roots;         /* complex roots pool (interleaved) */
+  float const *tapers;        /* time taper pool */
+  float const *tscale;        /* per global window trace: scaler * space tapers */
+  float const *tconst;        /* per global window trace: space tapers only */
+  AgdEntry const *went;       /* per global window trace blend entry */
+  int const *trace_win;       /* per global window trace: its window */
+  int n_wtr;                  /* global window traces */
+  /* spatial tasks */
+  AgdTask const *tasks;
+  int n_tasks;
+  size_t lds_spatial;         /* bytes */
+  /* time blocks: (window, first trace) pairs */
+  int const *tblocks;         /* 2 ints per block */
+  int n_tblocks;
+  int tb;                     /* traces per time block */
+  size_t lds_time;
+  /* record gather of window traces */
+  int64_t rec_len;
+  int tile;
+  int n_tiles;
+  int const *tile_ptr;        /* [n_tiles+1] */
+  int const *tile_ent;        /* entry ids (global window traces) */
+  unsigned char const *mute;  /* [rec_len] 1 = muted */
+  /* wavelets */
+  int use_wavelets;
+  float const *wspec;         /* complex spectra pool */
+  int const *wsrc_off;        /* per global window trace: complex offset of
+                                 its spectrum in wspec, or -1 */
+  size_t lds_wavelet;
+  /* buffers */
+  float *X;                   /* model, complex */
+  float *U;                   /* intermediate, complex */
+  float *T;                   /* traces */
+  float *T2;                  /* traces after wavelet adjoint */
+  float *d;                   /* blended input record */
+  float *r;                   /* forward / residual record */
+  unsigned int *scalar;       /* device scalars (max bits etc.) */
+  double *partials;           /* reduction partials */
+  int n_partials;
+} AgdDev;
+
+/* A gather of source traces into a flat record, or of record slices back
+ * into traces (blend / pseudo-deblend). */
+typedef struct {
+  int64_t rec_len;
+  int tile;
+  int n_tiles;
+  int const *tile_ptr;
+  int const *tile_ent;
+  AgdEntry const *ent;        /* per source trace */
+  unsigned char const *mute;  /* may be NULL */
+  float const *taper;         /* mean-mode weights [taper_len] */
+  int taper_len;
+} AgdGather;
+
+int agdg_init(void);
+void *agdg_alloc(size_t bytes);
+void agdg_free(void *p);
+int agdg_h2d(void *dst, void const *src, size_t bytes);
+int agdg_d2h(void *dst, void const *src, size_t bytes);
+int agdg_zero(void *dst, size_t bytes);
+int agdg_sync(void);
+void *agdg_stream(void);
+
+/* forward operator pieces */
+int agdg_fwd_spatial(AgdDev const *g, float const *X);
+int agdg_fwd_time(AgdDev const *g);
+int agdg_wavelet(AgdDev const *g, int adjoint);
+/* r = mute(blend(T)) - d (resid=1) or mute(blend(T)) (resid=0) */
+int agdg_blend_traces(AgdDev const *g, int resid);
+/* adjoint pieces */
+int agdg_adj_time(AgdDev const *g, int from_record);
+/* mode 0: X = X - alpha*G then shrink by lamb; mode 1: X = -alpha*G and
+ * record max |X|^2 bits in scalar[0]; mode 2: X = X - alpha*G (no shrink) */
+int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode);
+int agdg_shrink(AgdDev const *g, int64_t n_complex, float lamb);
+/* fold of spatial tapers for the stepsize; max written to scalar[1] */
+int agdg_stepsize_fold(AgdDev const *g);
+/* sum of squares of a float array in FP64 (deterministic order) */
+int agdg_norm2(AgdDev const *g, float const *v, int64_t n, double *out);
+/* output assembly: out[o_off[i] + t] = sum_c T[o_src[c] + t], t < o_nt[i] */
+int agdg_output(float const *T, int n_out, int64_t const *o_off,
+                int const *o_nt, int const *o_ptr, int64_t const *o_src,
+                float *out);
+/* gather sources into a record: mode 0 sum, 1 mean, 2 overwrite;
+ * muted samples set to 0 when gt->mute is set */
+int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec);
+/* extract record slices into traces (blend adjoint, blend.c:1262-1292):
+ * out[e.src + t] = rec[e.rec + t - e.trace_start] for t in
+ * [trace_start, trace_start + n), 0 for other t < trace_len[i] */
+int agdg_extract(AgdEntry const *ent, int n_ent, int const *trace_len,
+                 float const *rec, float *out);
+/* timing of the library's kernels (per kernel name accumulators) */
+int agdg_timing(int enable);
+int agdg_timing_get(char const *name, double *ms, long *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1029 @@
+// HIP kernels of the MI355X deblending path (gfx950, wave64) and the thin
+// C-ABI the C host calls (declared in agd_gpu.h).
+//
+// Operators (reference equivalents):
+//   k_fwd_spatial  spatial inverse DFTs of fk_forward's C2R (fk.c:325-327),
+//                  cropped to the window's traces
+//   k_fwd_time     time C2R + taper + overlap-add of fk_forward (fk.c:329-372)
+//   k_blend        deterministic gather form of blend_sum_forward +
+//                  apply_mute + set_residual (blend.c:936-960, 862-875,
+//                  model.c:1314-1341), and of the sum / mean / overwrite
+//                  blends of agd_blend (blend.c:1007-1027, 1155-1260)
+//   k_adj_time     blend_adjoint + time split/taper + R2C + x2 of fk_adjoint
+//                  (blend.c:1262-1292, fk.c:388-451)
+//   k_adj_spatial  spatial forward DFTs of fk_adjoint's R2C, fused with the
+//                  model update and soft threshold (model.c:1149-1154,
+//                  1392-1423) and the initial threshold max (model.c:1202)
+//   k_wavelet      frequency-domain source-wavelet convolution
+//                  (wavelet.c:239-301), as a circular convolution evaluated
+//                  with a 7-smooth FFT length
+//   k_output       set_output's overlap-add into the patches (model.c:1425)
+//
+// All FFTs are mixed-radix Stockham passes in LDS (radix 4, 2, 3 and a
+// direct butterfly for any other prime), one wave64 group per block of
+// lines. Sums that the reference performs serially are done here in the
+// same order (gathers over precomputed, ordered entry lists), so no float
+// atomics are used and results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "agd_gpu.h"
+
+#define NTHR 256
+
+static hipStream_t g_stream = nullptr;
+static int g_timing = 0;
+
+struct KTiming {
+  double ms;
+  long count;
+};
+static std::map<std::string, KTiming> g_times;
+struct PendingEv {
+  std::string name;
+  hipEvent_t a, b;
+};
+static std::vector<PendingEv> g_pending;
+
+static int hip_check(hipError_t e, char const *what) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "ERROR: HIP %s: %s\n", what, hipGetErrorString(e));
+    return 1;
+  }
+  return 0;
+}
+
+static void flush_timing() {
+  for (auto &p : g_pending) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(p.b);
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    auto &t = g_times[p.name];
+    t.ms += ms;
+    t.count += 1;
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  g_pending.clear();
+}
+
+// Launch helper: optional per-kernel event timing on the library stream.
+template <typename K, typename... Args>
+static int launch(char const *name, K kernel, dim3 grid, dim3 block, size_t lds,
+                  Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return 0;
+  if (g_timing) {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, g_stream);
+  }
+  hipLaunchKernelGGL(kernel, grid, block, lds, g_stream, args...);
+  if (hip_check(hipGetLastError(), name)) return 1;
+  if (g_timing) {
+    (void)hipEventRecord(b, g_stream);
+    g_pending.push_back({name, a, b});
+    if (g_pending.size() > 4096) flush_timing();
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+// roots[m] = exp(-2 pi i m / n); sign > 0 conjugates
+__device__ __forceinline__ float2 rootf(float2 const *roots, int n, long e, int sign) {
+  float2 w = roots[(int)(e % n)];
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+// ---------------------------------------------------------------------------
+// Line geometries for LDS-resident batched FFTs
+// ---------------------------------------------------------------------------
+struct LinesContig {  // line l at l * ld
+  int ld;
+  __device__ __forceinline__ int base(int l) const { return l * ld; }
+};
+
+// lines along dim d of nplanes arrays [ext_0..ext_{nsp-1}] laid out with
+// strides str[] (elements), plane stride pstr; dims other than d are
+// enumerated over their active extents, last dim fastest
+struct LinesND {
+  int nsp, d, pstr, count;
+  int ext[AGD_MAXSP];
+  int str[AGD_MAXSP];
+  __device__ __forceinline__ int base(int l) const {
+    int const p = l / count;
+    int r = l - p * count;
+    int off = p * pstr;
+    for (int e = nsp - 1; e >= 0; --e) {
+      if (e == d) continue;
+      int const c = r % ext[e];
+      r /= ext[e];
+      off += c * str[e];
+    }
+    return off;
+  }
+};
+
+// One Stockham pass (radix R, Ns = product of the radices already applied)
+// over nlines lines of length n; element j of line l at base(l) + j*stride.
+template <class L>
+__device__ void fft_pass(float2 const *__restrict__ src, float2 *__restrict__ dst,
+                         int n, int R, int Ns, int sign, float2 const *roots,
+                         int nlines, int stride, L const &lines) {
+  int const nr = n / R;
+  int const total = nlines * nr;
+  int const tw_step = n / (Ns * R);
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int const l = i / nr;
+    int const j = i - l * nr;
+    int const k = j % Ns;
+    int const b = lines.base(l);
+    int const ob = b + ((j / Ns) * Ns * R + k) * stride;
+    float2 const *in = src + b + j * stride;
+    int const istr = nr * stride;
+    if (R == 2) {
+      float2 v0 = in[0];
+      float2 v1 = in[istr];
+      if (k) v1 = cmul(v1, rootf(roots, n, (long)k * tw_step, sign));
+      dst[ob] = cadd(v0, v1);
+      dst[ob + Ns * stride] = csub(v0, v1);
+    } else if (R == 4) {
+      float2 v0 = in[0];
+      float2 v1 = in[istr];
+      float2 v2 = in[2 * istr];
+      float2 v3 = in[3 * istr];
+      if (k) {
+        v1 = cmul(v1, rootf(roots, n, (long)k * tw_step, sign));
+        v2 = cmul(v2, rootf(roots, n, (long)2 * k * tw_step, sign));
+        v3 = cmul(v3, rootf(roots, n, (long)3 * k * tw_step, sign));
+      }
+      float2 const s02 = cadd(v0, v2), d02 = csub(v0, v2);
+      float2 const s13 = cadd(v1, v3), d13 = csub(v1, v3);
+      // -i*d13 (sign -1) or +i*d13 (sign +1)
+      float2 const j13 = sign < 0 ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
+      dst[ob] = cadd(s02, s13);
+      dst[ob + Ns * stride] = cadd(d02, j13);
+      dst[ob + 2 * Ns * stride] = csub(s02, s13);
+      dst[ob + 3 * Ns * stride] = csub(d02, j13);
+    } else if (R == 3) {
+      float2 v0 = in[0];
+      float2 v1 = in[istr];
+      float2 v2 = in[2 * istr];
+      if (k) {
+        v1 = cmul(v1, rootf(roots, n, (long)k * tw_step, sign));
+        v2 = cmul(v2, rootf(roots, n, (long)2 * k * tw_step, sign));
+      }
+      float2 const t1 = cadd(v1, v2);
+      float2 const t2 = make_float2(v0.x - 0.5f * t1.x, v0.y - 0.5f * t1.y);
+      float const s3 = (sign < 0 ? -1.f : 1.f) * 0.866025403784438647f;
+      float2 const dd = csub(v1, v2);
+      float2 const t3 = make_float2(-s3 * dd.y, s3 * dd.x);  // i*s3*(v1-v2)
+      dst[ob] = cadd(v0, t1);
+      dst[ob + Ns * stride] = cadd(t2, t3);
+      dst[ob + 2 * Ns * stride] = csub(t2, t3);
+    } else {
+      // direct butterfly for any other radix (5, 7, larger primes)
+      int const rstep = n / R;
+      for (int q = 0; q < R; ++q) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int r = 0; r < R; ++r) {
+          float2 v = in[r * istr];
+          long const e = (long)r * k * tw_step + (long)((q * r) % R) * rstep;
+          if (e % n) v = cmul(v, rootf(roots, n, e, sign));
+          acc = cadd(acc, v);
+        }
+        dst[ob + q * Ns * stride] = acc;
+      }
+    }
+  }
+}
+
+// Full FFT along the lines; returns the buffer holding the result. Ends
+// with a barrier.
+template <class L>
+__device__ float2 *fft_lines(float2 *a, float2 *b, AgdFftPlan const &p,
+                             float2 const *roots, int sign, int nlines, int stride,
+                             L const &lines) {
+  int Ns = 1;
+  for (int s = 0; s < p.nstage; ++s) {
+    fft_pass(a, b, p.n, p.radix[s], Ns, sign, roots, nlines, stride, lines);
+    Ns *= p.radix[s];
+    __syncthreads();
+    float2 *t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// Forward spatial transform: X[w][tw][k][S] -> U[w][tw][k][ntr]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHR) void k_fwd_spatial(
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float2 const *__restrict__ X, float2 *__restrict__ U) {
+  extern __shared__ float2 sm[];
+  AgdTask const t = tasks[blockIdx.x];
+  AgdWin const w = wins[t.win];
+  int const S = w.S;
+  int const nk = t.nk;
+  int const nsp = w.nsp;
+  float2 *A = sm;
+  float2 *B = sm + nk * S;
+  float2 const *src = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * S;
+  for (int i = threadIdx.x; i < nk * S; i += blockDim.x) A[i] = src[i];
+  __syncthreads();
+  LinesND g;
+  g.nsp = nsp;
+  g.pstr = S;
+  int st = 1;
+  for (int e = nsp - 1; e >= 0; --e) {
+    g.str[e] = st;
+    st *= w.P[e];
+  }
+  for (int e = 0; e < nsp; ++e) g.ext[e] = w.P[e];
+  for (int d = 0; d < nsp; ++d) {
+    g.d = d;
+    int c = 1;
+    for (int e = 0; e < nsp; ++e)
+      if (e != d) c *= g.ext[e];
+    g.count = c;
+    AgdFftPlan const &pl = plans[w.splan[d]];
+    float2 *res = fft_lines(A, B, pl, roots + pl.roots_off, +1, nk * c, g.str[d], g);
+    if (res != A) {
+      B = A;
+      A = res;
+    }
+    g.ext[d] = w.W[d];
+  }
+  int const ntr = w.ntr;
+  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
+    int const k = i / ntr;
+    int const tr = i - k * ntr;
+    int r = tr, off = 0;
+    for (int e = nsp - 1; e >= 0; --e) {
+      int const c = r % w.W[e];
+      r /= w.W[e];
+      off += c * g.str[e];
+    }
+    U[w.u_off + ((int64_t)t.tw * w.Ft + t.k0 + k) * ntr + tr] = A[k * S + off];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward time transform: U -> traces T (C2R, taper, scale, overlap-add)
+// LDS: H [tb][Ft] + A,B [tb][M] complex + acc [tb][Wt] float
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHR) void k_fwd_time(
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int tb,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float const *__restrict__ tapers, float const *__restrict__ tscale,
+    float2 const *__restrict__ U, float *__restrict__ T) {
+  extern __shared__ float2 sm[];
+  AgdWin const w = wins[blocks[2 * blockIdx.x]];
+  int const t0 = blocks[2 * blockIdx.x + 1];
+  int const nt = min(tb, w.ntr - t0);
+  int const Pt = w.Pt, Ft = w.Ft, Wt = w.Wt, half = w.half;
+  bool const even = (Pt % 2 == 0) && Pt >= 4;
+  int const M = even ? Pt / 2 : Pt;
+  float2 *H = sm;
+  float2 *A = H + tb * Ft;
+  float2 *B = A + tb * M;
+  float *acc = (float *)(B + tb * M);
+  float *nxt = acc + tb * Wt;
+  AgdFftPlan const &pl = plans[w.tplan];
+  float2 const *rt = roots + pl.roots_off;
+  float2 const *rfull = roots + plans[w.tplan_full].roots_off;
+  float const *tt = tapers + w.tt_off;
+  for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) acc[i] = 0.f;
+  LinesContig lc{M};
+  for (int tw = 0; tw < w.ntw; ++tw) {
+    float2 const *u = U + w.u_off + (int64_t)tw * Ft * w.ntr + t0;
+    for (int i = threadIdx.x; i < Ft * tb; i += blockDim.x) {
+      int const k = i / tb;
+      int const j = i - k * tb;
+      H[j * Ft + k] = j < nt ? u[(int64_t)k * w.ntr + j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (even) {
+      // half-length packing of a C2R; DC and Nyquist imaginary parts ignored
+      for (int i = threadIdx.x; i < tb * M; i += blockDim.x) {
+        int const j = i / M;
+        int const k = i - j * M;
+        float2 xk = H[j * Ft + k];
+        float2 xc = H[j * Ft + (M - k)];
+        if (k == 0) {
+          xk.y = 0.f;
+          xc.y = 0.f;
+        }
+        xc.y = -xc.y;
+        float2 const e = cadd(xk, xc);
+        float2 o = csub(xk, xc);
+        o = cmul(o, rootf(rfull, Pt, k, +1));
+        A[i] = make_float2(e.x - o.y, e.y + o.x);
+      }
+    } else {
+      for (int i = threadIdx.x; i < tb * M; i += blockDim.x) {
+        int const j = i / M;
+        int const k = i - j * M;
+        float2 v;
+        if (k == 0) {
+          v = make_float2(H[j * Ft].x, 0.f);
+        } else if (k < Ft) {
+          v = H[j * Ft + k];
+        } else {
+          float2 const c = H[j * Ft + (M - k)];
+          v = make_float2(c.x, -c.y);
+        }
+        A[i] = v;
+      }
+    }
+    __syncthreads();
+    float2 *R = fft_lines(A, B, pl, rt, +1, tb, 1, lc);
+    // accumulate this time window into the overlap buffer
+    for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) {
+      int const j = i / Wt;
+      int const s = i - j * Wt;
+      if (j >= nt) continue;
+      float y;
+      if (even) {
+        float2 const z = R[j * M + (s >> 1)];
+        y = (s & 1) ? z.y : z.x;
+      } else {
+        y = R[j * M + s].x;
+      }
+      acc[i] += y * tscale[w.trace0 + t0 + j] * tt[s];
+    }
+    __syncthreads();
+    // the first `half` samples are complete: write them, and shift the rest
+    // into the other overlap buffer
+    for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) {
+      int const j = i / Wt;
+      int const s = i - j * Wt;
+      if (s < half) {
+        int const t = tw * half + s;
+        if (j < nt && t < w.nconv) T[w.t_off + (int64_t)(t0 + j) * w.nconv + t] = acc[i];
+      }
+      nxt[i] = (s + half < Wt) ? acc[i + half] : 0.f;
+    }
+    __syncthreads();
+    {
+      float *tmp = acc;
+      acc = nxt;
+      nxt = tmp;
+    }
+  }
+  // remaining samples after the last window
+  for (int i = threadIdx.x; i < tb * (Wt - half); i += blockDim.x) {
+    int const j = i / (Wt - half);
+    int const s = i - j * (Wt - half);
+    int const t = w.ntw * half + s;
+    if (j < nt && t < w.nconv) T[w.t_off + (int64_t)(t0 + j) * w.nconv + t] = acc[j * Wt + s];
+  }
+  // zero the capacity beyond the data portion (wavelet input padding)
+  int const tail0 = w.ntw * half + (Wt - half);
+  for (int j = 0; j < nt; ++j) {
+    for (int t = tail0 + threadIdx.x; t < w.nconv; t += blockDim.x)
+      T[w.t_off + (int64_t)(t0 + j) * w.nconv + t] = 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Record gather (deterministic blend). One block per tile of the flat record;
+// entries of a tile are listed in the reference's serial order.
+// modes: 0 sum, 1 mean (taper weights), 2 overwrite, 3 sum - d (residual),
+// 4 per-entry constant sum (stepsize fold)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHR) void k_blend(
+    int mode, int64_t rec_len, int tile, int const *__restrict__ tile_ptr,
+    int const *__restrict__ tile_ent, AgdEntry const *__restrict__ ent,
+    float const *__restrict__ src, float const *__restrict__ cst,
+    float const *__restrict__ taper, int taper_len,
+    unsigned char const *__restrict__ mute, float const *__restrict__ d,
+    float *__restrict__ out) {
+  int const e0 = tile_ptr[blockIdx.x];
+  int const e1 = tile_ptr[blockIdx.x + 1];
+  int64_t const base = (int64_t)blockIdx.x * tile;
+  for (int o = threadIdx.x; o < tile; o += blockDim.x) {
+    int64_t const s = base + o;
+    if (s >= rec_len) break;
+    float acc = 0.f;
+    float cnt = 0.f;
+    for (int q = e0; q < e1; ++q) {
+      int const id = tile_ent[q];
+      AgdEntry const e = ent[id];
+      int64_t const i = s - e.rec;
+      if (i < 0 || i >= e.n) continue;
+      if (mode == 4) {
+        acc += cst[id];
+        continue;
+      }
+      float const v = src[e.src + e.trace_start + i];
+      if (mode == 1) {
+        int const to_end = e.n - 1 - (int)i;
+        int const edge = (int)i < to_end ? (int)i : to_end;
+        float const wgt = (taper_len > 1 && edge < taper_len) ? taper[edge] : 1.f;
+        acc += v * wgt;
+        cnt += wgt;
+      } else if (mode == 2) {
+        acc = v;
+      } else {
+        acc += v;
+      }
+    }
+    if (mode == 1 && cnt > 0.f) acc /= cnt;
+    bool const muted = mute != nullptr && mute[s];
+    if (mode == 3) {
+      out[s] = muted ? 0.f : acc - d[s];
+    } else {
+      out[s] = muted ? 0.f : acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint time transform: record (or T2) -> V (taper, split, R2C, x2)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHR) void k_adj_time(
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int tb,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float const *__restrict__ tapers, float const *__restrict__ tscale,
+    AgdEntry const *__restrict__ went, float const *__restrict__ rec,
+    float const *__restrict__ Tin, float2 *__restrict__ V) {
+  extern __shared__ float2 sm[];
+  AgdWin const w = wins[blocks[2 * blockIdx.x]];
+  int const t0 = blocks[2 * blockIdx.x + 1];
+  int const nt = min(tb, w.ntr - t0);
+  int const Pt = w.Pt, Ft = w.Ft, Wt = w.Wt, half = w.half;
+  bool const even = (Pt % 2 == 0) && Pt >= 4;
+  int const M = even ? Pt / 2 : Pt;
+  float2 *A = sm;
+  float2 *B = A + tb * M;
+  AgdFftPlan const &pl = plans[w.tplan];
+  float2 const *rt = roots + pl.roots_off;
+  float2 const *rfull = roots + plans[w.tplan_full].roots_off;
+  float const *tt = tapers + w.tt_off;
+  LinesContig lc{M};
+  for (int tw = 0; tw < w.ntw; ++tw) {
+    // real input, zero-padded to Pt, packed as complex pairs when even
+    int const nreal = tb * Pt;
+    for (int i = threadIdx.x; i < nreal; i += blockDim.x) {
+      int const j = i / Pt;
+      int const s = i - j * Pt;
+      float v = 0.f;
+      if (j < nt && s < Wt) {
+        int const g = w.trace0 + t0 + j;
+        int const t = tw * half + s;
+        float x;
+        if (Tin != nullptr) {
+          x = Tin[w.t_off + (int64_t)(t0 + j) * w.nconv + t];
+        } else {
+          AgdEntry const e = went[g];
+          int const i2 = t - e.trace_start;
+          x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+        }
+        v = x * tscale[g] * tt[s];
+      }
+      if (even) {
+        float *Af = (float *)A;
+        Af[j * Pt + s] = v;  // (z[2m], z[2m+1]) pairs
+      } else {
+        A[i] = make_float2(v, 0.f);
+      }
+    }
+    __syncthreads();
+    float2 *R = fft_lines(A, B, pl, rt, -1, tb, 1, lc);
+    float2 *out = (R == A) ? B : A;  // free buffer for the spectrum staging
+    // post-process into the half spectrum and store (x2 on interior bins)
+    int const kmax2 = (Pt - 1) / 2;
+    for (int i = threadIdx.x; i < Ft * tb; i += blockDim.x) {
+      int const k = i / tb;
+      int const j = i - k * tb;
+      if (j >= nt) continue;
+      float2 X;
+      if (even) {
+        float2 const zk = R[j * M + (k % M)];
+        float2 zc = R[j * M + ((M - k) % M)];
+        zc.y = -zc.y;
+        float2 const e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+        float2 const o = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+        X = cadd(e, cmul(o, rootf(rfull, Pt, k, -1)));
+        if (k == 0 || k == M) X.y = 0.f;
+      } else {
+        X = R[j * M + k];
+        if (k == 0) X.y = 0.f;
+      }
+      if (k >= 1 && k <= kmax2) {
+        X.x *= 2.f;
+        X.y *= 2.f;
+      }
+      V[w.u_off + ((int64_t)tw * Ft + k) * w.ntr + t0 + j] = X;
+    }
+    (void)out;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint spatial transform fused with the model update and shrink
+// mode 0: X -= alpha*G, shrink(lamb); 1: X = -alpha*G, max |X|^2;
+// mode 2: X -= alpha*G; 3: G only (X = G, used by operator tests)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHR) void k_adj_spatial(
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float2 const *__restrict__ V, float2 *__restrict__ X, float alpha,
+    float lamb, int mode, unsigned int *__restrict__ maxbits) {
+  extern __shared__ float2 sm[];
+  AgdTask const t = tasks[blockIdx.x];
+  AgdWin const w = wins[t.win];
+  int const S = w.S;
+  int const nk = t.nk;
+  int const nsp = w.nsp;
+  int const ntr = w.ntr;
+  float2 *A = sm;
+  float2 *B = sm + nk * S;
+  LinesND g;
+  g.nsp = nsp;
+  g.pstr = S;
+  int st = 1;
+  for (int e = nsp - 1; e >= 0; --e) {
+    g.str[e] = st;
+    st *= w.P[e];
+  }
+  for (int i = threadIdx.x; i < nk * S; i += blockDim.x) A[i] = make_float2(0.f, 0.f);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
+    int const k = i / ntr;
+    int const tr = i - k * ntr;
+    int r = tr, off = 0;
+    for (int e = nsp - 1; e >= 0; --e) {
+      int const c = r % w.W[e];
+      r /= w.W[e];
+      off += c * g.str[e];
+    }
+    A[k * S + off] = V[w.u_off + ((int64_t)t.tw * w.Ft + t.k0 + k) * ntr + tr];
+  }
+  __syncthreads();
+  for (int e = 0; e < nsp; ++e) g.ext[e] = w.W[e];
+  for (int d = nsp - 1; d >= 0; --d) {
+    g.d = d;
+    int c = 1;
+    for (int e = 0; e < nsp; ++e)
+      if (e != d) c *= g.ext[e];
+    g.count = c;
+    AgdFftPlan const &pl = plans[w.splan[d]];
+    float2 *res = fft_lines(A, B, pl, roots + pl.roots_off, -1, nk * c, g.str[d], g);
+    if (res != A) {
+      B = A;
+      A = res;
+    }
+    g.ext[d] = w.P[d];
+  }
+  float2 *x = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * S;
+  float const l2 = lamb * lamb;
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < nk * S; i += blockDim.x) {
+    float2 const gv = A[i];
+    float2 v;
+    if (mode == 3) {
+      v = gv;
+    } else if (mode == 1) {
+      v = make_float2(0.f - alpha * gv.x, 0.f - alpha * gv.y);
+      mx = fmaxf(mx, v.x * v.x + v.y * v.y);
+    } else {
+      v = x[i];
+      v.x -= alpha * gv.x;
+      v.y -= alpha * gv.y;
+      if (mode == 0) {
+        float const a = v.x * v.x + v.y * v.y;
+        if (a <= l2) {
+          v = make_float2(0.f, 0.f);
+        } else {
+          float const ab = sqrtf(a);
+          float const sh = (ab - lamb) / ab;
+          v.x *= sh;
+          v.y *= sh;
+        }
+      }
+    }
+    x[i] = v;
+  }
+  if (mode == 1) {
+    // block max, then one atomic per block (max is order independent)
+    __shared__ float red[NTHR / 64];
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmaxf(m, red[q]);
+      atomicMax(maxbits, __float_as_uint(m));
+    }
+  }
+}
+
+__global__ void k_shrink(float2 *__restrict__ X, int64_t n, float lamb) {
+  float const l2 = lamb * lamb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float2 v = X[i];
+    float const a = v.x * v.x + v.y * v.y;
+    if (a <= l2) {
+      v = make_float2(0.f, 0.f);
+    } else {
+      float const ab = sqrtf(a);
+      float const sh = (ab - lamb) / ab;
+      v.x *= sh;
+      v.y *= sh;
+    }
+    X[i] = v;
+  }
+}
+
+__global__ void k_max_bits(float const *__restrict__ v, int64_t n,
+                           unsigned int *__restrict__ out) {
+  float m = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, v[i]);
+  __shared__ float red[NTHR / 64];
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float mm = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) mm = fmaxf(mm, red[q]);
+    atomicMax(out, __float_as_uint(mm));
+  }
+}
+
+// FP64 sum of squares: fixed-size partials (deterministic), summed on host
+__global__ void k_norm2(float const *__restrict__ v, int64_t n,
+                        double *__restrict__ partials) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double const x = (double)v[i];
+    acc += x * x;
+  }
+  __shared__ double red[NTHR];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Source-wavelet convolution. One block per window trace.
+// The reference applies C2R_N(R2C_N(x) * W) with N = nconv (wavelet.c:239);
+// for the live traces this is the circular convolution of length N of x with
+// h = C2R_N(W). It is evaluated with an FFT length M >= nconv + nfk - 1
+// (7-smooth) against the precomputed spectrum Hm of h laid out for lags in
+// (-nfk, N): forward y[t] = (IFFT_M(FFT_M(x) Hm))[t] for t < N (x supported
+// on [0, nfk)), adjoint x[t] = (IFFT_M(FFT_M(y) conj(Hm)))[t] for t < nfk.
+// Hm includes the 1/M normalisation. Non-live traces are transformed but not
+// multiplied in the reference, which scales them by N (wavelet.c:273).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHR) void k_wavelet(
+    AgdWin const *__restrict__ wins, int const *__restrict__ trace_win,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float2 const *__restrict__ wspec, int const *__restrict__ wsrc_off,
+    int adjoint, AgdEntry const *__restrict__ went, float const *__restrict__ rec,
+    float *__restrict__ T, float *__restrict__ T2) {
+  extern __shared__ float2 sm[];
+  int const g = blockIdx.x;
+  AgdWin const w = wins[trace_win[g]];
+  int const tr = g - w.trace0;
+  int const N = w.nconv;
+  int const nfk = w.nfk;
+  int const Mr = w.wM;  // real length
+  int const M = Mr / 2;
+  float *Tio = T + w.t_off + (int64_t)tr * N;
+  float *Tout = T2 + w.t_off + (int64_t)tr * N;
+  int const so = wsrc_off[g];
+  AgdEntry const e = went[g];
+  // input sample accessor
+  int const nin = adjoint ? N : nfk;
+  if (so < 0) {
+    // not multiplied: C2R(R2C(x)) = N x
+    float const fN = (float)N;
+    if (!adjoint) {
+      for (int t = threadIdx.x; t < N; t += blockDim.x) Tio[t] = t < nfk ? Tio[t] * fN : 0.f;
+    } else {
+      for (int t = threadIdx.x; t < N; t += blockDim.x) {
+        int const i2 = t - e.trace_start;
+        float const x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+        Tout[t] = t < nfk ? x * fN : 0.f;
+      }
+    }
+    return;
+  }
+  float2 *A = sm;
+  float2 *B = A + M;
+  float2 *Hs = B + M;  // half spectrum M+1
+  float *Af = (float *)A;
+  for (int t = threadIdx.x; t < Mr; t += blockDim.x) {
+    float x = 0.f;
+    if (t < nin) {
+      if (!adjoint) {
+        x = Tio[t];
+      } else {
+        int const i2 = t - e.trace_start;
+        x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+      }
+    }
+    Af[t] = x;
+  }
+  __syncthreads();
+  AgdFftPlan const &pl = plans[w.wplan];
+  float2 const *rt = roots + pl.roots_off;
+  float2 const *rfull = roots + plans[w.wplan_full].roots_off;
+  LinesContig lc{M};
+  float2 *R = fft_lines(A, B, pl, rt, -1, 1, 1, lc);
+  float2 const *hs = wspec + so;
+  for (int k = threadIdx.x; k <= M; k += blockDim.x) {
+    float2 const zk = R[k % M];
+    float2 zc = R[(M - k) % M];
+    zc.y = -zc.y;
+    float2 const ev = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    float2 const o = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+    float2 X = cadd(ev, cmul(o, rootf(rfull, Mr, k, -1)));
+    if (k == 0 || k == M) X.y = 0.f;
+    float2 h = hs[k];
+    if (adjoint) h.y = -h.y;
+    Hs[k] = cmul(X, h);
+  }
+  __syncthreads();
+  float2 *Z = (R == A) ? B : A;
+  for (int k = threadIdx.x; k < M; k += blockDim.x) {
+    float2 xk = Hs[k];
+    float2 xc = Hs[M - k];
+    if (k == 0) {
+      xk.y = 0.f;
+      xc.y = 0.f;
+    }
+    xc.y = -xc.y;
+    float2 const ev = cadd(xk, xc);
+    float2 o = csub(xk, xc);
+    o = cmul(o, rootf(rfull, Mr, k, +1));
+    Z[k] = make_float2(ev.x - o.y, ev.y + o.x);
+  }
+  __syncthreads();
+  float2 *Y = fft_lines(Z, (Z == A) ? B : A, pl, rt, +1, 1, 1, lc);
+  float const *Yf = (float const *)Y;
+  if (!adjoint) {
+    for (int t = threadIdx.x; t < N; t += blockDim.x) Tio[t] = Yf[t];
+  } else {
+    for (int t = threadIdx.x; t < N; t += blockDim.x) Tout[t] = t < nfk ? Yf[t] : 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Output assembly and blend extraction
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NTHR) void k_output(
+    float const *__restrict__ T, int64_t const *__restrict__ o_off,
+    int const *__restrict__ o_nt, int const *__restrict__ o_ptr,
+    int64_t const *__restrict__ o_src, float *__restrict__ out) {
+  int const i = blockIdx.x;
+  int const c0 = o_ptr[i], c1 = o_ptr[i + 1];
+  int const nt = o_nt[i];
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    float acc = 0.f;
+    for (int c = c0; c < c1; ++c) acc += T[o_src[c] + t];
+    out[o_off[i] + t] = acc;
+  }
+}
+
+__global__ __launch_bounds__(NTHR) void k_extract(
+    AgdEntry const *__restrict__ ent, int const *__restrict__ trace_len,
+    float const *__restrict__ rec, float *__restrict__ out) {
+  int const i = blockIdx.x;
+  AgdEntry const e = ent[i];
+  int const L = trace_len[i];
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    int const i2 = t - e.trace_start;
+    out[e.src + t] = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int agdg_init(void) {
+  if (g_stream != nullptr) return 0;
+  int n = 0;
+  if (hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount")) return 1;
+  if (n < 1) {
+    fprintf(stderr, "ERROR: no HIP device available\n");
+    return 1;
+  }
+  if (hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking),
+                "hipStreamCreate"))
+    return 1;
+  // allow dynamic LDS up to the CU's 160 KB for the LDS-resident transforms
+  {
+    int const lds = 160 * 1024;
+    (void)hipFuncSetAttribute((void const *)k_fwd_spatial,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((void const *)k_adj_spatial,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((void const *)k_fwd_time,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((void const *)k_adj_time,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((void const *)k_wavelet,
+                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipGetLastError();
+  }
+  return 0;
+}
+
+void *agdg_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    fprintf(stderr, "ERROR: hipMalloc of %zu bytes failed\n", bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+void agdg_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+int agdg_h2d(void *dst, void const *src, size_t bytes) {
+  if (bytes == 0) return 0;
+  return hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream),
+                   "memcpy H2D") ||
+         hip_check(hipStreamSynchronize(g_stream), "sync");
+}
+
+int agdg_d2h(void *dst, void const *src, size_t bytes) {
+  if (bytes == 0) return 0;
+  return hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream),
+                   "memcpy D2H") ||
+         hip_check(hipStreamSynchronize(g_stream), "sync");
+}
+
+int agdg_zero(void *dst, size_t bytes) {
+  if (bytes == 0) return 0;
+  return hip_check(hipMemsetAsync(dst, 0, bytes, g_stream), "memset");
+}
+
+int agdg_sync(void) {
+  int rc = hip_check(hipStreamSynchronize(g_stream), "stream sync");
+  if (g_timing) flush_timing();
+  return rc;
+}
+
+void *agdg_stream(void) { return (void *)g_stream; }
+
+int agdg_fwd_spatial(AgdDev const *g, float const *X) {
+  return launch("fwd_spatial", k_fwd_spatial, dim3(g->n_tasks), dim3(NTHR),
+                g->lds_spatial, g->win, g->tasks, g->plans, (float2 const *)g->roots,
+                (float2 const *)X, (float2 *)g->U);
+}
+
+int agdg_fwd_time(AgdDev const *g) {
+  return launch("fwd_time", k_fwd_time, dim3(g->n_tblocks), dim3(NTHR), g->lds_time,
+                g->win, g->tblocks, g->tb, g->plans, (float2 const *)g->roots,
+                g->tapers, g->tscale, (float2 const *)g->U, g->T);
+}
+
+int agdg_blend_traces(AgdDev const *g, int resid) {
+  return launch("blend", k_blend, dim3(g->n_tiles), dim3(NTHR), 0, resid ? 3 : 0,
+                g->rec_len, g->tile, g->tile_ptr, g->tile_ent, g->went,
+                (float const *)g->T, (float const *)nullptr, (float const *)nullptr, 0,
+                g->mute, (float const *)g->d, g->r);
+}
+
+int agdg_stepsize_fold(AgdDev const *g) {
+  if (launch("fold", k_blend, dim3(g->n_tiles), dim3(NTHR), 0, 4, g->rec_len, g->tile,
+             g->tile_ptr, g->tile_ent, g->went, (float const *)nullptr, g->tconst,
+             (float const *)nullptr, 0, (unsigned char const *)nullptr,
+             (float const *)nullptr, g->r))
+    return 1;
+  int blocks = (int)((g->rec_len + NTHR - 1) / NTHR);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  return launch("max", k_max_bits, dim3(blocks), dim3(NTHR), 0, (float const *)g->r,
+                g->rec_len, g->scalar + 1);
+}
+
+int agdg_adj_time(AgdDev const *g, int from_record) {
+  return launch("adj_time", k_adj_time, dim3(g->n_tblocks), dim3(NTHR), g->lds_time,
+                g->win, g->tblocks, g->tb, g->plans, (float2 const *)g->roots,
+                g->tapers, g->tscale, g->went, (float const *)g->r,
+                from_record ? (float const *)nullptr : (float const *)g->T2,
+                (float2 *)g->U);
+}
+
+int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode) {
+  return launch("adj_spatial", k_adj_spatial, dim3(g->n_tasks), dim3(NTHR),
+                g->lds_spatial, g->win, g->tasks, g->plans, (float2 const *)g->roots,
+                (float2 const *)g->U, (float2 *)g->X, alpha, lamb, mode, g->scalar);
+}
+
+int agdg_shrink(AgdDev const *g, int64_t n_complex, float lamb) {
+  int64_t blocks = (n_complex + NTHR - 1) / NTHR;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  return launch("shrink", k_shrink, dim3((unsigned)blocks), dim3(NTHR), 0,
+                (float2 *)g->X, n_complex, lamb);
+}
+
+int agdg_wavelet(AgdDev const *g, int adjoint) {
+  if (!g->use_wavelets) return 0;
+  return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet, dim3(g->n_wtr),
+                dim3(NTHR), g->lds_wavelet, g->win, g->trace_win,
+                g->plans, (float2 const *)g->roots, (float2 const *)g->wspec,
+                g->wsrc_off, adjoint, g->went, (float const *)g->r, g->T, g->T2);
+}
+
+int agdg_norm2(AgdDev const *g, float const *v, int64_t n, double *out) {
+  int blocks = g->n_partials;
+  std::vector<double> h(blocks);
+  if (launch("norm2", k_norm2, dim3(blocks), dim3(NTHR), 0, v, n, g->partials)) return 1;
+  if (hip_check(hipMemcpyAsync(h.data(), g->partials, blocks * sizeof(double),
+                               hipMemcpyDeviceToHost, g_stream),
+                "memcpy") ||
+      hip_check(hipStreamSynchronize(g_stream), "sync"))
+    return 1;
+  double s = 0.0;
+  for (int i = 0; i < blocks; ++i) s += h[i];
+  *out = s;
+  return 0;
+}
+
+int agdg_output(float const *T, int n_out, int64_t const *o_off, int const *o_nt,
+                int const *o_ptr, int64_t const *o_src, float *out) {
+  return launch("output", k_output, dim3(n_out), dim3(NTHR), 0, T, o_off, o_nt, o_ptr,
+                o_src, out);
+}
+
+int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec) {
+  return launch("gather", k_blend, dim3(gt->n_tiles), dim3(NTHR), 0, mode, gt->rec_len,
+                gt->tile, gt->tile_ptr, gt->tile_ent, gt->ent, src,
+                (float const *)nullptr, gt->taper, gt->taper_len, gt->mute,
+                (float const *)nullptr, rec);
+}
+
+int agdg_extract(AgdEntry const *ent, int n_ent, int const *trace_len, float const *rec,
+                 float *out) {
+  return launch("extract", k_extract, dim3(n_ent), dim3(NTHR), 0, ent, trace_len, rec,
+                out);
+}
+
+int agdg_timing(int enable) {
+  if (!enable) {
+    (void)hipStreamSynchronize(g_stream);
+    flush_timing();
+  }
+  g_timing = enable;
+  if (enable) g_times.clear();
+  return 0;
+}
+
+int agdg_timing_get(char const *name, double *ms, long *count) {
+  (void)hipStreamSynchronize(g_stream);
+  flush_timing();
+  auto it = g_times.find(name);
+  if (it == g_times.end()) {
+    *ms = 0.0;
+    *count = 0;
+    return 1;
+  }
+  *ms = it->second.ms;
+  *count = it->second.count;
+  return 0;
+}
+
+}  // extern "C"

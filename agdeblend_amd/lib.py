@@ -1,0 +1,242 @@
+"""Loader and ctypes bindings for libagdeblend.so (the HIP build).
+
+The library is built in-tree (agdeblend_amd/lib/libagdeblend.so) by
+agdeblend_amd.build. Loading fails loudly when it is missing: there is no CPU
+fallback for the product path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from ._marshal import C_INT, C_LONG, PtrTable, int_array, null
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "lib", "libagdeblend.so")
+
+_lib = None
+
+
+def load():
+    """Load libagdeblend.so and declare its entry points."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(
+            "libagdeblend.so is not built (%s); run agdeblend_amd.build.build()" % LIBPATH)
+    L = ctypes.CDLL(LIBPATH)
+    vp, ci, cf = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+
+    def fn(name, res, *args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    geo = [ci] + [vp] * 11
+    fn("agd_deblend", ci, *geo, cf, ci, ci, vp)
+    fn("agd_blend", ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, ci, vp, vp, vp, vp, vp, vp)
+    fn("agdeblend_deblend", ci, *geo, cf, ci, ci, vp)
+    fn("agdeblend_blend", ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, ci, vp, vp, vp, vp, vp, vp)
+    fn("agdx_create", vp, *geo)
+    fn("agdx_create_host", vp, *geo)
+    fn("agdx_destroy", None, vp)
+    fn("agdx_model_len", ctypes.c_long, vp)
+    fn("agdx_record_len", ctypes.c_long, vp)
+    fn("agdx_n_windows", ci, vp)
+    fn("agdx_window_info", ci, vp, ci, vp)
+    fn("agdx_window_blend", ci, vp, ci, vp)
+    fn("agdx_intervals", ci, vp, vp)
+    fn("agdx_mutes", ci, vp, vp)
+    fn("agdx_unique_channels", ci, vp, vp)
+    fn("agdx_stepsize", cf, vp)
+    fn("agdx_set_data", ci, vp, vp, vp)
+    fn("agdx_forward", ci, vp, vp, vp)
+    fn("agdx_adjoint", ci, vp, vp, vp)
+    fn("agdx_run", ci, vp, cf, ci, ci, vp)
+    fn("agdx_get_output", ci, vp, vp)
+    fn("agdx_get_model", ci, vp, vp)
+    fn("agdx_iterate", ci, vp, ci, cf)
+    fn("agdx_last_lambda0", cf)
+    fn("agdx_timing", ci, ci)
+    fn("agdx_timing_get", ci, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+       ctypes.POINTER(ctypes.c_long))
+    fn("agdx_bytes", ctypes.c_double, vp, ci)
+    _lib = L
+    return L
+
+
+EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
+           "agdx_create", "agdx_create_host", "agdx_destroy", "agdx_model_len",
+           "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
+           "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
+           "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_get_output",
+           "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
+           "agdx_timing_get", "agdx_bytes"]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Geometry:
+    """Marshalled geometry arguments of agd_deblend (everything but data)."""
+
+    def __init__(self, volumes, n_dims, window_shapes, coords, shapes, shottimes,
+                 channels, trace_types, wavelet_lengths=None, wavelet_idxs=None,
+                 wavelets=None):
+        self.n_patches = len(volumes)
+        self.volumes, self.volumes_p = int_array(volumes)
+        self.n_dims, self.n_dims_p = int_array(n_dims)
+        self.window_shapes = PtrTable(window_shapes, C_INT)
+        self.coords = PtrTable([np.asarray(c).reshape(-1) if np.size(c) else np.zeros(1)
+                                for c in coords], C_INT)
+        self.shapes = PtrTable(shapes, C_INT)
+        self.shottimes = PtrTable([np.asarray(s).reshape(-1) for s in shottimes], C_LONG)
+        self.channels = PtrTable([np.asarray(c).reshape(-1) for c in channels], C_INT)
+        self.trace_types = PtrTable([np.asarray(t).reshape(-1) for t in trace_types], C_INT)
+        if wavelet_idxs is not None:
+            self.wl, self.wl_p = int_array(wavelet_lengths)
+            self.widx = PtrTable([np.asarray(w).reshape(-1) for w in wavelet_idxs], C_INT)
+            self.wav = PtrTable(wavelets, np.float32)
+            self.widx_p, self.wav_p = self.widx.ptr, self.wav.ptr
+        else:
+            self.wl_p, self.widx_p, self.wav_p = null(), null(), null()
+
+    def args(self):
+        return (self.n_patches, self.volumes_p, self.n_dims_p, self.window_shapes.ptr,
+                self.coords.ptr, self.shapes.ptr, self.shottimes.ptr, self.channels.ptr,
+                self.trace_types.ptr, self.wl_p, self.widx_p, self.wav_p)
+
+
+class Problem:
+    """A deblending problem resident on the GPU (agdx_* interface)."""
+
+    def __init__(self, volumes, n_dims, window_shapes, coords, shapes, shottimes,
+                 channels, trace_types, wavelet_lengths=None, wavelet_idxs=None,
+                 wavelets=None, host_only=False):
+        self.L = load()
+        self.geo = Geometry(volumes, n_dims, window_shapes, coords, shapes, shottimes,
+                            channels, trace_types, wavelet_lengths, wavelet_idxs, wavelets)
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        create = self.L.agdx_create_host if host_only else self.L.agdx_create
+        self.h = create(*self.geo.args())
+        if not self.h:
+            raise RuntimeError("agdx_create failed")
+        self.host_only = host_only
+        self.model_len = self.L.agdx_model_len(self.h)
+        self.record_len = self.L.agdx_record_len(self.h)
+        self.n_windows = self.L.agdx_n_windows(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.agdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def window(self, w):
+        info = np.zeros(24, C_INT)
+        if self.L.agdx_window_info(self.h, w, _p(info)):
+            raise IndexError(w)
+        nsp = int(info[1])
+        return dict(patch=int(info[0]), nsp=nsp, ntr=int(info[2]), ntw=int(info[3]),
+                    nfk=int(info[4]), nconv=int(info[5]), Pt=int(info[6]), Ft=int(info[7]),
+                    Wt=int(info[8]), S=int(info[9]),
+                    x_off=int(info[10]) + (int(info[11]) << 31),
+                    P=[int(v) for v in info[12:12 + nsp]],
+                    W=[int(v) for v in info[16:16 + nsp]],
+                    coords=[int(v) for v in info[20:20 + nsp]])
+
+    def blend_coords(self, w):
+        n = self.window(w)["ntr"]
+        out = np.zeros((n, 5), np.int64)
+        self.L.agdx_window_blend(self.h, w, _p(out))
+        return out
+
+    def intervals(self):
+        n = self.L.agdx_intervals(self.h, null())
+        out = np.zeros((max(n, 1), 3), np.int64)
+        self.L.agdx_intervals(self.h, _p(out))
+        return out[:n]
+
+    def mutes(self):
+        n = self.L.agdx_mutes(self.h, null())
+        out = np.zeros((max(n, 1), 4), np.int64)
+        self.L.agdx_mutes(self.h, _p(out))
+        return out[:n]
+
+    def unique_channels(self):
+        n = self.L.agdx_unique_channels(self.h, null())
+        out = np.zeros(max(n, 1), C_INT)
+        self.L.agdx_unique_channels(self.h, _p(out))
+        return out[:n]
+
+    def stepsize(self):
+        return float(self.L.agdx_stepsize(self.h))
+
+    def set_data(self, data, want_record=False):
+        d = PtrTable([np.asarray(a, np.float32).reshape(-1) for a in data], np.float32)
+        rec = np.zeros(max(1, self.record_len), np.float32) if want_record else None
+        if self.L.agdx_set_data(self.h, d.ptr, _p(rec) if want_record else null()):
+            raise RuntimeError("agdx_set_data failed")
+        return rec[: self.record_len] if want_record else None
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.complex64)
+        rec = np.zeros(max(1, self.record_len), np.float32)
+        if self.L.agdx_forward(self.h, _p(x), _p(rec)):
+            raise RuntimeError("agdx_forward failed")
+        return rec[: self.record_len]
+
+    def adjoint(self, rec):
+        rec = np.ascontiguousarray(rec, np.float32)
+        g = np.zeros(max(1, self.model_len), np.complex64)
+        if self.L.agdx_adjoint(self.h, _p(rec), _p(g)):
+            raise RuntimeError("agdx_adjoint failed")
+        return g[: self.model_len]
+
+    def run(self, initial_factor, n_its, print_freq=0, x0=None):
+        x0p = null()
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, np.complex64)
+            x0p = _p(x0)
+        if self.L.agdx_run(self.h, float(initial_factor), int(n_its), int(print_freq), x0p):
+            raise RuntimeError("agdx_run failed")
+
+    def iterate(self, n, lamb):
+        if self.L.agdx_iterate(self.h, int(n), float(lamb)):
+            raise RuntimeError("agdx_iterate failed")
+
+    def output(self):
+        out = PtrTable([np.zeros(s, np.float32).reshape(-1) for s in self.shapes], np.float32,
+                       writable=True)
+        if self.L.agdx_get_output(self.h, out.ptr):
+            raise RuntimeError("agdx_get_output failed")
+        return [o.reshape(s) for o, s in zip(out.np, self.shapes)]
+
+    def model(self):
+        x = np.zeros(max(1, self.model_len), np.complex64)
+        if self.L.agdx_get_model(self.h, _p(x)):
+            raise RuntimeError("agdx_get_model failed")
+        return x[: self.model_len]
+
+    def bytes(self):
+        return dict(M=self.L.agdx_bytes(self.h, 0), T=self.L.agdx_bytes(self.h, 1),
+                    R=self.L.agdx_bytes(self.h, 2))
+
+
+def timing(enable):
+    load().agdx_timing(1 if enable else 0)
+
+
+def timing_get(name):
+    ms = ctypes.c_double(0)
+    n = ctypes.c_long(0)
+    load().agdx_timing_get(name.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value
+
+
+def last_lambda0():
+    return float(load().agdx_last_lambda0())

@@ -1,0 +1,109 @@
+/*
+ * Lower-level C interface of libagdeblend.so: a deblending problem that
+ * stays resident on the GPU between calls.
+ *
+ * agd_deblend (agdeblend.h) is exactly agdx_create + agdx_set_data +
+ * agdx_run + agdx_get_output + agdx_destroy. The split form exists for
+ * callers that reuse one geometry (benchmarks, warm starts computed by the
+ * Python training side, operator tests). It has no counterpart in the
+ * reference, whose internals are all static (agdeblend.c:241-245).
+ *
+ * Model vectors are complex float, interleaved (re, im), in the library's
+ * frequency-major window layout: window w, time window tw, time frequency
+ * k, then the padded space coefficients (row-major, last space dim
+ * fastest). Records are the concatenated (channel, interval) records in
+ * channel-then-interval order. Return codes: 0 success, 1 failure.
+ */
+#ifndef AGDEBLEND_GPU_H
+#define AGDEBLEND_GPU_H
+
+#include "agdeblend.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct AgdProblem AgdProblem;
+
+/* Validates the inputs like agd_deblend and builds the device tables.
+ * Returns NULL on failure. */
+AgdProblem *agdx_create(int n_patches, int const *volumes, int const *n_dims,
+                        int const *const *window_shapes,
+                        int const *const *coords, int const *const *shapes,
+                        long int const *const *shottimes,
+                        int const *const *channels,
+                        enum AGDTraceType const *const *trace_types,
+                        int const *wavelet_lengths,
+                        int const *const *wavelet_idxs,
+                        float const *const *wavelets);
+void agdx_destroy(AgdProblem *p);
+/* Same tables without touching the GPU (geometry and blend tables only;
+ * the operators, stepsize and runs are unavailable on such a problem). */
+AgdProblem *agdx_create_host(int n_patches, int const *volumes,
+                             int const *n_dims,
+                             int const *const *window_shapes,
+                             int const *const *coords,
+                             int const *const *shapes,
+                             long int const *const *shottimes,
+                             int const *const *channels,
+                             enum AGDTraceType const *const *trace_types,
+                             int const *wavelet_lengths,
+                             int const *const *wavelet_idxs,
+                             float const *const *wavelets);
+/* record intervals (channel, start, stop), mutes (channel, interval,
+ * interval_start, n) and unique channel values; each returns its count and
+ * writes only when out != NULL */
+int agdx_intervals(AgdProblem const *p, long long *out);
+int agdx_mutes(AgdProblem const *p, long long *out);
+int agdx_unique_channels(AgdProblem const *p, int *out);
+
+long agdx_model_len(AgdProblem const *p);  /* complex elements */
+long agdx_record_len(AgdProblem const *p); /* floats */
+int agdx_n_windows(AgdProblem const *p);
+/* info[24]: patch, nsp, ntr, ntw, nfk, nconv, Pt, Ft, Wt, S, x_off(lo),
+ * x_off(hi), P[4], W[4], coords[4] */
+int agdx_window_info(AgdProblem const *p, int w, int *info);
+/* per window trace: channel, interval, interval_start, trace_start, n */
+int agdx_window_blend(AgdProblem const *p, int w, long long *out);
+/* stepsize computed on the device (0.5 / max(1, fold)) */
+float agdx_stepsize(AgdProblem const *p);
+
+/* Blends the patch data (mean of duplicates, then mute) into the device
+ * record d: the pseudo-deblended input of the inversion
+ * (blend.c:1294-1335). Optionally copies d to host (record_len floats). */
+int agdx_set_data(AgdProblem *p, float const *const *data, float *d_out);
+
+/* Operators on host buffers (tests): rec = A x (forward, muted record) and
+ * g = A^H rec (model), without update. */
+int agdx_forward(AgdProblem *p, float const *x, float *rec);
+int agdx_adjoint(AgdProblem *p, float const *rec, float *g);
+
+/* Full inversion from the zero start, or from x0 (model_len complex) when
+ * x0 != NULL (warm start: the initial update is skipped and the schedule
+ * starts from a threshold lambda0 computed from the first adjoint).
+ * Leaves the final model on the device. */
+int agdx_run(AgdProblem *p, float initial_factor, int n_its, int print_freq,
+             float const *x0);
+/* Writes the forward of the current model into the patch data
+ * (set_output, model.c:1425-1515). */
+int agdx_get_output(AgdProblem *p, float *const *data);
+/* Copies the current model to host (model_len complex). */
+int agdx_get_model(AgdProblem *p, float *x);
+
+/* Benchmark hooks: run n ISTA iterations from the current device state with
+ * a fixed threshold, synchronising at the end. */
+int agdx_iterate(AgdProblem *p, int n_iterations, float lamb);
+/* Threshold lambda0 of the last agdx_run (first update's max |x|). */
+float agdx_last_lambda0(void);
+/* Per-kernel device times accumulated while timing is enabled. */
+int agdx_timing(int enable);
+int agdx_timing_get(char const *kernel, double *ms, long *count);
+/* Bytes each FK kernel is charged per launch (algorithmic, for rooflines):
+ * which = 0 model bytes M, 1 trace bytes T, 2 record bytes R */
+double agdx_bytes(AgdProblem const *p, int which);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

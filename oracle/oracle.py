@@ -462,3 +462,18 @@ class BlendConfig:
         out = np.zeros(max(1, self.n_traces[p] * self.n_times[p]), self.dt)
         self.L.blendcfg_adjoint(self.h, _p(r), p, _p(out))
         return out[: self.n_traces[p] * self.n_times[p]].reshape(self.n_traces[p], self.n_times[p])
+
+
+def blend_same_geometry(shottimes, channels, trace_types, data, blend_mode, taper_length=0,
+                        trace_types_out=None, double=False):
+    """agd_blend with the output geometry equal to the input, shaped like the
+    Python wrapper's blend(shottimes, channels, trace_types, data, mode)."""
+    shapes = [np.shape(d) for d in data]
+    n_traces = [int(np.prod(s[:-1])) for s in shapes]
+    n_times = [int(s[-1]) for s in shapes]
+    if trace_types_out is None:
+        trace_types_out = [np.where(np.asarray(t) == 1, 0, np.asarray(t)) for t in trace_types]
+    out = blend(n_traces, n_times, shottimes, channels, trace_types,
+                [np.asarray(d).reshape(-1) for d in data], blend_mode, taper_length,
+                n_traces, n_times, shottimes, channels, trace_types_out, double=double)
+    return [o.reshape(s) for o, s in zip(out, shapes)]

@@ -1,0 +1,214 @@
+"""GPU parity of the HIP path against the oracle (run on an MI355X).
+
+Bars (written in each test):
+  * forward / adjoint operators vs the FP64 oracle: relative error <= 2e-6
+  * operator adjointness in FP64 accumulation: |<Ax,y> - <x,A^H y>| <= 1e-6
+    relative
+  * agd_deblend after a fixed number of iterations vs the FP64 oracle:
+    relative L2 <= 1e-4
+  * agd_blend sum / overwrite / untapered mean: bit-exact vs the float
+    oracle; tapered mean: <= 2 ulp relative
+  * integer tables: bit-exact (also covered on the CPU by test_host_tables)
+"""
+import numpy as np
+import pytest
+
+from agdeblend_amd import agdeblend as agd
+from agdeblend_amd import lib as L
+from agdeblend_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LIVE, BAD, MISSING = 0, 1, 2
+
+
+def _kw(g):
+    keys = ["volumes", "n_dims", "window_shapes", "coords", "shapes", "shottimes",
+            "channels", "trace_types"]
+    kw = {k: g[k] for k in keys}
+    for k in ("wavelet_lengths", "wavelet_idxs", "wavelets"):
+        if k in g:
+            kw[k] = g[k]
+    return kw
+
+
+def _to_product_layout(om, x):
+    """oracle [ntw][P_space...][Ft] per window -> product [ntw][Ft][S]"""
+    out = np.empty_like(x)
+    for wi in om.windows:
+        S = wi["npad_tr"]
+        n = wi["ntw"] * S * wi["Ft"]
+        o = wi["model_off"]
+        out[o:o + n] = x[o:o + n].reshape(wi["ntw"], S, wi["Ft"]).transpose(0, 2, 1).reshape(-1)
+    return out
+
+
+def _to_oracle_layout(om, x):
+    out = np.empty_like(x)
+    for wi in om.windows:
+        S = wi["npad_tr"]
+        n = wi["ntw"] * S * wi["Ft"]
+        o = wi["model_off"]
+        out[o:o + n] = x[o:o + n].reshape(wi["ntw"], wi["Ft"], S).transpose(0, 2, 1).reshape(-1)
+    return out
+
+
+def _random_model(om, rng):
+    """Hermitian-consistent random model (R2C of random reals per window)."""
+    x = om.zeros_model().astype(np.complex128)
+    for wi in om.windows:
+        real = rng.random([wi["ntw"]] + wi["P"]) - 0.5
+        xw = np.fft.rfftn(real, axes=list(range(1, wi["ndim"] + 1))).reshape(-1)
+        x[wi["model_off"]:wi["model_off"] + len(xw)] = xw
+    return x
+
+
+def _geometries():
+    out = [("c0_small", synth.SMALL[0]()["geometry"]),
+           ("c1_small", synth.SMALL[1]()["geometry"]),
+           ("c3_small", synth.SMALL[3]()["geometry"])]
+    from tests.test_oracle_kats import _random_deblend_problem
+    rng = np.random.default_rng(5)
+    for k in range(6):
+        out.append(("random%d" % k, _random_deblend_problem(rng, wavelets=bool(k % 2))))
+    return out
+
+
+GEOS = _geometries()
+
+
+def _mute_vec(om, v):
+    iv = om.intervals()
+    offs = np.concatenate([[0], np.cumsum(iv[:, 2] - iv[:, 1])])
+    first = {}
+    for k, (c, a, b) in enumerate(iv):
+        first.setdefault(int(c), k)
+    v = v.copy()
+    for c, i, s, n in om.mutes():
+        o = offs[first[int(c)] + int(i)] + s
+        v[o:o + n] = 0
+    return v
+
+
+@pytest.mark.parametrize("name,geo", GEOS, ids=[g[0] for g in GEOS])
+def test_forward_adjoint_match_oracle_and_are_adjoint(name, geo):
+    kw = _kw(geo)
+    om = O.Model(double=True, **kw)
+    if om.n_windows == 0 or om.record_len == 0:
+        pytest.skip("empty geometry")
+    pm = L.Problem(**kw)
+    rng = np.random.default_rng(hash(name) % 2 ** 32)
+    x = _random_model(om, rng)
+    # forward
+    ref = om.forward(x)
+    got = pm.forward(_to_product_layout(om, x).astype(np.complex64)).astype(np.float64)
+    scale = max(np.abs(ref).max(), 1e-30)
+    assert np.abs(got - ref).max() <= 2e-6 * scale * max(1.0, np.sqrt(len(ref)) / 64)
+    rel = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30)
+    assert rel <= 2e-6, rel
+    # adjoint
+    y = _mute_vec(om, rng.random(om.record_len) - 0.5)
+    gref = om.adjoint_update(y, -1.0, om.zeros_model())
+    gp = _to_oracle_layout(om, pm.adjoint(y.astype(np.float32)).astype(np.complex128))
+    rel_g = np.linalg.norm(gp - gref) / max(np.linalg.norm(gref), 1e-30)
+    assert rel_g <= 2e-6, rel_g
+    # dot test of the GPU operators, accumulated in FP64
+    lhs = float(np.dot(got, y))
+    rhs = float(np.sum(x.real * gp.real + x.imag * gp.imag))
+    assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs), 1e-30), (lhs, rhs)
+    pm.close()
+
+
+def test_stepsize_matches_oracle():
+    for name, geo in GEOS:
+        kw = _kw(geo)
+        om = O.Model(double=False, **kw)
+        if om.n_windows == 0 or om.record_len == 0:
+            continue
+        pm = L.Problem(**kw)
+        assert pm.stepsize() == om.stepsize(), name
+        pm.close()
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 3])
+def test_deblend_matches_fp64_oracle(cfg):
+    prob = synth.SMALL[cfg]()
+    g = prob["geometry"]
+    kw = _kw(g)
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    n_its = prob["n_its"]
+    ref, stats = O.deblend(data=blended, initial_factor=1.0, n_its=n_its, double=True, **kw)
+    out = agd.deblend(g["volumes"], g["window_shapes"], g["coords"], g["shottimes"],
+                      g["channels"], g["trace_types"], 1.0, n_its, 0, blended,
+                      wavelet_idxs=g.get("wavelet_idxs"), wavelets=g.get("wavelets"))
+    num = sum(float(np.sum((np.asarray(a, np.float64) - b) ** 2)) for a, b in zip(out, ref))
+    den = sum(float(np.sum(np.asarray(b, np.float64) ** 2)) for b in ref)
+    rel = (num / den) ** 0.5
+    assert rel <= 1e-4, rel
+
+
+def test_deblend_with_initial_factor_below_one():
+    # Q1: with f < 1 the start is the sign-flipped shrunk update
+    prob = synth.SMALL[0]()
+    g = prob["geometry"]
+    kw = _kw(g)
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    ref, _ = O.deblend(data=blended, initial_factor=0.5, n_its=8, double=True, **kw)
+    out = agd.deblend(g["volumes"], g["window_shapes"], g["coords"], g["shottimes"],
+                      g["channels"], g["trace_types"], 0.5, 8, 0, blended)
+    rel = np.linalg.norm(out[0] - ref[0]) / np.linalg.norm(ref[0])
+    assert rel <= 1e-4, rel
+
+
+def test_agd_blend_kats():
+    # test.cpp:30-430
+    out = agd.blend([[25], [27]], [[123], [123]], [[LIVE], [LIVE]],
+                    [np.array([[1., 2, 3]]), np.array([[4., 5, 6]])], agd.AGDBlendSum)
+    assert out[0].tolist() == [[1, 2, 7]] and out[1].tolist() == [[7, 5, 6]]
+    for tin, tout in [(MISSING, MISSING), (MISSING, LIVE), (LIVE, MISSING)]:
+        out = agd.blend([[25]], [[123]], [[tin]], [np.array([[1., 2, 3]])], agd.AGDBlendSum,
+                        trace_types_out=[[tout]])
+        assert out[0].tolist() == [[0, 0, 0]]
+    out = agd.blend([[25, 27, 29, 31, -10]], [[123, 123, 123, 123, -789]],
+                    [[MISSING, LIVE, LIVE, BAD, LIVE]],
+                    [np.arange(1, 16, dtype=np.float32).reshape(5, 3)], agd.AGDBlendSum,
+                    n_times_out=[2, 3], shottimes_out=[[24, 26, -10], [28, 30]],
+                    channels_out=[[123, 123, -789], [123, 123]],
+                    trace_types_out=[[MISSING, LIVE, LIVE], [LIVE, LIVE]])
+    assert out[0].tolist() == [[0, 0], [0, 4], [13, 14]]
+    assert out[1].tolist() == [[5, 13, 8], [8, 0, 0]]
+
+
+@pytest.mark.parametrize("mode,taper", [(0, 0), (2, 0), (1, 0), (1, 5)])
+def test_agd_blend_random_matches_oracle(mode, taper):
+    rng = np.random.default_rng(100 + mode * 10 + taper)
+    for _ in range(4):
+        np_ = int(rng.integers(1, 4))
+        ntr = [int(rng.integers(1, 40)) for _ in range(np_)]
+        nt = [int(rng.integers(1, 60)) for _ in range(np_)]
+        smax = max(1, sum(a * b for a, b in zip(ntr, nt)) // 3)
+        shot = [rng.integers(0, smax, n) for n in ntr]
+        ch = [rng.integers(0, max(1, sum(ntr) // 10), n) for n in ntr]
+        tt = [rng.integers(0, 3, n) for n in ntr]
+        data = [rng.standard_normal((a, b)).astype(np.float32) for a, b in zip(ntr, nt)]
+        tto = [np.where(t == BAD, LIVE, t) for t in tt]
+        ref = O.blend(ntr, nt, shot, ch, tt, data, mode, taper, ntr, nt, shot, ch, tto)
+        got = agd.blend(shot, ch, tt, data, mode, taper_length=taper, trace_types_out=tto)
+        for a, b in zip(got, ref):
+            if mode == 1 and taper > 1:
+                np.testing.assert_allclose(a, b, rtol=3e-7, atol=1e-7)
+            else:
+                np.testing.assert_array_equal(a, b)
+
+
+def test_blend_input_record_matches_oracle():
+    prob = synth.SMALL[1]()
+    g = prob["geometry"]
+    kw = _kw(g)
+    om = O.Model(double=False, **kw)
+    pm = L.Problem(**kw)
+    ref = om.blend_input(prob["truth"], g["shottimes"], g["channels"], g["trace_types"])
+    got = pm.set_data(prob["truth"], want_record=True)
+    np.testing.assert_array_equal(got, ref)
+    pm.close()
