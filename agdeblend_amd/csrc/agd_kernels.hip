@@ -571,7 +571,10 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial(
     g.str[e] = st;
     st *= w.P[e];
   }
-  for (int i = threadIdx.x; i < nk * S; i += blockDim.x) A[i] = make_float2(0.f, 0.f);
+  // Both ping-pong buffers start at zero: each dimension's transform only
+  // writes the lines it processes, and the positions it skips are the zero
+  // padding that the next dimension's transform reads.
+  for (int i = threadIdx.x; i < 2 * nk * S; i += blockDim.x) A[i] = make_float2(0.f, 0.f);
   __syncthreads();
   for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
     int const k = i / ntr;
