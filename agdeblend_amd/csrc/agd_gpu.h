@@ -31,15 +31,19 @@ typedef struct {
   AgdEntry const *went;       /* per global window trace blend entry */
   int const *trace_win;       /* per global window trace: its window */
   int n_wtr;                  /* global window traces */
-  /* spatial tasks */
+  /* spatial tasks, grouped in classes of equal sizes */
   AgdTask const *tasks;
   int n_tasks;
-  size_t lds_spatial;         /* bytes */
+  size_t lds_spatial;         /* bytes (generic kernel) */
+  AgdClass const *sclass;     /* host array */
+  int n_sclass;
   /* time blocks: (window, first trace) pairs */
   int const *tblocks;         /* 2 ints per block */
   int n_tblocks;
   int tb;                     /* traces per time block */
-  size_t lds_time;
+  size_t lds_time;            /* bytes (generic kernel) */
+  AgdClass const *tclass;     /* host array */
+  int n_tclass;
   /* record gather of window traces */
   int64_t rec_len;
   int tile;
@@ -55,7 +59,8 @@ typedef struct {
   size_t lds_wavelet;
   /* buffers */
   float *X;                   /* model, complex */
-  float *U;                   /* intermediate, complex */
+  float *U;                   /* forward intermediate, complex */
+  float *V;                   /* adjoint intermediate, complex */
   float *T;                   /* traces */
   float *T2;                  /* traces after wavelet adjoint */
   float *d;                   /* blended input record */
@@ -97,8 +102,12 @@ int agdg_blend_traces(AgdDev const *g, int resid);
 /* adjoint pieces */
 int agdg_adj_time(AgdDev const *g, int from_record);
 /* mode 0: X = X - alpha*G then shrink by lamb; mode 1: X = -alpha*G and
- * record max |X|^2 bits in scalar[0]; mode 2: X = X - alpha*G (no shrink) */
-int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode);
+ * record max |X|^2 bits in scalar[0]; mode 2: X = X - alpha*G (no shrink);
+ * mode 3: X = G. fuse: also write the forward spatial transform of the new X
+ * into U (the next iteration's first forward stage). */
+int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fuse);
+/* 1 when every spatial class has a specialised kernel (fusion available) */
+int agdg_can_fuse(AgdDev const *g);
 int agdg_shrink(AgdDev const *g, int64_t n_complex, float lamb);
 /* fold of spatial tapers for the stepsize; max written to scalar[1] */
 int agdg_stepsize_fold(AgdDev const *g);

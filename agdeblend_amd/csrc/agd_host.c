@@ -661,6 +661,10 @@ struct AgdProblem {
   double bytes_M, bytes_T, bytes_R;
   int use_wavelets;
   int host_only; /* tables only, no device (CPU tests of the geometry) */
+  AgdClass *sclass, *tclass;
+  int n_sclass, n_tclass;
+  int fuse;      /* adjoint kernels also write the next forward's stage 1 */
+  int u_valid;   /* U holds the forward spatial transform of X */
 };
 
 static void *dev_alloc(AgdProblem *p, size_t bytes) {
@@ -696,6 +700,8 @@ EXPORT void agdx_destroy(AgdProblem *p) {
   free(p->wcoord);
   free(p->wpatch_trace);
   free(p->dcoord);
+  free(p->sclass);
+  free(p->tclass);
   blendcfg_free(&p->cfg);
   free(p);
 }
@@ -978,6 +984,168 @@ static int wavelet_conv_spectrum(float const *wav, int len, int N, int nfk, int 
   return 0;
 }
 
+/* Sizes with specialised kernels (must match SPATIAL_SIZES / TIME_SIZES in
+ * agd_kernels.hip) */
+static int const kSpatialSizes[][2] = {{1, 10},  {1, 20},  {1, 40},  {10, 10},
+                                       {10, 20}, {10, 40}, {20, 10}, {20, 20},
+                                       {20, 40}, {40, 10}, {40, 20}, {40, 40}};
+static int const kTimeSizes[][2] = {{8, 5}, {8, 10}, {16, 10}, {16, 20}};
+
+static void spatial_key(AgdWin const *W, int *p0, int *p1) {
+  size_t i;
+  int a = 0, b = 0;
+  if (W->nsp == 1) {
+    a = 1;
+    b = W->P[0];
+  } else if (W->nsp == 2) {
+    a = W->P[0];
+    b = W->P[1];
+  }
+  *p0 = 0;
+  *p1 = 0;
+  for (i = 0; i < sizeof(kSpatialSizes) / sizeof(kSpatialSizes[0]); i++)
+    if (kSpatialSizes[i][0] == a && kSpatialSizes[i][1] == b) {
+      *p0 = a;
+      *p1 = b;
+    }
+}
+
+static void time_key(AgdWin const *W, int *m1, int *m2) {
+  size_t i;
+  *m1 = 0;
+  *m2 = 0;
+  if (W->Pt % 2) return;
+  for (i = 0; i < sizeof(kTimeSizes) / sizeof(kTimeSizes[0]); i++)
+    if (kTimeSizes[i][0] * kTimeSizes[i][1] == W->Pt / 2) {
+      *m1 = kTimeSizes[i][0];
+      *m2 = kTimeSizes[i][1];
+    }
+}
+
+/* Spatial tasks (chunks of time-frequency planes) and time blocks (runs of
+ * traces), each grouped into classes of windows with equal transform
+ * sizes; a class is served by one kernel launch. */
+static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp_out,
+                         size_t *lds_t_out) {
+  int w, c;
+  size_t lds_sp = 0, lds_t = 0;
+  vec sc, tc;
+  vec_init(&sc, sizeof(AgdClass));
+  vec_init(&tc, sizeof(AgdClass));
+  for (w = 0; w < p->n_win; w++) {
+    AgdWin const *W = &p->win[w];
+    int a, b, found = 0;
+    AgdClass *cl = (AgdClass *)sc.p;
+    spatial_key(W, &a, &b);
+    for (c = 0; c < (int)sc.n; c++)
+      if (cl[c].p0 == a && cl[c].p1 == b) found = 1;
+    if (!found) {
+      AgdClass *n = (AgdClass *)vec_push(&sc);
+      if (n == NULL) return 1;
+      memset(n, 0, sizeof(*n));
+      n->p0 = a;
+      n->p1 = b;
+    }
+    time_key(W, &a, &b);
+    found = 0;
+    cl = (AgdClass *)tc.p;
+    for (c = 0; c < (int)tc.n; c++)
+      if (cl[c].p0 == a && cl[c].p1 == b) found = 1;
+    if (!found) {
+      AgdClass *n = (AgdClass *)vec_push(&tc);
+      if (n == NULL) return 1;
+      memset(n, 0, sizeof(*n));
+      n->p0 = a;
+      n->p1 = b;
+    }
+  }
+  for (c = 0; c < (int)sc.n; c++) {
+    AgdClass *cl = (AgdClass *)sc.p + c;
+    cl->begin = (int)tasks->n;
+    for (w = 0; w < p->n_win; w++) {
+      AgdWin const *W = &p->win[w];
+      int a, b, nk, tw, k0;
+      spatial_key(W, &a, &b);
+      if (a != cl->p0 || b != cl->p1) continue;
+      if (a) {
+        nk = (int)(40960 / ((size_t)a * (b + 1) * 8));
+        if (nk > 64) nk = 64;
+      } else {
+        nk = (int)(LDS_BUDGET / (2 * (size_t)W->S * 8));
+      }
+      if (nk < 1) {
+        fprintf(stderr, "ERROR: window of %d padded traces exceeds the LDS budget\n", W->S);
+        return 1;
+      }
+      if (nk > W->Ft) nk = W->Ft;
+      for (tw = 0; tw < W->ntw; tw++) {
+        for (k0 = 0; k0 < W->Ft; k0 += nk) {
+          AgdTask *tk = (AgdTask *)vec_push(tasks);
+          if (tk == NULL) return 1;
+          tk->win = w;
+          tk->tw = tw;
+          tk->k0 = k0;
+          tk->nk = (k0 + nk <= W->Ft) ? nk : W->Ft - k0;
+        }
+      }
+      if (a) {
+        size_t const l = (size_t)nk * a * (b + 1) * 8;
+        if (l > cl->lds) cl->lds = l;
+      } else if ((size_t)2 * nk * W->S * 8 > lds_sp) {
+        lds_sp = (size_t)2 * nk * W->S * 8;
+      }
+    }
+    cl->count = (int)tasks->n - cl->begin;
+  }
+  for (c = 0; c < (int)tc.n; c++) {
+    AgdClass *cl = (AgdClass *)tc.p + c;
+    int tb;
+    /* traces per block: as many as fit the LDS budget, up to 16 */
+    for (tb = 16; tb >= 1; tb /= 2) {
+      size_t need = 0;
+      for (w = 0; w < p->n_win; w++) {
+        AgdWin const *W = &p->win[w];
+        int a, b;
+        size_t l;
+        time_key(W, &a, &b);
+        if (a != cl->p0 || b != cl->p1) continue;
+        if (a) {
+          l = (size_t)2 * tb * (W->Pt / 2 + 1) * 8 + (size_t)2 * tb * W->Wt * 4;
+        } else {
+          int const M = (W->Pt % 2 == 0 && W->Pt >= 4) ? W->Pt / 2 : W->Pt;
+          l = (size_t)tb * W->Ft * 8 + (size_t)2 * tb * M * 8 + (size_t)2 * tb * W->Wt * 4;
+        }
+        if (l > need) need = l;
+      }
+      cl->lds = need;
+      if (need <= (cl->p0 ? (size_t)80 * 1024 : (size_t)LDS_BUDGET) || tb == 1) break;
+    }
+    cl->tb = tb;
+    if (!cl->p0) lds_t = cl->lds > lds_t ? cl->lds : lds_t;
+    cl->begin = (int)tblocks->n;
+    for (w = 0; w < p->n_win; w++) {
+      AgdWin const *W = &p->win[w];
+      int a, b, t0;
+      time_key(W, &a, &b);
+      if (a != cl->p0 || b != cl->p1) continue;
+      for (t0 = 0; t0 < W->ntr; t0 += tb) {
+        int *b2 = (int *)vec_push(tblocks);
+        if (b2 == NULL) return 1;
+        b2[0] = w;
+        b2[1] = t0;
+      }
+    }
+    cl->count = (int)tblocks->n - cl->begin;
+  }
+  p->sclass = (AgdClass *)sc.p;
+  p->n_sclass = (int)sc.n;
+  p->tclass = (AgdClass *)tc.p;
+  p->n_tclass = (int)tc.n;
+  *lds_sp_out = lds_sp;
+  *lds_t_out = lds_t;
+  return 0;
+}
+
 static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
                                int const *const *wshape,
                                int const *const *coords,
@@ -1194,55 +1362,9 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
           wsrc_off[g] = (int)q; /* resolved to an offset below */
         }
       }
-      /* spatial tasks: chunks of time frequencies */
-      {
-        int nk = (int)(LDS_BUDGET / (2 * (size_t)W->S * 8));
-        int tw;
-        if (nk < 1) {
-          fprintf(stderr, "ERROR: window of %d padded traces exceeds the LDS budget\n",
-                  W->S);
-          goto err;
-        }
-        if (nk > W->Ft) nk = W->Ft;
-        for (tw = 0; tw < W->ntw; tw++) {
-          for (k0 = 0; k0 < W->Ft; k0 += nk) {
-            AgdTask *tk = (AgdTask *)vec_push(&tasks);
-            if (tk == NULL) goto err;
-            tk->win = w;
-            tk->tw = tw;
-            tk->k0 = k0;
-            tk->nk = (k0 + nk <= W->Ft) ? nk : W->Ft - k0;
-          }
-        }
-        if ((size_t)2 * nk * W->S * 8 > lds_sp) lds_sp = (size_t)2 * nk * W->S * 8;
-      }
     }
-    /* time blocks: traces per block chosen so the largest window's LDS
-     * footprint (spectra, ping-pong FFT buffers, overlap buffers) fits */
-    {
-      int tb;
-      for (tb = 8; tb >= 1; tb /= 2) {
-        lds_t = 0;
-        for (w = 0; w < p->n_win; w++) {
-          AgdWin const *W = &p->win[w];
-          int const M = (W->Pt % 2 == 0 && W->Pt >= 4) ? W->Pt / 2 : W->Pt;
-          size_t const l1 = (size_t)tb * W->Ft * 8 + (size_t)2 * tb * M * 8 +
-                            (size_t)2 * tb * W->Wt * 4;
-          if (l1 > lds_t) lds_t = l1;
-        }
-        if (lds_t <= LDS_BUDGET || tb == 1) break;
-      }
-      p->dev.tb = tb;
-      for (w = 0; w < p->n_win; w++) {
-        int t0;
-        for (t0 = 0; t0 < p->win[w].ntr; t0 += tb) {
-          int *b2 = (int *)vec_push(&tblocks);
-          if (b2 == NULL) goto err;
-          b2[0] = w;
-          b2[1] = t0;
-        }
-      }
-    }
+    /* spatial tasks and time blocks, grouped by transform sizes */
+    if (build_classes(p, &tasks, &tblocks, &lds_sp, &lds_t)) goto err;
     if (lds_sp > LDS_MAX || lds_t > LDS_MAX) {
       fprintf(stderr, "ERROR: window too large for the LDS-resident transforms\n");
       goto err;
@@ -1316,6 +1438,10 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->dev.tasks = (AgdTask const *)dev_upload(p, tasks.p, tasks.n * sizeof(AgdTask));
     p->dev.n_tasks = (int)tasks.n;
     p->dev.lds_spatial = lds_sp;
+    p->dev.sclass = p->sclass;
+    p->dev.n_sclass = p->n_sclass;
+    p->dev.tclass = p->tclass;
+    p->dev.n_tclass = p->n_tclass;
     p->dev.tblocks = (int const *)dev_upload(p, tblocks.p, tblocks.n * 2 * sizeof(int));
     p->dev.n_tblocks = (int)tblocks.n;
     p->dev.lds_time = lds_t;
@@ -1326,6 +1452,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->dev.lds_wavelet = lds_w;
     p->dev.X = (float *)dev_alloc(p, (size_t)xo * 8);
     p->dev.U = (float *)dev_alloc(p, (size_t)uo * 8);
+    p->dev.V = (float *)dev_alloc(p, (size_t)uo * 8);
     p->dev.T = (float *)dev_alloc(p, (size_t)to * 4);
     p->dev.T2 = p->use_wavelets ? (float *)dev_alloc(p, (size_t)to * 4) : p->dev.T;
     p->dev.d = (float *)dev_alloc(p, (size_t)p->cfg.rec_len * 4 + 16);
@@ -1347,7 +1474,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     if (!p->dev.win || !p->dev.plans || !p->dev.roots || !p->dev.tapers ||
         !p->dev.tscale || !p->dev.tconst || !p->dev.went || !p->dev.trace_win ||
         !p->dev.tasks || !p->dev.tblocks || !p->dev.wsrc_off || !p->dev.X ||
-        !p->dev.U || !p->dev.T || !p->dev.T2 || !p->dev.d || !p->dev.r ||
+        !p->dev.U || !p->dev.V || !p->dev.T || !p->dev.T2 || !p->dev.d || !p->dev.r ||
         !p->dev.scalar || !p->dev.partials || !p->dev.tile_ptr || !p->dev.tile_ent ||
         !p->dev.mute || !p->dev.wspec)
       goto err;
@@ -1454,6 +1581,8 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
         !p->dgather.tile_ptr || !p->dgather.tile_ent || !p->dgather.ent)
       goto err;
   }
+  p->fuse = !host_only && agdg_can_fuse(&p->dev) && getenv("AGD_NO_FUSE") == NULL;
+  p->u_valid = 0;
   /* stepsize (model.c:1230-1312): fold of the space tapers, max with 1 */
   if (!host_only) {
     unsigned int bits = 0;
@@ -1620,28 +1749,36 @@ EXPORT int agdx_set_data(AgdProblem *p, float const *const *data, float *d_out) 
   return agdg_sync();
 }
 
-/* model_forward into dev.r (resid: minus d) */
+/* model_forward into dev.r (resid: minus d). The spatial stage is skipped
+ * when the previous adjoint already produced it for the current model. */
 static int forward_record(AgdProblem *p, int resid) {
-  if (agdg_fwd_spatial(&p->dev, p->dev.X)) return 1;
+  if (!p->u_valid) {
+    if (agdg_fwd_spatial(&p->dev, p->dev.X)) return 1;
+    p->u_valid = 1;
+  }
   if (agdg_fwd_time(&p->dev)) return 1;
   if (p->use_wavelets && agdg_wavelet(&p->dev, 0)) return 1;
   return agdg_blend_traces(&p->dev, resid);
 }
 
-/* model_adjoint_update of dev.r into dev.X */
+/* model_adjoint_update of dev.r into dev.X (modes of agdg_adj_spatial) */
 static int adjoint_update(AgdProblem *p, float alpha, float lamb, int mode) {
+  int const fuse = p->fuse && (mode == 0 || mode == 2);
   if (p->use_wavelets) {
     if (agdg_wavelet(&p->dev, 1)) return 1;
     if (agdg_adj_time(&p->dev, 0)) return 1;
   } else {
     if (agdg_adj_time(&p->dev, 1)) return 1;
   }
-  return agdg_adj_spatial(&p->dev, alpha, lamb, mode);
+  if (agdg_adj_spatial(&p->dev, alpha, lamb, mode, fuse)) return 1;
+  p->u_valid = fuse;
+  return 0;
 }
 
 EXPORT int agdx_forward(AgdProblem *p, float const *x, float *rec) {
   size_t const mb = (size_t)agdx_model_len(p) * 8;
   if (agdg_h2d(p->dev.X, x, mb)) return 1;
+  p->u_valid = 0;
   if (forward_record(p, 0)) return 1;
   return agdg_d2h(rec, p->dev.r, (size_t)p->cfg.rec_len * 4);
 }
@@ -1719,6 +1856,7 @@ EXPORT int agdx_run(AgdProblem *p, float f, int n_its, int print_freq,
   if (x0 == NULL) {
     if (agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
   }
+  p->u_valid = 0;
   if (print_freq > 0) {
     double r2 = 0.0;
     if (agdg_norm2(&p->dev, p->dev.d, p->cfg.rec_len, &r2)) return 1;
@@ -1749,15 +1887,24 @@ EXPORT int agdx_iterate(AgdProblem *p, int n, float lamb) {
   return agdg_sync();
 }
 
-EXPORT int agdx_get_output(AgdProblem *p, float *const *data) {
-  int a;
-  if (agdg_fwd_spatial(&p->dev, p->dev.X)) return 1;
+/* set_output on the device: forward of the current model assembled into
+ * the device copy of the patch data (no host transfer) */
+EXPORT int agdx_finish(AgdProblem *p) {
+  if (!p->u_valid) {
+    if (agdg_fwd_spatial(&p->dev, p->dev.X)) return 1;
+    p->u_valid = 1;
+  }
   if (agdg_fwd_time(&p->dev)) return 1;
   if (p->use_wavelets && agdg_wavelet(&p->dev, 0)) return 1;
-  if (agdg_zero(p->d_data, (size_t)p->data_len * 4)) return 1;
   if (agdg_output(p->dev.T, p->n_out, p->d_o_off, p->d_o_nt, p->d_o_ptr, p->d_o_src,
                   p->d_data))
     return 1;
+  return agdg_sync();
+}
+
+EXPORT int agdx_get_output(AgdProblem *p, float *const *data) {
+  int a;
+  if (agdx_finish(p)) return 1;
   for (a = 0; a < p->n_patches; a++) {
     size_t const n = (size_t)p->ntr[a] * p->nt[a];
     if (agdg_d2h(data[a], p->d_data + p->data_off[a], n * 4)) return 1;

@@ -34,6 +34,7 @@
 #include <string>
 #include <vector>
 
+#include "agd_fft.h"
 #include "agd_gpu.h"
 
 #define NTHR 256
@@ -239,11 +240,11 @@ __device__ float2 *fft_lines(float2 *a, float2 *b, AgdFftPlan const &p,
 // Forward spatial transform: X[w][tw][k][S] -> U[w][tw][k][ntr]
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NTHR) void k_fwd_spatial(
-    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks,
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float2 const *__restrict__ X, float2 *__restrict__ U) {
   extern __shared__ float2 sm[];
-  AgdTask const t = tasks[blockIdx.x];
+  AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const w = wins[t.win];
   int const S = w.S;
   int const nk = t.nk;
@@ -295,13 +296,14 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial(
 // LDS: H [tb][Ft] + A,B [tb][M] complex + acc [tb][Wt] float
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NTHR) void k_fwd_time(
-    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int tb,
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float const *__restrict__ tapers, float const *__restrict__ tscale,
     float2 const *__restrict__ U, float *__restrict__ T) {
   extern __shared__ float2 sm[];
-  AgdWin const w = wins[blocks[2 * blockIdx.x]];
-  int const t0 = blocks[2 * blockIdx.x + 1];
+  int const bid = block0 + blockIdx.x;
+  AgdWin const w = wins[blocks[2 * bid]];
+  int const t0 = blocks[2 * bid + 1];
   int const nt = min(tb, w.ntr - t0);
   int const Pt = w.Pt, Ft = w.Ft, Wt = w.Wt, half = w.half;
   bool const even = (Pt % 2 == 0) && Pt >= 4;
@@ -465,14 +467,15 @@ __global__ __launch_bounds__(NTHR) void k_blend(
 // Adjoint time transform: record (or T2) -> V (taper, split, R2C, x2)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NTHR) void k_adj_time(
-    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int tb,
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float const *__restrict__ tapers, float const *__restrict__ tscale,
     AgdEntry const *__restrict__ went, float const *__restrict__ rec,
     float const *__restrict__ Tin, float2 *__restrict__ V) {
   extern __shared__ float2 sm[];
-  AgdWin const w = wins[blocks[2 * blockIdx.x]];
-  int const t0 = blocks[2 * blockIdx.x + 1];
+  int const bid = block0 + blockIdx.x;
+  AgdWin const w = wins[blocks[2 * bid]];
+  int const t0 = blocks[2 * bid + 1];
   int const nt = min(tb, w.ntr - t0);
   int const Pt = w.Pt, Ft = w.Ft, Wt = w.Wt, half = w.half;
   bool const even = (Pt % 2 == 0) && Pt >= 4;
@@ -550,12 +553,12 @@ __global__ __launch_bounds__(NTHR) void k_adj_time(
 // mode 2: X -= alpha*G; 3: G only (X = G, used by operator tests)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NTHR) void k_adj_spatial(
-    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks,
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float2 const *__restrict__ V, float2 *__restrict__ X, float alpha,
     float lamb, int mode, unsigned int *__restrict__ maxbits) {
   extern __shared__ float2 sm[];
-  AgdTask const t = tasks[blockIdx.x];
+  AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const w = wins[t.win];
   int const S = w.S;
   int const nk = t.nk;
@@ -835,8 +838,392 @@ __global__ __launch_bounds__(NTHR) void k_extract(
 }
 
 // ---------------------------------------------------------------------------
+// Size-specialised spatial transforms (one or two space dims).
+// A task holds nk time-frequency planes of one (window, time window); each
+// plane is P0 x P1 (P0 = 1 for one space dim). Column transforms (length
+// P0) run from global memory into registers, row transforms (length P1)
+// through one LDS tile with padded rows; every FFT is a register codelet.
+// ---------------------------------------------------------------------------
+template <int P0, int P1>
+__device__ __forceinline__ void win_w01(AgdWin const &w, int &W0, int &W1) {
+  if (P0 == 1) {
+    W0 = 1;
+    W1 = w.W[0];
+  } else {
+    W0 = w.W[0];
+    W1 = w.W[1];
+  }
+}
+
+// row transforms of rows [0, nrows) of each plane, in place, keeping nout
+// outputs (S = +1: inverse of the forward operator)
+template <int P0, int P1, int S>
+__device__ __forceinline__ void rows_fft(float2 *sm, int nk, int nrows, int nin, int nout) {
+  constexpr int LD = P1 + 1;
+  for (int i = threadIdx.x; i < nk * nrows; i += blockDim.x) {
+    int const k = i / nrows;
+    int const r = i - k * nrows;
+    float2 *row = sm + (k * P0 + r) * LD;
+    float2 v[P1];
+#pragma unroll
+    for (int c = 0; c < P1; ++c) v[c] = (c < nin) ? row[c] : make_float2(0.f, 0.f);
+    agdfft::fft<P1, S>(v);
+#pragma unroll
+    for (int c = 0; c < P1; ++c)
+      if (c < nout) row[c] = v[c];
+  }
+}
+
+template <int P0, int P1>
+__global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
+    float2 const *__restrict__ X, float2 *__restrict__ U) {
+  extern __shared__ float2 sm[];
+  constexpr int LD = P1 + 1;
+  constexpr int S = P0 * P1;
+  AgdTask const t = tasks[task0 + blockIdx.x];
+  AgdWin const &w = wins[t.win];
+  int W0, W1;
+  win_w01<P0, P1>(w, W0, W1);
+  int const nk = t.nk;
+  float2 const *src = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * S;
+  for (int i = threadIdx.x; i < nk * P1; i += blockDim.x) {
+    int const k = i / P1;
+    int const c1 = i - k * P1;
+    float2 v[P0];
+#pragma unroll
+    for (int c0 = 0; c0 < P0; ++c0) v[c0] = src[k * S + c0 * P1 + c1];
+    agdfft::fft<P0, +1>(v);
+#pragma unroll
+    for (int n0 = 0; n0 < P0; ++n0)
+      if (n0 < W0) sm[(k * P0 + n0) * LD + c1] = v[n0];
+  }
+  __syncthreads();
+  rows_fft<P0, P1, +1>(sm, nk, W0, P1, W1);
+  __syncthreads();
+  int const ntr = W0 * W1;
+  float2 *dst = U + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * ntr;
+  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
+    int const k = i / ntr;
+    int const r = i - k * ntr;
+    int const n0 = r / W1;
+    int const n1 = r - n0 * W1;
+    dst[i] = sm[(k * P0 + n0) * LD + n1];
+  }
+}
+
+// Adjoint spatial transform fused with the update / shrink (modes as in
+// k_adj_spatial) and, when fuse is set, with the forward spatial transform
+// of the updated model into U.
+template <int P0, int P1>
+__global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
+    float2 const *__restrict__ V, float2 *__restrict__ X, float2 *__restrict__ U,
+    float alpha, float lamb, int mode, int fuse, unsigned int *__restrict__ maxbits) {
+  extern __shared__ float2 sm[];
+  constexpr int LD = P1 + 1;
+  constexpr int S = P0 * P1;
+  AgdTask const t = tasks[task0 + blockIdx.x];
+  AgdWin const &w = wins[t.win];
+  int W0, W1;
+  win_w01<P0, P1>(w, W0, W1);
+  int const nk = t.nk;
+  int const ntr = W0 * W1;
+  float2 const *vin = V + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * ntr;
+  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
+    int const k = i / ntr;
+    int const r = i - k * ntr;
+    int const n0 = r / W1;
+    int const n1 = r - n0 * W1;
+    sm[(k * P0 + n0) * LD + n1] = vin[i];
+  }
+  __syncthreads();
+  rows_fft<P0, P1, -1>(sm, nk, W0, W1, P1);
+  __syncthreads();
+  float2 *x = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * S;
+  float const l2 = lamb * lamb;
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < nk * P1; i += blockDim.x) {
+    int const k = i / P1;
+    int const c1 = i - k * P1;
+    float2 v[P0];
+#pragma unroll
+    for (int r = 0; r < P0; ++r)
+      v[r] = (r < W0) ? sm[(k * P0 + r) * LD + c1] : make_float2(0.f, 0.f);
+    agdfft::fft<P0, -1>(v);
+#pragma unroll
+    for (int c0 = 0; c0 < P0; ++c0) {
+      float2 const gv = v[c0];
+      float2 nv;
+      int const gi = k * S + c0 * P1 + c1;
+      if (mode == 3) {
+        nv = gv;
+      } else if (mode == 1) {
+        nv = make_float2(0.f - alpha * gv.x, 0.f - alpha * gv.y);
+        mx = fmaxf(mx, nv.x * nv.x + nv.y * nv.y);
+      } else {
+        nv = x[gi];
+        nv.x -= alpha * gv.x;
+        nv.y -= alpha * gv.y;
+        if (mode == 0) {
+          float const a = nv.x * nv.x + nv.y * nv.y;
+          if (a <= l2) {
+            nv = make_float2(0.f, 0.f);
+          } else {
+            float const ab = sqrtf(a);
+            float const sh = (ab - lamb) / ab;
+            nv.x *= sh;
+            nv.y *= sh;
+          }
+        }
+      }
+      x[gi] = nv;
+      v[c0] = nv;
+    }
+    if (fuse) {
+      agdfft::fft<P0, +1>(v);
+#pragma unroll
+      for (int n0 = 0; n0 < P0; ++n0)
+        if (n0 < W0) sm[(k * P0 + n0) * LD + c1] = v[n0];
+    }
+  }
+  if (mode == 1) {
+    __shared__ float red[NTHR / 64];
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmaxf(m, red[q]);
+      atomicMax(maxbits, __float_as_uint(m));
+    }
+  }
+  if (!fuse) return;
+  __syncthreads();
+  rows_fft<P0, P1, +1>(sm, nk, W0, P1, W1);
+  __syncthreads();
+  float2 *dst = U + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * ntr;
+  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
+    int const k = i / ntr;
+    int const r = i - k * ntr;
+    int const n0 = r / W1;
+    int const n1 = r - n0 * W1;
+    dst[i] = sm[(k * P0 + n0) * LD + n1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Size-specialised time transforms, Pt = 2 M, M = M1 * M2 (two-phase FFT:
+// M2 register FFTs of length M1, twiddle, M1 register DFTs of length M2).
+// LDS per block: two [tb][M+1] complex buffers (+ the overlap buffers of
+// the forward).
+// ---------------------------------------------------------------------------
+// In-place-ish two-phase FFT of length M on lines of buffer Z (line stride
+// M+1); the result is written, in natural order, into buffer O.
+template <int M1, int M2, int S>
+__device__ __forceinline__ void fft2phase(float2 *Z, float2 *O, int tb,
+                                          float2 const *__restrict__ rootsM) {
+  constexpr int M = M1 * M2;
+  constexpr int LDL = M + 1;
+  for (int i = threadIdx.x; i < tb * M2; i += blockDim.x) {
+    int const j = i / M2;
+    int const k2 = i - j * M2;
+    float2 *z = Z + j * LDL;
+    float2 v[M1];
+#pragma unroll
+    for (int k1 = 0; k1 < M1; ++k1) v[k1] = z[k1 * M2 + k2];
+    agdfft::fft<M1, S>(v);
+#pragma unroll
+    for (int n1 = 0; n1 < M1; ++n1) {
+      float2 tw = rootsM[(n1 * k2) % M];
+      if (S > 0) tw.y = -tw.y;
+      z[n1 * M2 + k2] = (n1 == 0 || k2 == 0) ? v[n1] : agdfft::c_mul(v[n1], tw);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tb * M1; i += blockDim.x) {
+    int const j = i / M1;
+    int const n1 = i - j * M1;
+    float2 const *z = Z + j * LDL;
+    float2 *o = O + j * LDL;
+    float2 v[M2];
+#pragma unroll
+    for (int k2 = 0; k2 < M2; ++k2) v[k2] = z[n1 * M2 + k2];
+    agdfft::fft<M2, S>(v);
+#pragma unroll
+    for (int n2 = 0; n2 < M2; ++n2) o[n1 + M1 * n2] = v[n2];
+  }
+  __syncthreads();
+}
+
+template <int M1, int M2>
+__global__ __launch_bounds__(NTHR) void k_fwd_time_t(
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float const *__restrict__ tapers, float const *__restrict__ tscale,
+    float2 const *__restrict__ U, float *__restrict__ T) {
+  extern __shared__ float2 sm[];
+  constexpr int M = M1 * M2;
+  constexpr int Pt = 2 * M;
+  constexpr int LDL = M + 1;
+  int const bid = block0 + blockIdx.x;
+  AgdWin const &w = wins[blocks[2 * bid]];
+  int const t0 = blocks[2 * bid + 1];
+  int const ntr = w.ntr;
+  int const nt = min(tb, ntr - t0);
+  int const Wt = w.Wt, half = w.half, nconv = w.nconv;
+  float2 *SP = sm;
+  float2 *Z = SP + tb * LDL;
+  float *acc = (float *)(Z + tb * LDL);
+  float *nxt = acc + tb * Wt;
+  float2 const *rootsM = roots + plans[w.tplan].roots_off;
+  float2 const *rootsP = roots + plans[w.tplan_full].roots_off;
+  float const *tt = tapers + w.tt_off;
+  float const *sc = tscale + w.trace0 + t0;
+  float *Tb = T + w.t_off + (int64_t)t0 * nconv;
+  for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) acc[i] = 0.f;
+  for (int tw = 0; tw < w.ntw; ++tw) {
+    float2 const *u = U + w.u_off + (int64_t)tw * (M + 1) * ntr + t0;
+    for (int i = threadIdx.x; i < (M + 1) * tb; i += blockDim.x) {
+      int const k = i / tb;
+      int const j = i - k * tb;
+      SP[j * LDL + k] = j < nt ? u[(int64_t)k * ntr + j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    // half-length packing of the C2R (DC / Nyquist imaginary parts ignored)
+    for (int i = threadIdx.x; i < tb * M; i += blockDim.x) {
+      int const j = i / M;
+      int const k = i - j * M;
+      float2 xk = SP[j * LDL + k];
+      float2 xc = SP[j * LDL + (M - k)];
+      if (k == 0) {
+        xk.y = 0.f;
+        xc.y = 0.f;
+      }
+      xc.y = -xc.y;
+      float2 const e = agdfft::c_add(xk, xc);
+      float2 o = agdfft::c_sub(xk, xc);
+      float2 r = rootsP[k];
+      r.y = -r.y;
+      o = agdfft::c_mul(o, r);
+      Z[j * LDL + k] = make_float2(e.x - o.y, e.y + o.x);
+    }
+    __syncthreads();
+    fft2phase<M1, M2, +1>(Z, SP, tb, rootsM);
+    for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) {
+      int const j = i / Wt;
+      int const s = i - j * Wt;
+      if (j >= nt) continue;
+      float2 const z = SP[j * LDL + (s >> 1)];
+      float const y = (s & 1) ? z.y : z.x;
+      acc[i] += y * sc[j] * tt[s];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) {
+      int const j = i / Wt;
+      int const s = i - j * Wt;
+      if (s < half) {
+        int const t = tw * half + s;
+        if (j < nt && t < nconv) Tb[(int64_t)j * nconv + t] = acc[i];
+      }
+      nxt[i] = (s + half < Wt) ? acc[i + half] : 0.f;
+    }
+    __syncthreads();
+    float *tmp = acc;
+    acc = nxt;
+    nxt = tmp;
+  }
+  for (int i = threadIdx.x; i < tb * (Wt - half); i += blockDim.x) {
+    int const j = i / (Wt - half);
+    int const s = i - j * (Wt - half);
+    int const t = w.ntw * half + s;
+    if (j < nt && t < nconv) Tb[(int64_t)j * nconv + t] = acc[j * Wt + s];
+  }
+  int const tail0 = w.ntw * half + (Wt - half);
+  for (int j = 0; j < nt; ++j)
+    for (int t = tail0 + threadIdx.x; t < nconv; t += blockDim.x)
+      Tb[(int64_t)j * nconv + t] = 0.f;
+  (void)Pt;
+}
+
+template <int M1, int M2>
+__global__ __launch_bounds__(NTHR) void k_adj_time_t(
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float const *__restrict__ tapers, float const *__restrict__ tscale,
+    AgdEntry const *__restrict__ went, float const *__restrict__ rec,
+    float const *__restrict__ Tin, float2 *__restrict__ V) {
+  extern __shared__ float2 sm[];
+  constexpr int M = M1 * M2;
+  constexpr int Pt = 2 * M;
+  constexpr int LDL = M + 1;
+  int const bid = block0 + blockIdx.x;
+  AgdWin const &w = wins[blocks[2 * bid]];
+  int const t0 = blocks[2 * bid + 1];
+  int const ntr = w.ntr;
+  int const nt = min(tb, ntr - t0);
+  int const Wt = w.Wt, half = w.half, nconv = w.nconv;
+  float2 *Z = sm;
+  float2 *SP = Z + tb * LDL;
+  float2 const *rootsM = roots + plans[w.tplan].roots_off;
+  float2 const *rootsP = roots + plans[w.tplan_full].roots_off;
+  float const *tt = tapers + w.tt_off;
+  for (int tw = 0; tw < w.ntw; ++tw) {
+    for (int i = threadIdx.x; i < tb * Pt; i += blockDim.x) {
+      int const j = i / Pt;
+      int const s = i - j * Pt;
+      float v = 0.f;
+      if (j < nt && s < Wt) {
+        int const g = w.trace0 + t0 + j;
+        int const t = tw * half + s;
+        float x;
+        if (Tin != nullptr) {
+          x = Tin[w.t_off + (int64_t)(t0 + j) * nconv + t];
+        } else {
+          AgdEntry const e = went[g];
+          int const i2 = t - e.trace_start;
+          x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+        }
+        v = x * tscale[g] * tt[s];
+      }
+      float *zf = (float *)(Z + j * LDL);
+      zf[s] = v;
+    }
+    __syncthreads();
+    fft2phase<M1, M2, -1>(Z, SP, tb, rootsM);
+    constexpr int kmax2 = (Pt - 1) / 2;
+    float2 *vout = V + w.u_off + (int64_t)tw * (M + 1) * ntr + t0;
+    for (int i = threadIdx.x; i < (M + 1) * tb; i += blockDim.x) {
+      int const k = i / tb;
+      int const j = i - k * tb;
+      if (j >= nt) continue;
+      float2 const zk = SP[j * LDL + (k % M)];
+      float2 zc = SP[j * LDL + ((M - k) % M)];
+      zc.y = -zc.y;
+      float2 const e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+      float2 const o = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+      float2 X = agdfft::c_add(e, agdfft::c_mul(o, rootsP[k]));
+      if (k == 0 || k == M) X.y = 0.f;
+      if (k >= 1 && k <= kmax2) {
+        X.x *= 2.f;
+        X.y *= 2.f;
+      }
+      vout[(int64_t)k * ntr + j] = X;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
+// Sizes with specialised kernels; the host groups work by these keys
+// (agd_host.c keeps the same lists).
+#define SPATIAL_SIZES(X) \
+  X(1, 10) X(1, 20) X(1, 40) X(10, 10) X(10, 20) X(10, 40) X(20, 10) X(20, 20) \
+  X(20, 40) X(40, 10) X(40, 20) X(40, 40)
+#define TIME_SIZES(X) X(8, 5) X(8, 10) X(16, 10) X(16, 20)
+
 extern "C" {
 
 int agdg_init(void) {
@@ -863,6 +1250,20 @@ int agdg_init(void) {
                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((void const *)k_wavelet,
                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+#define ATTR_S(a, b)                                                              \
+    (void)hipFuncSetAttribute((void const *)k_fwd_spatial_t<a, b>,                \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
+    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b>,                \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+#define ATTR_T(a, b)                                                              \
+    (void)hipFuncSetAttribute((void const *)k_fwd_time_t<a, b>,                   \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
+    (void)hipFuncSetAttribute((void const *)k_adj_time_t<a, b>,                   \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    SPATIAL_SIZES(ATTR_S)
+    TIME_SIZES(ATTR_T)
+#undef ATTR_S
+#undef ATTR_T
     (void)hipGetLastError();
   }
   return 0;
@@ -909,16 +1310,55 @@ int agdg_sync(void) {
 
 void *agdg_stream(void) { return (void *)g_stream; }
 
+int agdg_can_fuse(AgdDev const *g) {
+  for (int c = 0; c < g->n_sclass; ++c)
+    if (g->sclass[c].p0 == 0) return 0;
+  return 1;
+}
+
 int agdg_fwd_spatial(AgdDev const *g, float const *X) {
-  return launch("fwd_spatial", k_fwd_spatial, dim3(g->n_tasks), dim3(NTHR),
-                g->lds_spatial, g->win, g->tasks, g->plans, (float2 const *)g->roots,
-                (float2 const *)X, (float2 *)g->U);
+  for (int ci = 0; ci < g->n_sclass; ++ci) {
+    AgdClass const &c = g->sclass[ci];
+    bool done = false;
+#define FWD_CASE(a, b)                                                                 \
+    if (!done && c.p0 == a && c.p1 == b) {                                             \
+      if (launch("fwd_spatial", k_fwd_spatial_t<a, b>, dim3(c.count), dim3(NTHR), c.lds, \
+                 g->win, g->tasks, c.begin, (float2 const *)X, (float2 *)g->U))        \
+        return 1;                                                                      \
+      done = true;                                                                     \
+    }
+    SPATIAL_SIZES(FWD_CASE)
+#undef FWD_CASE
+    if (!done &&
+        launch("fwd_spatial", k_fwd_spatial, dim3(c.count), dim3(NTHR), g->lds_spatial,
+               g->win, g->tasks, c.begin, g->plans, (float2 const *)g->roots,
+               (float2 const *)X, (float2 *)g->U))
+      return 1;
+  }
+  return 0;
 }
 
 int agdg_fwd_time(AgdDev const *g) {
-  return launch("fwd_time", k_fwd_time, dim3(g->n_tblocks), dim3(NTHR), g->lds_time,
-                g->win, g->tblocks, g->tb, g->plans, (float2 const *)g->roots,
-                g->tapers, g->tscale, (float2 const *)g->U, g->T);
+  for (int ci = 0; ci < g->n_tclass; ++ci) {
+    AgdClass const &c = g->tclass[ci];
+    bool done = false;
+#define FT_CASE(a, b)                                                                   \
+    if (!done && c.p0 == a && c.p1 == b) {                                              \
+      if (launch("fwd_time", k_fwd_time_t<a, b>, dim3(c.count), dim3(NTHR), c.lds,        \
+                 g->win, g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots,  \
+                 g->tapers, g->tscale, (float2 const *)g->U, g->T))                     \
+        return 1;                                                                       \
+      done = true;                                                                      \
+    }
+    TIME_SIZES(FT_CASE)
+#undef FT_CASE
+    if (!done &&
+        launch("fwd_time", k_fwd_time, dim3(c.count), dim3(NTHR), g->lds_time, g->win,
+               g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots, g->tapers,
+               g->tscale, (float2 const *)g->U, g->T))
+      return 1;
+  }
+  return 0;
 }
 
 int agdg_blend_traces(AgdDev const *g, int resid) {
@@ -942,17 +1382,56 @@ int agdg_stepsize_fold(AgdDev const *g) {
 }
 
 int agdg_adj_time(AgdDev const *g, int from_record) {
-  return launch("adj_time", k_adj_time, dim3(g->n_tblocks), dim3(NTHR), g->lds_time,
-                g->win, g->tblocks, g->tb, g->plans, (float2 const *)g->roots,
-                g->tapers, g->tscale, g->went, (float const *)g->r,
-                from_record ? (float const *)nullptr : (float const *)g->T2,
-                (float2 *)g->U);
+  float const *Tin = from_record ? (float const *)nullptr : (float const *)g->T2;
+  for (int ci = 0; ci < g->n_tclass; ++ci) {
+    AgdClass const &c = g->tclass[ci];
+    bool done = false;
+#define AT_CASE(a, b)                                                                   \
+    if (!done && c.p0 == a && c.p1 == b) {                                              \
+      if (launch("adj_time", k_adj_time_t<a, b>, dim3(c.count), dim3(NTHR), c.lds,        \
+                 g->win, g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots,  \
+                 g->tapers, g->tscale, g->went, (float const *)g->r, Tin,               \
+                 (float2 *)g->V))                                                       \
+        return 1;                                                                       \
+      done = true;                                                                      \
+    }
+    TIME_SIZES(AT_CASE)
+#undef AT_CASE
+    if (!done &&
+        launch("adj_time", k_adj_time, dim3(c.count), dim3(NTHR), g->lds_time, g->win,
+               g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots, g->tapers,
+               g->tscale, g->went, (float const *)g->r, Tin, (float2 *)g->V))
+      return 1;
+  }
+  return 0;
 }
 
-int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode) {
-  return launch("adj_spatial", k_adj_spatial, dim3(g->n_tasks), dim3(NTHR),
-                g->lds_spatial, g->win, g->tasks, g->plans, (float2 const *)g->roots,
-                (float2 const *)g->U, (float2 *)g->X, alpha, lamb, mode, g->scalar);
+int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fuse) {
+  for (int ci = 0; ci < g->n_sclass; ++ci) {
+    AgdClass const &c = g->sclass[ci];
+    bool done = false;
+#define AS_CASE(a, b)                                                                  \
+    if (!done && c.p0 == a && c.p1 == b) {                                             \
+      if (launch("adj_spatial", k_adj_spatial_t<a, b>, dim3(c.count), dim3(NTHR), c.lds, \
+                 g->win, g->tasks, c.begin, (float2 const *)g->V, (float2 *)g->X,      \
+                 (float2 *)g->U, alpha, lamb, mode, fuse, g->scalar))                  \
+        return 1;                                                                      \
+      done = true;                                                                     \
+    }
+    SPATIAL_SIZES(AS_CASE)
+#undef AS_CASE
+    if (!done) {
+      if (fuse) {
+        fprintf(stderr, "ERROR: fused update requested for a generic window size\n");
+        return 1;
+      }
+      if (launch("adj_spatial", k_adj_spatial, dim3(c.count), dim3(NTHR), g->lds_spatial,
+                 g->win, g->tasks, c.begin, g->plans, (float2 const *)g->roots,
+                 (float2 const *)g->V, (float2 *)g->X, alpha, lamb, mode, g->scalar))
+        return 1;
+    }
+  }
+  return 0;
 }
 
 int agdg_shrink(AgdDev const *g, int64_t n_complex, float lamb) {

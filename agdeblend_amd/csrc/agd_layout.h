@@ -66,6 +66,18 @@ typedef struct {
   int n;
 } AgdEntry;
 
+/* A contiguous range of spatial tasks (or time blocks) whose windows share
+ * the transform sizes, so one specialised kernel instance serves them.
+ * p0 == 0: no specialised kernel, the generic one runs the range. */
+typedef struct {
+  int p0, p1;   /* spatial: padded sizes (p0 = 1 when one space dim);
+                   time: p0 = M1, p1 = M2 with M1*M2 = Pt/2 */
+  int begin, count;
+  int tb;       /* time classes: traces per block */
+  int pad;
+  size_t lds;
+} AgdClass;
+
 /* A unit of work for the spatial (per time-frequency chunk) kernels */
 typedef struct {
   int win;

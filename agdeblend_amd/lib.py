@@ -55,6 +55,7 @@ def load():
     fn("agdx_adjoint", ci, vp, vp, vp)
     fn("agdx_run", ci, vp, cf, ci, ci, vp)
     fn("agdx_get_output", ci, vp, vp)
+    fn("agdx_finish", ci, vp)
     fn("agdx_get_model", ci, vp, vp)
     fn("agdx_iterate", ci, vp, ci, cf)
     fn("agdx_last_lambda0", cf)
@@ -71,6 +72,7 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_get_output",
+           "agdx_finish",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
            "agdx_timing_get", "agdx_bytes"]
 
@@ -208,6 +210,11 @@ class Problem:
     def iterate(self, n, lamb):
         if self.L.agdx_iterate(self.h, int(n), float(lamb)):
             raise RuntimeError("agdx_iterate failed")
+
+    def finish(self):
+        """Output assembled on the device (no host copy)."""
+        if self.L.agdx_finish(self.h):
+            raise RuntimeError("agdx_finish failed")
 
     def output(self):
         out = PtrTable([np.zeros(s, np.float32).reshape(-1) for s in self.shapes], np.float32,

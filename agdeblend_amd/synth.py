@@ -113,7 +113,7 @@ def _volume3d(rng, n_ch, n_src, n_t, dt, f, dx=25.0, n_lin=6, n_hyp=4):
 
 
 def config1(seed=1001, n_ch=64, n_src=512, n_t=2048, windows=(16, 32, 256), n_its=200,
-            p_bad=0.05, p_missing=0.03):
+            p_bad=0.05, p_missing=0.03, interval_s=2.0, dither_s=0.5):
     """3-D (ChannelX, SourceX, Time) 64 x 512 x 2048 at 4 ms, 25 Hz Ricker
     dipping and hyperbolic events; channel id = ChannelX index; shots every
     2 s with +-0.5 s dither (the same shot time on every channel); 5% of
@@ -122,7 +122,7 @@ def config1(seed=1001, n_ch=64, n_src=512, n_t=2048, windows=(16, 32, 256), n_it
     rng = np.random.default_rng(seed)
     dt, f = 0.004, 25.0
     data = _volume3d(rng, n_ch, n_src, n_t, dt, f)
-    shot1 = _shot_times(rng, n_src, 2.0, 0.5, dt)
+    shot1 = _shot_times(rng, n_src, interval_s, dither_s, dt)
     shot = np.broadcast_to(shot1[None, :], (n_ch, n_src)).astype(np.int64)
     channels = np.broadcast_to(np.arange(n_ch, dtype=np.intc)[:, None], (n_ch, n_src)).copy()
     u = rng.random((n_ch, n_src))

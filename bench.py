@@ -1,0 +1,226 @@
+"""Benchmark of the MI355X deblending path (one GPU).
+
+A step is one ISTA iteration (forward FK operator, blend + residual, adjoint
+FK operator, update + soft threshold) over the whole synthetic volume of the
+workload, with the problem resident in HBM.
+
+Reported (one JSON line):
+  value        seconds to reach 20 dB deblending SNR with the plain GPU
+               inversion: the device time of agd_deblend's work (initial
+               update, the loop and the output assembly, inputs and tables
+               already in HBM) for the smallest n_its of a sweep whose output
+               reaches >= 20 dB (the threshold schedule depends on n_its, so
+               each n_its is a separate run)
+  ms_per_step  mean time of the K timed iterations
+  roofline     the FK operator: algorithmic bytes 3M + 2T per iteration
+               (M model bytes, T trace-buffer bytes) over the summed average
+               durations of its four kernels, timed with HIP events on the
+               library stream
+  cpu_baseline the oracle (float32, OpenMP over windows, the reference's
+               algorithm with a project FFT) timed on this host on a bounded
+               sample and scaled to the same n_its
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FK_KERNELS = ["fwd_spatial", "fwd_time", "adj_time", "adj_spatial"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def geometry_kwargs(g):
+    keys = ["volumes", "n_dims", "window_shapes", "coords", "shapes", "shottimes",
+            "channels", "trace_types"]
+    kw = {k: g[k] for k in keys}
+    for k in ("wavelet_lengths", "wavelet_idxs", "wavelets"):
+        if k in g:
+            kw[k] = g[k]
+    return kw
+
+
+def make_workload(cfg):
+    from agdeblend_amd import synth
+    if cfg == 1:
+        # shots every 4 s +- 1 s: two-fold blending, as in the 2-D workload
+        return synth.config1(interval_s=4.0, dither_s=1.0)
+    return synth.CONFIGS[cfg]()
+
+
+def time_to_snr(pm, prob, target_db, sweep, synth):
+    """Smallest n_its of the sweep reaching target_db: (seconds, n_its, snr)."""
+    results = []
+    for n in sweep:
+        t0 = time.perf_counter()
+        pm.run(1.0, n)
+        pm.finish()
+        el = time.perf_counter() - t0
+        out = pm.output()
+        snr = synth.snr(prob, out)
+        results.append((n, el, snr))
+        log("  n_its %5d  %.3f s  SNR %.2f dB" % (n, el, snr))
+        if snr >= target_db:
+            return el, n, snr, results
+    return None, None, None, results
+
+
+def subvolume(prob, n_first):
+    """The first n_first entries of the leading space axis of every patch
+    (same seed, same geometry otherwise)."""
+    g = dict(prob["geometry"])
+    cut = lambda a: [np.asarray(x)[:n_first] for x in a]  # noqa: E731
+    for k in ("shottimes", "channels", "trace_types", "wavelet_idxs"):
+        if k in g:
+            g[k] = cut(g[k])
+    g["shapes"] = [[n_first] + list(s[1:]) for s in g["shapes"]]
+    sub = dict(prob)
+    sub["geometry"] = g
+    sub["blended"] = cut(prob["blended"])
+    sub["truth"] = cut(prob["truth"])
+    return sub
+
+
+def cpu_baseline(prob, n_its_target, n_windows_full):
+    """Oracle float32 on this host on a bounded sample: a sub-volume (the
+    first channels) run with n_its=1 and n_its=2; the per-iteration and fixed
+    costs are scaled by the window count to the full volume and to
+    n_its_target iterations (the work per window is identical)."""
+    from oracle import oracle
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    oracle.lib(False).set_threads(cores)
+    lead = int(prob["geometry"]["shapes"][0][0])
+    n_sub = lead if len(prob["geometry"]["shapes"][0]) < 3 else max(2, lead // 2)
+    sub = subvolume(prob, n_sub)
+    kw = geometry_kwargs(sub["geometry"])
+    m = oracle.Model(double=False, **kw)
+    n_windows_sub = m.n_windows
+    del m
+    t0 = time.perf_counter()
+    oracle.deblend(data=sub["blended"], initial_factor=1.0, n_its=1, double=False, **kw)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.deblend(data=sub["blended"], initial_factor=1.0, n_its=2, double=False, **kw)
+    t2 = time.perf_counter() - t0
+    scale = n_windows_full / max(1, n_windows_sub)
+    per_it = max(t2 - t1, 1e-9) * scale
+    fixed = max(t1 * scale - per_it, 0.0)
+    value = fixed + per_it * n_its_target
+    return dict(value=round(value, 3), unit="s", cores=cores, kind="port",
+                sample="oracle float32 (%d threads) on the first %d of %d lines of the "
+                       "volume (%d of %d windows): n_its=1 %.2f s, n_its=2 %.2f s; scaled "
+                       "x%.3f to the full volume (%.3f s per iteration) and to %d "
+                       "iterations" % (cores, n_sub, lead, n_windows_sub, n_windows_full,
+                                       t1, t2, scale, per_it, n_its_target),
+                per_iteration_s=round(per_it, 4))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--target-db", type=float, default=20.0)
+    ap.add_argument("--sweep", type=str, default="50,100,150,200,300,400,600,800,1200")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-snr", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(the multi-GPU sharding is a design note, not an implementation)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from agdeblend_amd import agdeblend as agd
+    from agdeblend_amd import lib as L
+    from agdeblend_amd import synth
+
+    prob = make_workload(args.config)
+    g = prob["geometry"]
+    t0 = time.perf_counter()
+    prob["blended"] = synth.blend_input(prob, agd.blend)
+    log("workload %s; blend %.2f s; input SNR %.2f dB" %
+        (prob["name"], time.perf_counter() - t0, synth.snr(prob, prob["blended"])))
+    kw = geometry_kwargs(g)
+    t0 = time.perf_counter()
+    pm = L.Problem(**kw)
+    t_setup = time.perf_counter() - t0
+    pm.set_data(prob["blended"])
+    n_windows = pm.n_windows
+    by = pm.bytes()
+    b_fk = 3 * by["M"] + 2 * by["T"]
+    log("setup %.2f s; %d windows; M %.3f GB, T %.3f GB, R %.3f GB; B_FK %.3f GB" %
+        (t_setup, pm.n_windows, by["M"] / 1e9, by["T"] / 1e9, by["R"] / 1e9, b_fk / 1e9))
+
+    # state for the timed iterations: a short inversion, then fixed-threshold steps
+    pm.run(1.0, 3)
+    lamb = 0.05 * L.last_lambda0()
+    pm.iterate(args.warmup, lamb)
+    t0 = time.perf_counter()
+    pm.iterate(args.steps, lamb)  # synchronises at the end
+    el = time.perf_counter() - t0
+    ms_per_step = el / args.steps * 1e3
+    log("timed %d iterations: %.3f ms/iteration" % (args.steps, ms_per_step))
+
+    # per-kernel device times (HIP events on the library stream)
+    L.timing(True)
+    pm.iterate(args.steps, lamb)
+    L.timing(False)
+    ktimes = {}
+    for k in FK_KERNELS + ["blend", "wavelet_fwd", "wavelet_adj"]:
+        ms, n = L.timing_get(k)
+        if n:
+            ktimes[k] = ms / n
+            log("  %-12s %9.4f ms/launch" % (k, ms / n))
+    fk_ms = sum(ktimes.get(k, 0.0) for k in FK_KERNELS)
+    achieved = b_fk / (fk_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="FK operator (%s)" % "+".join(FK_KERNELS),
+                    algorithmic_bytes=b_fk, ms_per_launch=round(fk_ms, 4),
+                    per_kernel_ms={k: round(v, 4) for k, v in ktimes.items()})
+
+    value, n20, snr20, sweep_res = None, None, None, []
+    if not args.no_snr:
+        sweep = [int(s) for s in args.sweep.split(",") if s]
+        value, n20, snr20, sweep_res = time_to_snr(pm, prob, args.target_db, sweep, synth)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(prob, n20 if n20 else prob["n_its"], n_windows)
+        except Exception as e:  # the baseline must not hide the GPU result
+            log("cpu baseline failed: %r" % (e,))
+    pm.close()
+
+    line = dict(
+        metric="s to 20 dB deblend SNR (plain vs learned warm start, 1 GPU); FK-op HBM GB/s",
+        value=None if value is None else round(value, 4),
+        unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=round(ms_per_step, 4), higher_is_better=False, scaling="weak",
+        vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(workload=prob["name"], n_its_to_target=n20,
+                    snr_db=None if snr20 is None else round(snr20, 2),
+                    target_db=args.target_db, setup_s=round(t_setup, 3),
+                    sweep=[(n, round(t, 4), round(s, 2)) for n, t, s in sweep_res],
+                    parallelism="1 GPU"),
+        roofline=roofline, cpu_baseline=cpu)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -84,8 +84,10 @@ int agdx_adjoint(AgdProblem *p, float const *rec, float *g);
  * Leaves the final model on the device. */
 int agdx_run(AgdProblem *p, float initial_factor, int n_its, int print_freq,
              float const *x0);
-/* Writes the forward of the current model into the patch data
- * (set_output, model.c:1425-1515). */
+/* Forward of the current model assembled into the device copy of the patch
+ * data (set_output, model.c:1425-1515), without copying it to the host. */
+int agdx_finish(AgdProblem *p);
+/* agdx_finish, then copies the patches into data. */
 int agdx_get_output(AgdProblem *p, float *const *data);
 /* Copies the current model to host (model_len complex). */
 int agdx_get_model(AgdProblem *p, float *x);
