@@ -1014,7 +1014,7 @@ static void time_key(AgdWin const *W, int *m1, int *m2) {
   size_t i;
   *m1 = 0;
   *m2 = 0;
-  if (W->Pt % 2) return;
+  if (W->Pt % 2 || W->Wt % 2) return; /* specialised kernels: even Pt and Wt */
   for (i = 0; i < sizeof(kTimeSizes) / sizeof(kTimeSizes[0]); i++)
     if (kTimeSizes[i][0] * kTimeSizes[i][1] == W->Pt / 2) {
       *m1 = kTimeSizes[i][0];
@@ -1100,8 +1100,21 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
   for (c = 0; c < (int)tc.n; c++) {
     AgdClass *cl = (AgdClass *)tc.p + c;
     int tb;
-    /* traces per block: as many as fit the LDS budget, up to 16 */
-    for (tb = 16; tb >= 1; tb /= 2) {
+    /* traces per block: as many as fit the LDS budget, up to tb_max
+     * (specialised kernels: AGD_TIME_TB, default 8, with AGD_TIME_THREADS
+     * threads, default 256) */
+    int tb_max = 16;
+    int nthr = 256;
+    if (cl->p0) {
+      char const *e1 = getenv("AGD_TIME_TB");
+      char const *e2 = getenv("AGD_TIME_THREADS");
+      tb_max = e1 ? atoi(e1) : 8;
+      nthr = e2 ? atoi(e2) : 256;
+      if (tb_max < 1 || tb_max > 16) tb_max = 8;
+      if (nthr != 64 && nthr != 128 && nthr != 256) nthr = 256;
+    }
+    cl->pad = nthr;
+    for (tb = tb_max; tb >= 1; tb /= 2) {
       size_t need = 0;
       for (w = 0; w < p->n_win; w++) {
         AgdWin const *W = &p->win[w];
@@ -1110,7 +1123,11 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
         time_key(W, &a, &b);
         if (a != cl->p0 || b != cl->p1) continue;
         if (a) {
-          l = (size_t)2 * tb * (W->Pt / 2 + 1) * 8 + (size_t)2 * tb * W->Wt * 4;
+          /* spectra + packed buffers, then the larger of the forward's carry,
+           * taper and scales and the adjoint's taper, scales and entries */
+          size_t const fwd = (size_t)tb * W->half * 4 + (size_t)W->Wt * 4 + 16 * 4;
+          size_t const adj = (size_t)W->Wt * 4 + 16 * 4 + 16 * 8 + 32 * 4;
+          l = (size_t)2 * tb * (W->Pt / 2 + 1) * 8 + (fwd > adj ? fwd : adj) + 64;
         } else {
           int const M = (W->Pt % 2 == 0 && W->Pt >= 4) ? W->Pt / 2 : W->Pt;
           l = (size_t)tb * W->Ft * 8 + (size_t)2 * tb * M * 8 + (size_t)2 * tb * W->Wt * 4;

@@ -1056,6 +1056,9 @@ __device__ __forceinline__ void fft2phase(float2 *Z, float2 *O, int tb,
   __syncthreads();
 }
 
+// Forward time transform for even Wt (Wt = 2 half): spectra of one time
+// window per iteration; overlap-add through a carry of the second half of
+// each windowed trace. Launched with 64, 128 or 256 threads.
 template <int M1, int M2>
 __global__ __launch_bounds__(NTHR) void k_fwd_time_t(
     AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
@@ -1064,27 +1067,28 @@ __global__ __launch_bounds__(NTHR) void k_fwd_time_t(
     float2 const *__restrict__ U, float *__restrict__ T) {
   extern __shared__ float2 sm[];
   constexpr int M = M1 * M2;
-  constexpr int Pt = 2 * M;
   constexpr int LDL = M + 1;
   int const bid = block0 + blockIdx.x;
   AgdWin const &w = wins[blocks[2 * bid]];
   int const t0 = blocks[2 * bid + 1];
   int const ntr = w.ntr;
   int const nt = min(tb, ntr - t0);
-  int const Wt = w.Wt, half = w.half, nconv = w.nconv;
+  int const half = w.half, nconv = w.nconv, ntw = w.ntw;
   float2 *SP = sm;
   float2 *Z = SP + tb * LDL;
-  float *acc = (float *)(Z + tb * LDL);
-  float *nxt = acc + tb * Wt;
+  float *carry = (float *)(Z + tb * LDL);
+  float *ttl = carry + tb * half;  // time taper, 2*half
+  float *scl = ttl + 2 * half;     // per-trace scale
   float2 const *rootsM = roots + plans[w.tplan].roots_off;
   float2 const *rootsP = roots + plans[w.tplan_full].roots_off;
-  float const *tt = tapers + w.tt_off;
-  float const *sc = tscale + w.trace0 + t0;
   float *Tb = T + w.t_off + (int64_t)t0 * nconv;
-  for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) acc[i] = 0.f;
-  for (int tw = 0; tw < w.ntw; ++tw) {
-    float2 const *u = U + w.u_off + (int64_t)tw * (M + 1) * ntr + t0;
-    for (int i = threadIdx.x; i < (M + 1) * tb; i += blockDim.x) {
+  int const ntile = tb * LDL;
+  for (int i = threadIdx.x; i < tb * half; i += blockDim.x) carry[i] = 0.f;
+  for (int i = threadIdx.x; i < 2 * half; i += blockDim.x) ttl[i] = tapers[w.tt_off + i];
+  for (int i = threadIdx.x; i < tb; i += blockDim.x) scl[i] = i < nt ? tscale[w.trace0 + t0 + i] : 0.f;
+  for (int tw = 0; tw < ntw; ++tw) {
+    float2 const *u = U + w.u_off + (int64_t)tw * LDL * ntr + t0;
+    for (int i = threadIdx.x; i < ntile; i += blockDim.x) {
       int const k = i / tb;
       int const j = i - k * tb;
       SP[j * LDL + k] = j < nt ? u[(int64_t)k * ntr + j] : make_float2(0.f, 0.f);
@@ -1110,42 +1114,40 @@ __global__ __launch_bounds__(NTHR) void k_fwd_time_t(
     }
     __syncthreads();
     fft2phase<M1, M2, +1>(Z, SP, tb, rootsM);
-    for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) {
-      int const j = i / Wt;
-      int const s = i - j * Wt;
+    // first half: complete samples tw*half + s
+    for (int i = threadIdx.x; i < tb * half; i += blockDim.x) {
+      int const j = i / half;
+      int const s = i - j * half;
       if (j >= nt) continue;
       float2 const z = SP[j * LDL + (s >> 1)];
       float const y = (s & 1) ? z.y : z.x;
-      acc[i] += y * sc[j] * tt[s];
+      Tb[(int64_t)j * nconv + tw * half + s] = carry[i] + y * scl[j] * ttl[s];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < tb * Wt; i += blockDim.x) {
-      int const j = i / Wt;
-      int const s = i - j * Wt;
-      if (s < half) {
-        int const t = tw * half + s;
-        if (j < nt && t < nconv) Tb[(int64_t)j * nconv + t] = acc[i];
-      }
-      nxt[i] = (s + half < Wt) ? acc[i + half] : 0.f;
+    // second half: new carry
+    for (int i = threadIdx.x; i < tb * half; i += blockDim.x) {
+      int const j = i / half;
+      int const s = i - j * half + half;
+      float2 const z = SP[j * LDL + (s >> 1)];
+      float const y = (s & 1) ? z.y : z.x;
+      carry[i] = y * scl[j] * ttl[s];
     }
     __syncthreads();
-    float *tmp = acc;
-    acc = nxt;
-    nxt = tmp;
   }
-  for (int i = threadIdx.x; i < tb * (Wt - half); i += blockDim.x) {
-    int const j = i / (Wt - half);
-    int const s = i - j * (Wt - half);
-    int const t = w.ntw * half + s;
-    if (j < nt && t < nconv) Tb[(int64_t)j * nconv + t] = acc[j * Wt + s];
+  for (int i = threadIdx.x; i < tb * half; i += blockDim.x) {
+    int const j = i / half;
+    int const s = i - j * half;
+    int const t = ntw * half + s;
+    if (j < nt && t < nconv) Tb[(int64_t)j * nconv + t] = carry[i];
   }
-  int const tail0 = w.ntw * half + (Wt - half);
+  int const tail0 = (ntw + 1) * half;
   for (int j = 0; j < nt; ++j)
     for (int t = tail0 + threadIdx.x; t < nconv; t += blockDim.x)
       Tb[(int64_t)j * nconv + t] = 0.f;
-  (void)Pt;
 }
 
+// Adjoint time transform for even Wt: record (or wavelet-adjoint traces)
+// -> windowed, tapered time windows -> R2C (x2 interior bins) -> V.
 template <int M1, int M2>
 __global__ __launch_bounds__(NTHR) void k_adj_time_t(
     AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
@@ -1162,38 +1164,55 @@ __global__ __launch_bounds__(NTHR) void k_adj_time_t(
   int const t0 = blocks[2 * bid + 1];
   int const ntr = w.ntr;
   int const nt = min(tb, ntr - t0);
-  int const Wt = w.Wt, half = w.half, nconv = w.nconv;
+  int const Wt = w.Wt, half = w.half, nconv = w.nconv, ntw = w.ntw;
   float2 *Z = sm;
   float2 *SP = Z + tb * LDL;
+  float *ttl = (float *)(SP + tb * LDL);  // Wt
+  float *scl = ttl + Wt;                  // tb
+  int64_t *erec = (int64_t *)(scl + 16);  // tb record starts (8-byte aligned: Wt even)
+  int *ets = (int *)(erec + 16);          // tb trace starts
+  int *en = ets + 16;                     // tb lengths
   float2 const *rootsM = roots + plans[w.tplan].roots_off;
   float2 const *rootsP = roots + plans[w.tplan_full].roots_off;
-  float const *tt = tapers + w.tt_off;
-  for (int tw = 0; tw < w.ntw; ++tw) {
+  for (int i = threadIdx.x; i < Wt; i += blockDim.x) ttl[i] = tapers[w.tt_off + i];
+  for (int i = threadIdx.x; i < tb; i += blockDim.x) {
+    int const g = w.trace0 + t0 + i;
+    scl[i] = i < nt ? tscale[g] : 0.f;
+    if (i < nt) {
+      AgdEntry const e = went[g];
+      erec[i] = e.rec;
+      ets[i] = e.trace_start;
+      en[i] = e.n;
+    } else {
+      erec[i] = 0;
+      ets[i] = 0;
+      en[i] = 0;
+    }
+  }
+  __syncthreads();
+  for (int tw = 0; tw < ntw; ++tw) {
+    // windowed samples into the packed real buffer (zero padded to Pt)
     for (int i = threadIdx.x; i < tb * Pt; i += blockDim.x) {
       int const j = i / Pt;
       int const s = i - j * Pt;
-      float v = 0.f;
+      float x = 0.f;
       if (j < nt && s < Wt) {
-        int const g = w.trace0 + t0 + j;
         int const t = tw * half + s;
-        float x;
         if (Tin != nullptr) {
           x = Tin[w.t_off + (int64_t)(t0 + j) * nconv + t];
         } else {
-          AgdEntry const e = went[g];
-          int const i2 = t - e.trace_start;
-          x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+          int const i2 = t - ets[j];
+          x = (i2 >= 0 && i2 < en[j]) ? rec[erec[j] + i2] : 0.f;
         }
-        v = x * tscale[g] * tt[s];
+        x *= scl[j] * ttl[s];
       }
-      float *zf = (float *)(Z + j * LDL);
-      zf[s] = v;
+      ((float *)(Z + j * LDL))[s] = x;
     }
     __syncthreads();
     fft2phase<M1, M2, -1>(Z, SP, tb, rootsM);
     constexpr int kmax2 = (Pt - 1) / 2;
-    float2 *vout = V + w.u_off + (int64_t)tw * (M + 1) * ntr + t0;
-    for (int i = threadIdx.x; i < (M + 1) * tb; i += blockDim.x) {
+    float2 *vout = V + w.u_off + (int64_t)tw * LDL * ntr + t0;
+    for (int i = threadIdx.x; i < LDL * tb; i += blockDim.x) {
       int const k = i / tb;
       int const j = i - k * tb;
       if (j >= nt) continue;
@@ -1344,7 +1363,7 @@ int agdg_fwd_time(AgdDev const *g) {
     bool done = false;
 #define FT_CASE(a, b)                                                                   \
     if (!done && c.p0 == a && c.p1 == b) {                                              \
-      if (launch("fwd_time", k_fwd_time_t<a, b>, dim3(c.count), dim3(NTHR), c.lds,        \
+      if (launch("fwd_time", k_fwd_time_t<a, b>, dim3(c.count), dim3(c.pad), c.lds,       \
                  g->win, g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots,  \
                  g->tapers, g->tscale, (float2 const *)g->U, g->T))                     \
         return 1;                                                                       \
@@ -1388,7 +1407,7 @@ int agdg_adj_time(AgdDev const *g, int from_record) {
     bool done = false;
 #define AT_CASE(a, b)                                                                   \
     if (!done && c.p0 == a && c.p1 == b) {                                              \
-      if (launch("adj_time", k_adj_time_t<a, b>, dim3(c.count), dim3(NTHR), c.lds,        \
+      if (launch("adj_time", k_adj_time_t<a, b>, dim3(c.count), dim3(c.pad), c.lds,       \
                  g->win, g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots,  \
                  g->tapers, g->tscale, g->went, (float const *)g->r, Tin,               \
                  (float2 *)g->V))                                                       \

@@ -74,7 +74,7 @@ typedef struct {
                    time: p0 = M1, p1 = M2 with M1*M2 = Pt/2 */
   int begin, count;
   int tb;       /* time classes: traces per block */
-  int pad;
+  int pad;      /* time classes: threads per block */
   size_t lds;
 } AgdClass;
 
