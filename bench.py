@@ -13,9 +13,12 @@ Reported (one JSON line):
                each n_its is a separate run)
   ms_per_step  mean time of the K timed iterations
   roofline     the FK operator: algorithmic bytes 3M + 2T per iteration
-               (M model bytes, T trace-buffer bytes) over the summed average
-               durations of its four kernels, timed with HIP events on the
-               library stream
+               (M model bytes, T trace-buffer bytes) over the device time per
+               iteration of its kernels (the spatial and time transforms, the
+               update and the shrink, fused or not), timed with HIP events on
+               the library stream; traffic is the HBM bytes per iteration of
+               the same kernels from the FETCH_SIZE / WRITE_SIZE counter runs
+               summarised in profiles/, when that summary matches the workload
   cpu_baseline the oracle (float32, OpenMP over windows, the reference's
                algorithm with a project FFT) timed on this host on a bounded
                sample and scaled to the same n_its
@@ -32,7 +35,9 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-FK_KERNELS = ["fwd_spatial", "fwd_time", "adj_time", "adj_spatial"]
+FK_KERNELS = ["fwd_spatial", "fwd_time", "adj_time", "adj_spatial", "shrink"]
+OTHER_KERNELS = ["blend", "wavelet_fwd", "wavelet_adj", "norm2"]
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "fk_operator_c1_hbm_traffic.md")
 
 
 def log(*a):
@@ -55,6 +60,37 @@ def make_workload(cfg):
         # shots every 4 s +- 1 s: two-fold blending, as in the 2-D workload
         return synth.config1(interval_s=4.0, dither_s=1.0)
     return synth.CONFIGS[cfg]()
+
+
+def traffic_from_summary(workload):
+    """HBM bytes per iteration of the FK kernels from the committed counter
+    summary (a line 'traffic_bytes_per_iteration <workload, spaces as _>
+    <bytes>'), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                parts = line.split()
+                if (len(parts) == 3 and parts[0] == "traffic_bytes_per_iteration"
+                        and parts[1] == workload.replace(" ", "_")):
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
+def calibration_copy(mib):
+    """A plain device copy of mib MiB (read + write), the known-size kernel the
+    FETCH_SIZE / WRITE_SIZE counter runs are scaled by."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("calibration copy: 3 x %d MiB" % mib)
+    del a, b
 
 
 def time_to_snr(pm, prob, target_db, sweep, synth):
@@ -138,6 +174,8 @@ def main():
     ap.add_argument("--sweep", type=str, default="50,100,150,200,300,400,600,800,1200")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-snr", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -148,6 +186,9 @@ def main():
     from agdeblend_amd import agdeblend as agd
     from agdeblend_amd import lib as L
     from agdeblend_amd import synth
+
+    if args.calib_copy_mib > 0:
+        calibration_copy(args.calib_copy_mib)
 
     prob = make_workload(args.config)
     g = prob["geometry"]
@@ -180,24 +221,42 @@ def main():
     L.timing(True)
     pm.iterate(args.steps, lamb)
     L.timing(False)
-    ktimes = {}
-    for k in FK_KERNELS + ["blend", "wavelet_fwd", "wavelet_adj"]:
+    per_launch, per_it = {}, {}
+    for k in FK_KERNELS + OTHER_KERNELS:
         ms, n = L.timing_get(k)
         if n:
-            ktimes[k] = ms / n
-            log("  %-12s %9.4f ms/launch" % (k, ms / n))
-    fk_ms = sum(ktimes.get(k, 0.0) for k in FK_KERNELS)
+            per_launch[k] = ms / n
+            per_it[k] = ms / args.steps
+            log("  %-12s %9.4f ms/launch  %9.4f ms/iteration" % (k, ms / n, ms / args.steps))
+    fk_ms = sum(per_it.get(k, 0.0) for k in FK_KERNELS)
     achieved = b_fk / (fk_ms * 1e-3) / 1e9
+    traffic = traffic_from_summary(prob["name"])
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="FK operator (%s)" % "+".join(FK_KERNELS),
-                    algorithmic_bytes=b_fk, ms_per_launch=round(fk_ms, 4),
-                    per_kernel_ms={k: round(v, 4) for k, v in ktimes.items()})
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    kernel="FK operator per iteration (%s)" %
+                           "+".join(k for k in FK_KERNELS if k in per_it),
+                    algorithmic_bytes=b_fk, ms_per_iteration=round(fk_ms, 4),
+                    per_kernel_ms_per_launch={k: round(v, 4) for k, v in per_launch.items()},
+                    per_kernel_ms_per_iteration={k: round(v, 4) for k, v in per_it.items()})
 
     value, n20, snr20, sweep_res = None, None, None, []
     if not args.no_snr:
         sweep = [int(s) for s in args.sweep.split(",") if s]
         value, n20, snr20, sweep_res = time_to_snr(pm, prob, args.target_db, sweep, synth)
+
+    e2e = None
+    if not args.no_snr and not args.no_e2e and n20:
+        # the whole agd_deblend call from host buffers (setup, H2D, loop, D2H)
+        wav = {k: g[k] for k in ("wavelet_idxs", "wavelets") if k in g}
+        t0 = time.perf_counter()
+        out = agd.deblend(volumes=g["volumes"], window_shapes=g["window_shapes"],
+                          coords=g["coords"], shottimes=g["shottimes"],
+                          channels=g["channels"], trace_types=g["trace_types"],
+                          initial_factor=1.0, n_its=n20, print_freq=0,
+                          data=prob["blended"], **wav)
+        e2e = time.perf_counter() - t0
+        log("agd_deblend end to end (n_its %d): %.3f s, SNR %.2f dB" %
+            (n20, e2e, synth.snr(prob, out)))
 
     cpu = None
     if not args.no_cpu:
@@ -216,6 +275,7 @@ def main():
         config=dict(workload=prob["name"], n_its_to_target=n20,
                     snr_db=None if snr20 is None else round(snr20, 2),
                     target_db=args.target_db, setup_s=round(t_setup, 3),
+                    end_to_end_s=None if e2e is None else round(e2e, 3),
                     sweep=[(n, round(t, 4), round(s, 2)) for n, t, s in sweep_res],
                     parallelism="1 GPU"),
         roofline=roofline, cpu_baseline=cpu)
