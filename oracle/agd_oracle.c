@@ -1352,6 +1352,18 @@ err:
 /* agdeblend.c:511-649. Optional x0 (model_len complex) replaces the zero
  * start and skips the initial step when non-NULL (warm-start hook); stats
  * receives the first threshold and stepsize when non-NULL. */
+/* Threshold schedule (agdeblend.c:526-533, :623-627): lamb decays by
+ * `decay` per iteration until first_linear_it, then falls linearly to zero
+ * at the last iteration; slope is per unit of the first threshold. */
+static void lamb_schedule(real f, int n_its, real *decay, int *first_linear_it,
+                          real *slope) {
+  *decay = REXP(RLOG(O_TARGET_LAMB / f) / (real)n_its);
+  *first_linear_it =
+      (int)((RLOG(*decay) * ((real)n_its - (real)1) + (real)1) / RLOG(*decay));
+  *slope = f * RPOW(*decay, (real)*first_linear_it) /
+           (real)(n_its - 1 - *first_linear_it);
+}
+
 static int deblend_impl(int np, int const *volumes, int const *n_dims,
                         int const *const *wshape, int const *const *coords,
                         int const *const *shape, long const *const *shot,
@@ -1359,11 +1371,8 @@ static int deblend_impl(int np, int const *volumes, int const *n_dims,
                         int const *wl, int const *const *widx,
                         real const *const *wav, real f, int n_its,
                         int print_freq, real *const *data, double *stats) {
-  real const decay = REXP(RLOG(O_TARGET_LAMB / f) / (real)n_its);
-  int const first_linear_it =
-      (int)((RLOG(decay) * ((real)n_its - (real)1) + (real)1) / RLOG(decay));
-  real slope = f * RPOW(decay, (real)first_linear_it) /
-               (real)(n_its - 1 - first_linear_it);
+  real decay, slope;
+  int first_linear_it;
   omodel *m;
   ocpx *x;
   real *d, *r;
@@ -1372,6 +1381,7 @@ static int deblend_impl(int np, int const *volumes, int const *n_dims,
   if (check_deblend(np, volumes, n_dims, wshape, coords, shape, shot, ch, tt,
                     wl, widx, wav, f, n_its, data))
     goto err0;
+  lamb_schedule(f, n_its, &decay, &first_linear_it, &slope);
   m = model_new(np, volumes, n_dims, wshape, coords, shape, shot, ch, tt, wl,
                 widx, wav);
   if (m == NULL) goto err0;
@@ -1692,6 +1702,25 @@ EXPORT void OSYM(window_wavelet)(void *h, int w, real dir, real *buf) {
   scratch_alloc(m, &s);
   wavelet_op(m, &m->win[w], buf, dir, s.c, s.f);
   scratch_free(&s);
+}
+
+/* The thresholds of iterations 0..n_its-1 for a first threshold lamb0
+ * (the sequence deblend_impl applies) */
+EXPORT void OSYM(lamb_schedule)(real f, int n_its, real lamb0, real *out) {
+  real decay, slope, lamb;
+  int first_linear_it, it;
+  lamb_schedule(f, n_its, &decay, &first_linear_it, &slope);
+  slope *= lamb0;
+  lamb = lamb0 * f;
+  out[0] = lamb;
+  for (it = 1; it < n_its; it++) {
+    if (it < first_linear_it) {
+      lamb *= decay;
+    } else {
+      lamb = (real)(n_its - 1 - it) * slope;
+    }
+    out[it] = lamb;
+  }
 }
 
 /* Stand-alone pieces for known-answer tests */

@@ -73,6 +73,7 @@ def _declare(L, key):
     fn("window_wavelet", None, vp, ci, real, vp)
     fn("add_intervals", ci, ci, vp, vp, vp, vp)
     fn("taper_window", None, ci, ci, ci, vp)
+    fn("lamb_schedule", None, real, ci, real, vp)
     fn("fft_c2c", None, ci, ci, vp)
     fn("fft_r2c", None, ci, vp, vp)
     fn("fft_c2r", None, ci, vp, vp)
@@ -153,6 +154,13 @@ def fft_r2c_nd(x, double=True):
 def taper_window(n, left, right, double=False):
     out = np.zeros(n, _rdt(double))
     lib(double).taper_window(n, int(left), int(right), _p(out))
+    return out
+
+
+def lamb_schedule(f, n_its, lamb0, double=False):
+    """Thresholds of iterations 0..n_its-1 as the oracle's deblend applies them."""
+    out = np.zeros(n_its, _rdt(double))
+    lib(double).lamb_schedule(f, int(n_its), lamb0, _p(out))
     return out
 
 

@@ -670,3 +670,87 @@ def test_agd_blend_rejects_bad_output_trace():
     with pytest.raises(RuntimeError):
         O.blend([1], [3], [[25]], [[123]], [[LIVE]], [np.array([1., 2, 3])], 0, 0,
                 [1], [3], [[25]], [[123]], [[BAD]])
+
+
+# ---------------------------------------------------------------------------
+# Threshold schedule (agdeblend.c:526-533, :623-627), closed form
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n_its,decay,first", [(100, 0.933254, 84), (150, 0.954993, 127),
+                                                (200, 0.966051, 170)])
+def test_lamb_schedule_closed_form(n_its, decay, first):
+    # lamb_it = lamb0 d^it up to the switch, then linear to 0 at the last
+    # iteration, continuous at the switch; d^n = 1e-3 and the switch is
+    # floor(n - 1 + n / ln(1e-3)) (the point where the line through zero at
+    # n - 1 is tangent-continuous with the decay, rounded down)
+    s = O.lamb_schedule(1.0, n_its, 2.0, double=True)
+    d = math.exp(math.log(1e-3) / n_its)
+    assert abs(d - decay) < 1e-6
+    k = math.floor(n_its - 1 + n_its / math.log(1e-3))
+    assert k == first
+    for it in range(n_its):
+        if it < k:
+            exp = 2.0 * d ** it
+        else:
+            exp = 2.0 * d ** k * (n_its - 1 - it) / (n_its - 1 - k)
+        assert abs(s[it] - exp) <= 1e-12 * 2.0, it
+    assert s[-1] == 0.0
+    f32 = O.lamb_schedule(1.0, n_its, 2.0, double=False)
+    assert np.allclose(f32, s, rtol=2e-5, atol=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# Complex soft threshold (model.c:1392-1423), closed form
+# ---------------------------------------------------------------------------
+def test_shrink_closed_form():
+    m = O.Model([0], [2], [[8, 8]], [[0]], [[16, 16]], [np.arange(16) * 1000],
+                [np.zeros(16)], [np.zeros(16)], double=True)
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(m.model_len) + 1j * rng.standard_normal(m.model_len))
+    x[:4] = [0, 0.5, 0.5j, 0.3 + 0.4j]  # |x| = 0, 0.5 (= lamb), 0.5, 0.5
+    lamb = 0.5
+    got = m.shrink(lamb, x)
+    a = np.abs(x)
+    exp = np.where(a <= lamb, 0, x * (1 - lamb / np.where(a > 0, a, 1)))
+    assert np.allclose(got, exp, rtol=1e-13, atol=1e-15)
+    assert np.all(got[:4] == 0)
+
+
+# ---------------------------------------------------------------------------
+# Output assembly (model.c:1425-1515): Hann partition of unity
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("shape,wshape", [([40, 300], [16, 64]), ([24, 40, 200], [8, 16, 64]),
+                                          ([12, 12, 20, 130], [8, 8, 8, 32])])
+def test_output_partition_of_unity(shape, wshape):
+    """Windows of the nominal length overlap by half, their tapers sum to one,
+    so the output of the model x = R2C(pad(y_w)) / (scaler prod P) built from
+    the windowed pieces y_w of any data y is y itself."""
+    ntr = int(np.prod(shape[:-1]))
+    nd = len(shape)
+    m = O.Model([0], [nd], [wshape], [[0] * (nd - 1)], [shape],
+                [np.arange(ntr) * 10000], [np.arange(ntr)], [np.zeros(ntr)], double=True)
+    rng = np.random.default_rng(11)
+    y = rng.standard_normal(shape)
+    yt = y.reshape(ntr, shape[-1])
+    x = m.zeros_model()
+    for w, wi in enumerate(m.windows):
+        assert wi["W"][:-1] == list(wshape[:-1])  # nominal lengths everywhere
+        tr = m.window_traces(w, shape[:-1])
+        Wt = wi["W"][-1]
+        half = Wt // 2
+        Psp = wi["P"][:-1]
+        Pt = wi["P"][-1]
+        blk = wi["npad_tr"] * wi["Ft"]
+        for tw in range(wi["ntw"]):
+            seg = np.zeros((len(tr), Wt))
+            for s in range(Wt):
+                t = tw * half + s - half
+                if 0 <= t < shape[-1]:
+                    seg[:, s] = yt[tr, t]
+            pad = np.zeros(list(Psp) + [Pt])
+            pad[tuple(slice(0, n) for n in wi["W"][:-1]) + (slice(0, Wt),)] = \
+                seg.reshape(list(wi["W"][:-1]) + [Wt])
+            X = np.fft.rfftn(pad) / (wi["scaler"] * np.prod(wi["P"]))
+            o = wi["model_off"] + tw * blk
+            x[o:o + blk] = X.reshape(-1)
+    out = m.output(x, [shape])[0]
+    assert np.allclose(out, y, rtol=0, atol=1e-12), np.abs(out - y).max()
