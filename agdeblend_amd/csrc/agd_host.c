@@ -1003,12 +1003,18 @@ static void spatial_key(AgdWin const *W, int *p0, int *p1) {
   }
   *p0 = 0;
   *p1 = 0;
+  /* the specialised kernels take the window lengths as W = 4 P / 5 */
+  for (i = 0; i < (size_t)W->nsp; i++)
+    if (5 * W->W[i] != 4 * W->P[i]) return;
   for (i = 0; i < sizeof(kSpatialSizes) / sizeof(kSpatialSizes[0]); i++)
     if (kSpatialSizes[i][0] == a && kSpatialSizes[i][1] == b) {
       *p0 = a;
       *p1 = b;
     }
 }
+
+/* window length of a specialised padded size (W = 4 P / 5; 1 for P = 1) */
+static int spec_len(int P) { return P == 1 ? 1 : P * 4 / 5; }
 
 static void time_key(AgdWin const *W, int *m1, int *m2) {
   size_t i;
@@ -1068,8 +1074,14 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
       spatial_key(W, &a, &b);
       if (a != cl->p0 || b != cl->p1) continue;
       if (a) {
-        nk = (int)(40960 / ((size_t)a * (b + 1) * 8));
-        if (nk > 64) nk = 64;
+        /* one column transform per thread of a 256-thread block (P1 per
+         * plane), within a 48 KB LDS tile of W0 rows of P1 + 1 per plane;
+         * AGD_SPATIAL_NK overrides */
+        size_t const plane = (size_t)spec_len(a) * (b + 1) * 8;
+        char const *e = getenv("AGD_SPATIAL_NK");
+        nk = e ? atoi(e) : 256 / b;
+        if (nk < 1) nk = 1;
+        if ((size_t)nk * plane > 48 * 1024) nk = (int)(48 * 1024 / plane);
       } else {
         nk = (int)(LDS_BUDGET / (2 * (size_t)W->S * 8));
       }
@@ -1089,7 +1101,7 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
         }
       }
       if (a) {
-        size_t const l = (size_t)nk * a * (b + 1) * 8;
+        size_t const l = (size_t)nk * spec_len(a) * (b + 1) * 8;
         if (l > cl->lds) cl->lds = l;
       } else if ((size_t)2 * nk * W->S * 8 > lds_sp) {
         lds_sp = (size_t)2 * nk * W->S * 8;
@@ -1780,7 +1792,7 @@ static int forward_record(AgdProblem *p, int resid) {
 
 /* model_adjoint_update of dev.r into dev.X (modes of agdg_adj_spatial) */
 static int adjoint_update(AgdProblem *p, float alpha, float lamb, int mode) {
-  int const fuse = p->fuse && (mode == 0 || mode == 2);
+  int const fuse = p->fuse && mode == 0;
   if (p->use_wavelets) {
     if (agdg_wavelet(&p->dev, 1)) return 1;
     if (agdg_adj_time(&p->dev, 0)) return 1;

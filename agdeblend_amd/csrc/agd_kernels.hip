@@ -840,37 +840,55 @@ __global__ __launch_bounds__(NTHR) void k_extract(
 // ---------------------------------------------------------------------------
 // Size-specialised spatial transforms (one or two space dims).
 // A task holds nk time-frequency planes of one (window, time window); each
-// plane is P0 x P1 (P0 = 1 for one space dim). Column transforms (length
-// P0) run from global memory into registers, row transforms (length P1)
-// through one LDS tile with padded rows; every FFT is a register codelet.
+// plane is P0 x P1 (P0 = 1 for one space dim) with W0 x W1 traces. The
+// specialised sizes are multiples of 5, for which the window lengths are
+// W = 4 P / 5 exactly (P = ceil(1.25 W)), so every loop bound is a
+// compile-time constant. Column transforms (length P0) run from global
+// memory into registers, row transforms (length P1) through one LDS tile
+// holding the W0 trace rows of each plane (padded rows of P1 + 1); every
+// FFT is a register codelet.
 // ---------------------------------------------------------------------------
-template <int P0, int P1>
-__device__ __forceinline__ void win_w01(AgdWin const &w, int &W0, int &W1) {
-  if (P0 == 1) {
-    W0 = 1;
-    W1 = w.W[0];
-  } else {
-    W0 = w.W[0];
-    W1 = w.W[1];
+template <int P>
+__host__ __device__ constexpr int win_len() {
+  return P == 1 ? 1 : P * 4 / 5;
+}
+
+// row transforms of the W0 rows of each plane, in place: NIN inputs
+// (zero padded to P1), NOUT outputs kept (S = +1: inverse of the forward
+// operator)
+template <int P0, int P1, int S, int NIN, int NOUT>
+__device__ __forceinline__ void rows_fft(float2 *sm, int nk) {
+  constexpr int LD = P1 + 1;
+  constexpr int W0 = win_len<P0>();
+  for (int i = threadIdx.x; i < nk * W0; i += blockDim.x) {
+    float2 *row = sm + i * LD;
+    float2 v[P1];
+#pragma unroll
+    for (int c = 0; c < P1; ++c) v[c] = (c < NIN) ? row[c] : make_float2(0.f, 0.f);
+    agdfft::fft<P1, S>(v);
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) row[c] = v[c];
   }
 }
 
-// row transforms of rows [0, nrows) of each plane, in place, keeping nout
-// outputs (S = +1: inverse of the forward operator)
-template <int P0, int P1, int S>
-__device__ __forceinline__ void rows_fft(float2 *sm, int nk, int nrows, int nin, int nout) {
+// LDS rows of the nk planes -> the window traces of the planes in U
+// ([k][W0][W1] contiguous), or back
+template <int P0, int P1>
+__device__ __forceinline__ void planes_to_global(float2 const *sm, float2 *g, int nk) {
   constexpr int LD = P1 + 1;
-  for (int i = threadIdx.x; i < nk * nrows; i += blockDim.x) {
-    int const k = i / nrows;
-    int const r = i - k * nrows;
-    float2 *row = sm + (k * P0 + r) * LD;
-    float2 v[P1];
-#pragma unroll
-    for (int c = 0; c < P1; ++c) v[c] = (c < nin) ? row[c] : make_float2(0.f, 0.f);
-    agdfft::fft<P1, S>(v);
-#pragma unroll
-    for (int c = 0; c < P1; ++c)
-      if (c < nout) row[c] = v[c];
+  constexpr int W1 = win_len<P1>();
+  for (int i = threadIdx.x; i < nk * win_len<P0>() * W1; i += blockDim.x) {
+    int const row = i / W1;
+    g[i] = sm[row * LD + (i - row * W1)];
+  }
+}
+template <int P0, int P1>
+__device__ __forceinline__ void planes_to_lds(float2 const *g, float2 *sm, int nk) {
+  constexpr int LD = P1 + 1;
+  constexpr int W1 = win_len<P1>();
+  for (int i = threadIdx.x; i < nk * win_len<P0>() * W1; i += blockDim.x) {
+    int const row = i / W1;
+    sm[row * LD + (i - row * W1)] = g[i];
   }
 }
 
@@ -881,10 +899,10 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
   extern __shared__ float2 sm[];
   constexpr int LD = P1 + 1;
   constexpr int S = P0 * P1;
+  constexpr int W0 = win_len<P0>();
+  constexpr int W1 = win_len<P1>();
   AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const &w = wins[t.win];
-  int W0, W1;
-  win_w01<P0, P1>(w, W0, W1);
   int const nk = t.nk;
   float2 const *src = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * S;
   for (int i = threadIdx.x; i < nk * P1; i += blockDim.x) {
@@ -895,53 +913,40 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
     for (int c0 = 0; c0 < P0; ++c0) v[c0] = src[k * S + c0 * P1 + c1];
     agdfft::fft<P0, +1>(v);
 #pragma unroll
-    for (int n0 = 0; n0 < P0; ++n0)
-      if (n0 < W0) sm[(k * P0 + n0) * LD + c1] = v[n0];
+    for (int n0 = 0; n0 < W0; ++n0) sm[(k * W0 + n0) * LD + c1] = v[n0];
   }
   __syncthreads();
-  rows_fft<P0, P1, +1>(sm, nk, W0, P1, W1);
+  rows_fft<P0, P1, +1, P1, W1>(sm, nk);
   __syncthreads();
-  int const ntr = W0 * W1;
-  float2 *dst = U + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * ntr;
-  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
-    int const k = i / ntr;
-    int const r = i - k * ntr;
-    int const n0 = r / W1;
-    int const n1 = r - n0 * W1;
-    dst[i] = sm[(k * P0 + n0) * LD + n1];
-  }
+  planes_to_global<P0, P1>(sm, U + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * (W0 * W1), nk);
 }
 
-// Adjoint spatial transform fused with the update / shrink (modes as in
-// k_adj_spatial) and, when fuse is set, with the forward spatial transform
-// of the updated model into U.
-template <int P0, int P1>
+// Adjoint spatial transform of V and the model update, by MODE:
+//   0  x <- S_lamb(x - alpha g)      (update and soft threshold)
+//   1  x <- -alpha g, max |x|^2 into maxbits (first iteration)
+//   2  x <- x - alpha g
+//   3  x <- g
+// With FUSE (mode 0) the updated model's forward spatial transform, the
+// next iteration's first stage, is written to U.
+template <int P0, int P1, int MODE, bool FUSE>
 __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     float2 const *__restrict__ V, float2 *__restrict__ X, float2 *__restrict__ U,
-    float alpha, float lamb, int mode, int fuse, unsigned int *__restrict__ maxbits) {
+    float alpha, float lamb, unsigned int *__restrict__ maxbits) {
   extern __shared__ float2 sm[];
   constexpr int LD = P1 + 1;
   constexpr int S = P0 * P1;
+  constexpr int W0 = win_len<P0>();
+  constexpr int W1 = win_len<P1>();
   AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const &w = wins[t.win];
-  int W0, W1;
-  win_w01<P0, P1>(w, W0, W1);
   int const nk = t.nk;
-  int const ntr = W0 * W1;
-  float2 const *vin = V + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * ntr;
-  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
-    int const k = i / ntr;
-    int const r = i - k * ntr;
-    int const n0 = r / W1;
-    int const n1 = r - n0 * W1;
-    sm[(k * P0 + n0) * LD + n1] = vin[i];
-  }
+  int64_t const uoff = w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * (W0 * W1);
+  planes_to_lds<P0, P1>(V + uoff, sm, nk);
   __syncthreads();
-  rows_fft<P0, P1, -1>(sm, nk, W0, W1, P1);
+  rows_fft<P0, P1, -1, W1, P1>(sm, nk);
   __syncthreads();
   float2 *x = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * S;
-  float const l2 = lamb * lamb;
   float mx = 0.f;
   for (int i = threadIdx.x; i < nk * P1; i += blockDim.x) {
     int const k = i / P1;
@@ -949,45 +954,40 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
     float2 v[P0];
 #pragma unroll
     for (int r = 0; r < P0; ++r)
-      v[r] = (r < W0) ? sm[(k * P0 + r) * LD + c1] : make_float2(0.f, 0.f);
+      v[r] = (r < W0) ? sm[(k * W0 + r) * LD + c1] : make_float2(0.f, 0.f);
     agdfft::fft<P0, -1>(v);
+    float2 *xc = x + k * S + c1;
 #pragma unroll
     for (int c0 = 0; c0 < P0; ++c0) {
       float2 const gv = v[c0];
       float2 nv;
-      int const gi = k * S + c0 * P1 + c1;
-      if (mode == 3) {
+      if constexpr (MODE == 3) {
         nv = gv;
-      } else if (mode == 1) {
+      } else if constexpr (MODE == 1) {
         nv = make_float2(0.f - alpha * gv.x, 0.f - alpha * gv.y);
         mx = fmaxf(mx, nv.x * nv.x + nv.y * nv.y);
       } else {
-        nv = x[gi];
+        nv = xc[c0 * P1];
         nv.x -= alpha * gv.x;
         nv.y -= alpha * gv.y;
-        if (mode == 0) {
+        if constexpr (MODE == 0) {
+          // x (|x| - lamb) / |x| when |x| > lamb, else 0
           float const a = nv.x * nv.x + nv.y * nv.y;
-          if (a <= l2) {
-            nv = make_float2(0.f, 0.f);
-          } else {
-            float const ab = sqrtf(a);
-            float const sh = (ab - lamb) / ab;
-            nv.x *= sh;
-            nv.y *= sh;
-          }
+          float const sh = fmaxf(1.f - lamb * __builtin_amdgcn_rsqf(a), 0.f);
+          nv.x *= sh;
+          nv.y *= sh;
         }
       }
-      x[gi] = nv;
+      xc[c0 * P1] = nv;
       v[c0] = nv;
     }
-    if (fuse) {
+    if constexpr (FUSE) {
       agdfft::fft<P0, +1>(v);
 #pragma unroll
-      for (int n0 = 0; n0 < P0; ++n0)
-        if (n0 < W0) sm[(k * P0 + n0) * LD + c1] = v[n0];
+      for (int n0 = 0; n0 < W0; ++n0) sm[(k * W0 + n0) * LD + c1] = v[n0];
     }
   }
-  if (mode == 1) {
+  if constexpr (MODE == 1) {
     __shared__ float red[NTHR / 64];
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
@@ -998,17 +998,11 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
       atomicMax(maxbits, __float_as_uint(m));
     }
   }
-  if (!fuse) return;
-  __syncthreads();
-  rows_fft<P0, P1, +1>(sm, nk, W0, P1, W1);
-  __syncthreads();
-  float2 *dst = U + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * ntr;
-  for (int i = threadIdx.x; i < nk * ntr; i += blockDim.x) {
-    int const k = i / ntr;
-    int const r = i - k * ntr;
-    int const n0 = r / W1;
-    int const n1 = r - n0 * W1;
-    dst[i] = sm[(k * P0 + n0) * LD + n1];
+  if constexpr (FUSE) {
+    __syncthreads();
+    rows_fft<P0, P1, +1, P1, W1>(sm, nk);
+    __syncthreads();
+    planes_to_global<P0, P1>(sm, U + uoff, nk);
   }
 }
 
@@ -1272,7 +1266,15 @@ int agdg_init(void) {
 #define ATTR_S(a, b)                                                              \
     (void)hipFuncSetAttribute((void const *)k_fwd_spatial_t<a, b>,                \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
-    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b>,                \
+    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 0, false>,      \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
+    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 0, true>,       \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
+    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 1, false>,      \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
+    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 2, false>,      \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
+    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 3, false>,      \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 #define ATTR_T(a, b)                                                              \
     (void)hipFuncSetAttribute((void const *)k_fwd_time_t<a, b>,                   \
@@ -1425,17 +1427,42 @@ int agdg_adj_time(AgdDev const *g, int from_record) {
   return 0;
 }
 
+}  // extern "C"
+
+template <int P0, int P1, int MODE, bool FUSE>
+static int launch_adj_spatial(AgdDev const *g, AgdClass const &c, float alpha, float lamb) {
+  return launch("adj_spatial", k_adj_spatial_t<P0, P1, MODE, FUSE>, dim3(c.count),
+                dim3(NTHR), c.lds, g->win, g->tasks, c.begin, (float2 const *)g->V,
+                (float2 *)g->X, (float2 *)g->U, alpha, lamb, g->scalar);
+}
+
+template <int P0, int P1>
+static int adj_spatial_sized(AgdDev const *g, AgdClass const &c, float alpha, float lamb,
+                             int mode, int fuse) {
+  if (fuse && mode != 0) {
+    fprintf(stderr, "ERROR: fused update requested for mode %d\n", mode);
+    return 1;
+  }
+  switch (mode) {
+    case 0:
+      return fuse ? launch_adj_spatial<P0, P1, 0, true>(g, c, alpha, lamb)
+                  : launch_adj_spatial<P0, P1, 0, false>(g, c, alpha, lamb);
+    case 1: return launch_adj_spatial<P0, P1, 1, false>(g, c, alpha, lamb);
+    case 2: return launch_adj_spatial<P0, P1, 2, false>(g, c, alpha, lamb);
+    default: return launch_adj_spatial<P0, P1, 3, false>(g, c, alpha, lamb);
+  }
+}
+
+extern "C" {
+
 int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fuse) {
   for (int ci = 0; ci < g->n_sclass; ++ci) {
     AgdClass const &c = g->sclass[ci];
     bool done = false;
-#define AS_CASE(a, b)                                                                  \
-    if (!done && c.p0 == a && c.p1 == b) {                                             \
-      if (launch("adj_spatial", k_adj_spatial_t<a, b>, dim3(c.count), dim3(NTHR), c.lds, \
-                 g->win, g->tasks, c.begin, (float2 const *)g->V, (float2 *)g->X,      \
-                 (float2 *)g->U, alpha, lamb, mode, fuse, g->scalar))                  \
-        return 1;                                                                      \
-      done = true;                                                                     \
+#define AS_CASE(a, b)                                                       \
+    if (!done && c.p0 == a && c.p1 == b) {                                  \
+      if (adj_spatial_sized<a, b>(g, c, alpha, lamb, mode, fuse)) return 1; \
+      done = true;                                                          \
     }
     SPATIAL_SIZES(AS_CASE)
 #undef AS_CASE
