@@ -989,7 +989,7 @@ static int wavelet_conv_spectrum(float const *wav, int len, int N, int nfk, int 
 static int const kSpatialSizes[][2] = {{1, 10},  {1, 20},  {1, 40},  {10, 10},
                                        {10, 20}, {10, 40}, {20, 10}, {20, 20},
                                        {20, 40}, {40, 10}, {40, 20}, {40, 40}};
-static int const kTimeSizes[][2] = {{8, 5}, {8, 10}, {16, 10}, {16, 20}};
+static int const kTimeSizes[][2] = {{5, 8}, {10, 8}, {10, 16}, {20, 16}};
 
 static void spatial_key(AgdWin const *W, int *p0, int *p1) {
   size_t i;
@@ -1016,16 +1016,37 @@ static void spatial_key(AgdWin const *W, int *p0, int *p1) {
 /* window length of a specialised padded size (W = 4 P / 5; 1 for P = 1) */
 static int spec_len(int P) { return P == 1 ? 1 : P * 4 / 5; }
 
-static void time_key(AgdWin const *W, int *m1, int *m2) {
+/* specialised time sizes of a window (Pt = 2 M1 M2 and Wt = 4 Pt / 5),
+ * regardless of the layout */
+static void time_sizes(AgdWin const *W, int *m1, int *m2) {
   size_t i;
   *m1 = 0;
   *m2 = 0;
-  if (W->Pt % 2 || W->Wt % 2) return; /* specialised kernels: even Pt and Wt */
+  if (W->Pt % 2 || 5 * W->Wt != 4 * W->Pt) return;
   for (i = 0; i < sizeof(kTimeSizes) / sizeof(kTimeSizes[0]); i++)
     if (kTimeSizes[i][0] * kTimeSizes[i][1] == W->Pt / 2) {
       *m1 = kTimeSizes[i][0];
       *m2 = kTimeSizes[i][1];
     }
+}
+
+/* Traces per U / V tile: 16 when both transforms of the window have
+ * specialised kernels and the traces form whole tiles, else the whole
+ * window (untiled layout of the generic kernels). */
+#define TIME_TILE 16
+static int window_ug(AgdWin const *W) {
+  int a, b;
+  spatial_key(W, &a, &b);
+  if (!a || W->ntr % TIME_TILE) return W->ntr;
+  time_sizes(W, &a, &b);
+  return a ? TIME_TILE : W->ntr;
+}
+
+/* the specialised time kernels serve tiled windows only */
+static void time_key(AgdWin const *W, int *m1, int *m2) {
+  *m1 = 0;
+  *m2 = 0;
+  if (W->ug == TIME_TILE && W->ntr % TIME_TILE == 0) time_sizes(W, m1, m2);
 }
 
 /* Spatial tasks (chunks of time-frequency planes) and time blocks (runs of
@@ -1115,17 +1136,8 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
     /* traces per block: as many as fit the LDS budget, up to tb_max
      * (specialised kernels: AGD_TIME_TB, default 8, with AGD_TIME_THREADS
      * threads, default 256) */
-    int tb_max = 16;
-    int nthr = 256;
-    if (cl->p0) {
-      char const *e1 = getenv("AGD_TIME_TB");
-      char const *e2 = getenv("AGD_TIME_THREADS");
-      tb_max = e1 ? atoi(e1) : 8;
-      nthr = e2 ? atoi(e2) : 256;
-      if (tb_max < 1 || tb_max > 16) tb_max = 8;
-      if (nthr != 64 && nthr != 128 && nthr != 256) nthr = 256;
-    }
-    cl->pad = nthr;
+    int tb_max = cl->p0 ? TIME_TILE : 16;
+    cl->pad = 0;
     for (tb = tb_max; tb >= 1; tb /= 2) {
       size_t need = 0;
       for (w = 0; w < p->n_win; w++) {
@@ -1135,11 +1147,10 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
         time_key(W, &a, &b);
         if (a != cl->p0 || b != cl->p1) continue;
         if (a) {
-          /* spectra + packed buffers, then the larger of the forward's carry,
-           * taper and scales and the adjoint's taper, scales and entries */
-          size_t const fwd = (size_t)tb * W->half * 4 + (size_t)W->Wt * 4 + 16 * 4;
-          size_t const adj = (size_t)W->Wt * 4 + 16 * 4 + 16 * 8 + 32 * 4;
-          l = (size_t)2 * tb * (W->Pt / 2 + 1) * 8 + (fwd > adj ? fwd : adj) + 64;
+          /* the forward's tile, work buffer and roots (the larger of the
+           * two kernels; the launcher passes each its own size) */
+          size_t const M = (size_t)a * b;
+          l = 8 * (2 * TIME_TILE * (M + 1) + 2 * M + 1);
         } else {
           int const M = (W->Pt % 2 == 0 && W->Pt >= 4) ? W->Pt / 2 : W->Pt;
           l = (size_t)tb * W->Ft * 8 + (size_t)2 * tb * M * 8 + (size_t)2 * tb * W->Wt * 4;
@@ -1147,7 +1158,7 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
         if (l > need) need = l;
       }
       cl->lds = need;
-      if (need <= (cl->p0 ? (size_t)80 * 1024 : (size_t)LDS_BUDGET) || tb == 1) break;
+      if (cl->p0 || need <= (size_t)LDS_BUDGET || tb == 1) break;
     }
     cl->tb = tb;
     if (!cl->p0) lds_t = cl->lds > lds_t ? cl->lds : lds_t;
@@ -1327,6 +1338,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
       if (W->tplan < 0 || W->tplan_full < 0) goto err;
       W->wplan = -1;
       W->wplan_full = -1;
+      W->ug = window_ug(W);
       W->x_off = xo;
       W->u_off = uo;
       W->t_off = to;

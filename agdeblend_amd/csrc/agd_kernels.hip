@@ -871,24 +871,30 @@ __device__ __forceinline__ void rows_fft(float2 *sm, int nk) {
   }
 }
 
-// LDS rows of the nk planes -> the window traces of the planes in U
-// ([k][W0][W1] contiguous), or back
-template <int P0, int P1>
-__device__ __forceinline__ void planes_to_global(float2 const *sm, float2 *g, int nk) {
+// LDS rows of the nk planes <-> the window traces of the planes in U / V.
+// Per time window U holds [ntr / ug][Ft][ug] (ug = 16: tiles of 16 traces
+// for the specialised time kernels; ug = ntr: [Ft][ntr]); the element
+// order below walks each tile's nk planes contiguously. Utw points at the
+// task's time window, k0 is its first plane.
+template <int P0, int P1, bool TO_LDS>
+__device__ __forceinline__ void planes_copy(float2 *sm, float2 *Utw, int k0, int Ft, int ug,
+                                            int nk) {
   constexpr int LD = P1 + 1;
+  constexpr int W0 = win_len<P0>();
   constexpr int W1 = win_len<P1>();
-  for (int i = threadIdx.x; i < nk * win_len<P0>() * W1; i += blockDim.x) {
-    int const row = i / W1;
-    g[i] = sm[row * LD + (i - row * W1)];
-  }
-}
-template <int P0, int P1>
-__device__ __forceinline__ void planes_to_lds(float2 const *g, float2 *sm, int nk) {
-  constexpr int LD = P1 + 1;
-  constexpr int W1 = win_len<P1>();
-  for (int i = threadIdx.x; i < nk * win_len<P0>() * W1; i += blockDim.x) {
-    int const row = i / W1;
-    sm[row * LD + (i - row * W1)] = g[i];
+  int const span = nk * ug;
+  for (int i = threadIdx.x; i < nk * W0 * W1; i += blockDim.x) {
+    int const g = i / span;
+    int const rem = i - g * span;
+    int const k = rem / ug;
+    int const tr = g * ug + (rem - k * ug);
+    int const row = tr / W1;
+    float2 *l = sm + (k * W0 + row) * LD + (tr - row * W1);
+    float2 *u = Utw + ((int64_t)g * Ft + k0) * ug + rem;
+    if (TO_LDS)
+      *l = *u;
+    else
+      *u = *l;
   }
 }
 
@@ -918,7 +924,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
   __syncthreads();
   rows_fft<P0, P1, +1, P1, W1>(sm, nk);
   __syncthreads();
-  planes_to_global<P0, P1>(sm, U + w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * (W0 * W1), nk);
+  planes_copy<P0, P1, false>(sm, U + w.u_off + (int64_t)t.tw * w.Ft * (W0 * W1), t.k0, w.Ft,
+                             w.ug, nk);
 }
 
 // Adjoint spatial transform of V and the model update, by MODE:
@@ -941,8 +948,8 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
   AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const &w = wins[t.win];
   int const nk = t.nk;
-  int64_t const uoff = w.u_off + ((int64_t)t.tw * w.Ft + t.k0) * (W0 * W1);
-  planes_to_lds<P0, P1>(V + uoff, sm, nk);
+  int64_t const uoff = w.u_off + (int64_t)t.tw * w.Ft * (W0 * W1);
+  planes_copy<P0, P1, true>(sm, const_cast<float2 *>(V) + uoff, t.k0, w.Ft, w.ug, nk);
   __syncthreads();
   rows_fft<P0, P1, -1, W1, P1>(sm, nk);
   __syncthreads();
@@ -1002,226 +1009,293 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
     __syncthreads();
     rows_fft<P0, P1, +1, P1, W1>(sm, nk);
     __syncthreads();
-    planes_to_global<P0, P1>(sm, U + uoff, nk);
+    planes_copy<P0, P1, false>(sm, U + uoff, t.k0, w.Ft, w.ug, nk);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Size-specialised time transforms, Pt = 2 M, M = M1 * M2 (two-phase FFT:
-// M2 register FFTs of length M1, twiddle, M1 register DFTs of length M2).
-// LDS per block: two [tb][M+1] complex buffers (+ the overlap buffers of
-// the forward).
+// Size-specialised time transforms of tiled windows. Pt = 2 M, M = M1 M2,
+// and Wt = 8 M / 5 (Pt = ceil(1.25 Wt) with Pt a multiple of 5). One block
+// serves a group of TG = 16 consecutive window traces, whose half spectra of
+// one time window form one contiguous tile [Ft][TG] of U / V. Per time
+// window: a two-phase FFT in one LDS buffer [TG][M + 1] (TG M2 register FFTs
+// of length M1, twiddle, TG M1 register FFTs of length M2, in place), with
+// the half-length packing of the real transform fused into the first phase
+// (forward) or a last pass (adjoint). The overlap with the neighbouring time
+// window stays in registers (each thread owns fixed samples of fixed
+// traces), and the next time window's input is loaded into registers while
+// the current one is transformed.
 // ---------------------------------------------------------------------------
-// In-place-ish two-phase FFT of length M on lines of buffer Z (line stride
-// M+1); the result is written, in natural order, into buffer O.
-template <int M1, int M2, int S>
-__device__ __forceinline__ void fft2phase(float2 *Z, float2 *O, int tb,
-                                          float2 const *__restrict__ rootsM) {
-  constexpr int M = M1 * M2;
-  constexpr int LDL = M + 1;
-  for (int i = threadIdx.x; i < tb * M2; i += blockDim.x) {
-    int const j = i / M2;
-    int const k2 = i - j * M2;
-    float2 *z = Z + j * LDL;
-    float2 v[M1];
-#pragma unroll
-    for (int k1 = 0; k1 < M1; ++k1) v[k1] = z[k1 * M2 + k2];
-    agdfft::fft<M1, S>(v);
-#pragma unroll
-    for (int n1 = 0; n1 < M1; ++n1) {
-      float2 tw = rootsM[(n1 * k2) % M];
-      if (S > 0) tw.y = -tw.y;
-      z[n1 * M2 + k2] = (n1 == 0 || k2 == 0) ? v[n1] : agdfft::c_mul(v[n1], tw);
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < tb * M1; i += blockDim.x) {
-    int const j = i / M1;
-    int const n1 = i - j * M1;
-    float2 const *z = Z + j * LDL;
-    float2 *o = O + j * LDL;
-    float2 v[M2];
-#pragma unroll
-    for (int k2 = 0; k2 < M2; ++k2) v[k2] = z[n1 * M2 + k2];
-    agdfft::fft<M2, S>(v);
-#pragma unroll
-    for (int n2 = 0; n2 < M2; ++n2) o[n1 + M1 * n2] = v[n2];
-  }
-  __syncthreads();
+constexpr int TG = 16;  // traces per tile
+
+template <int M>
+__host__ __device__ constexpr int time_threads() {
+  return M >= 160 ? 256 : 128;
+}
+// dynamic LDS bytes of the two kernels
+template <int M>
+__host__ __device__ constexpr size_t fwd_time_lds() {
+  return (size_t)8 * (2 * TG * (M + 1) + M + (M + 1)) + (size_t)4 * TG * (4 * M / 5) + 4 * TG;
+}
+template <int M>
+__host__ __device__ constexpr size_t adj_time_lds() {
+  return (size_t)8 * (TG * (M + 1) + M + (M + 1)) + TG * 16;
 }
 
-// Forward time transform for even Wt (Wt = 2 half): spectra of one time
-// window per iteration; overlap-add through a carry of the second half of
-// each windowed trace. Launched with 64, 128 or 256 threads.
+// position of complex output n of the in-place two-phase FFT
 template <int M1, int M2>
-__global__ __launch_bounds__(NTHR) void k_fwd_time_t(
-    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
+__device__ __forceinline__ int fft2_pos(int n) {
+  return (n % M1) * M2 + n / M1;
+}
+
+template <int M1, int M2>
+__global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float const *__restrict__ tapers, float const *__restrict__ tscale,
     float2 const *__restrict__ U, float *__restrict__ T) {
-  extern __shared__ float2 sm[];
   constexpr int M = M1 * M2;
+  constexpr int NT = time_threads<M>();
+  constexpr int Ft = M + 1;
   constexpr int LDL = M + 1;
+  constexpr int half = 4 * M / 5;
+  constexpr int JS = NT / half;          // trace stride of a thread's outputs
+  constexpr int NJ = TG / JS;            // traces per thread in the output pass
+  constexpr int NCH = Ft * TG / 2;       // 16-byte chunks per tile
+  constexpr int R = (NCH + NT - 1) / NT;
+  static_assert(NT % half == 0 && TG % JS == 0, "output mapping");
+  extern __shared__ float2 sm[];
+  float2 *IN = sm;            // [Ft][TG] tile
+  float2 *Z = IN + Ft * TG;   // [TG][LDL]
+  float2 *TW = Z + TG * LDL;  // exp(+2 pi i m / M)
+  float2 *RP = TW + M;        // exp(+2 pi i k / Pt), k <= M
+  float *CA = (float *)(RP + Ft);  // [TG][half] overlap carry
+  float *SC = CA + TG * half;      // [TG] per-trace scale
   int const bid = block0 + blockIdx.x;
   AgdWin const &w = wins[blocks[2 * bid]];
   int const t0 = blocks[2 * bid + 1];
-  int const ntr = w.ntr;
-  int const nt = min(tb, ntr - t0);
-  int const half = w.half, nconv = w.nconv, ntw = w.ntw;
-  float2 *SP = sm;
-  float2 *Z = SP + tb * LDL;
-  float *carry = (float *)(Z + tb * LDL);
-  float *ttl = carry + tb * half;  // time taper, 2*half
-  float *scl = ttl + 2 * half;     // per-trace scale
-  float2 const *rootsM = roots + plans[w.tplan].roots_off;
-  float2 const *rootsP = roots + plans[w.tplan_full].roots_off;
+  int const nconv = w.nconv, ntw = w.ntw;
+  int const tid = threadIdx.x;
+  {
+    float2 const *rM = roots + plans[w.tplan].roots_off;
+    float2 const *rP = roots + plans[w.tplan_full].roots_off;
+    for (int m = tid; m < M; m += NT) TW[m] = make_float2(rM[m].x, -rM[m].y);
+    for (int k = tid; k < Ft; k += NT) RP[k] = make_float2(rP[k].x, -rP[k].y);
+    for (int i = tid; i < TG * half; i += NT) CA[i] = 0.f;
+    if (tid < TG) SC[tid] = tscale[w.trace0 + t0 + tid];
+  }
+  int const s = tid % half;
+  int const j0 = tid / half;
+  float const tap0 = tapers[w.tt_off + s];
+  float const tap1 = tapers[w.tt_off + half + s];
   float *Tb = T + w.t_off + (int64_t)t0 * nconv;
-  int const ntile = tb * LDL;
-  for (int i = threadIdx.x; i < tb * half; i += blockDim.x) carry[i] = 0.f;
-  for (int i = threadIdx.x; i < 2 * half; i += blockDim.x) ttl[i] = tapers[w.tt_off + i];
-  for (int i = threadIdx.x; i < tb; i += blockDim.x) scl[i] = i < nt ? tscale[w.trace0 + t0 + i] : 0.f;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  f4 const *tile = (f4 const *)(U + w.u_off + (int64_t)(t0 / TG) * Ft * TG);
+  int64_t const tw_step = (int64_t)(w.ntr / TG) * Ft * TG / 2;  // float4 per time window
+  // chunk r of this thread: min(tid + NT r, NCH - 1) (the clamped ones
+  // repeat the last chunk and are not stored)
+  f4 pre[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) pre[r] = tile[min(tid + NT * r, NCH - 1)];
   for (int tw = 0; tw < ntw; ++tw) {
-    float2 const *u = U + w.u_off + (int64_t)tw * LDL * ntr + t0;
-    for (int i = threadIdx.x; i < ntile; i += blockDim.x) {
-      int const k = i / tb;
-      int const j = i - k * tb;
-      SP[j * LDL + k] = j < nt ? u[(int64_t)k * ntr + j] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (r + 1 < R || tid + NT * r < NCH) ((f4 *)IN)[tid + NT * r] = pre[r];
+    if (tw + 1 < ntw) {
+      f4 const *nt = tile + (int64_t)(tw + 1) * tw_step;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pre[r] = nt[min(tid + NT * r, NCH - 1)];
     }
     __syncthreads();
-    // half-length packing of the C2R (DC / Nyquist imaginary parts ignored)
-    for (int i = threadIdx.x; i < tb * M; i += blockDim.x) {
-      int const j = i / M;
-      int const k = i - j * M;
-      float2 xk = SP[j * LDL + k];
-      float2 xc = SP[j * LDL + (M - k)];
-      if (k == 0) {
-        xk.y = 0.f;
-        xc.y = 0.f;
+    // packing of the C2R (DC / Nyquist imaginary parts ignored), length-M1
+    // FFTs over k1 of Z[k1 M2 + k2], twiddle
+    for (int i = tid; i < TG * M2; i += NT) {
+      int const k2 = i / TG;
+      int const j = i - k2 * TG;
+      float2 v[M1];
+#pragma unroll
+      for (int k1 = 0; k1 < M1; ++k1) {
+        int const k = k1 * M2 + k2;
+        float2 xk = IN[k * TG + j];
+        float2 xc = IN[(M - k) * TG + j];
+        if (k == 0) {
+          xk.y = 0.f;
+          xc.y = 0.f;
+        }
+        xc.y = -xc.y;
+        float2 const e = agdfft::c_add(xk, xc);
+        float2 const o = agdfft::c_mul(agdfft::c_sub(xk, xc), RP[k]);
+        v[k1] = make_float2(e.x - o.y, e.y + o.x);
       }
-      xc.y = -xc.y;
-      float2 const e = agdfft::c_add(xk, xc);
-      float2 o = agdfft::c_sub(xk, xc);
-      float2 r = rootsP[k];
-      r.y = -r.y;
-      o = agdfft::c_mul(o, r);
-      Z[j * LDL + k] = make_float2(e.x - o.y, e.y + o.x);
+      agdfft::fft<M1, +1>(v);
+      float2 *zz = Z + j * LDL + k2;
+#pragma unroll
+      for (int n1 = 0; n1 < M1; ++n1)
+        zz[n1 * M2] = (n1 == 0 || k2 == 0) ? v[n1] : agdfft::c_mul(v[n1], TW[(n1 * k2) % M]);
     }
     __syncthreads();
-    fft2phase<M1, M2, +1>(Z, SP, tb, rootsM);
-    // first half: complete samples tw*half + s
-    for (int i = threadIdx.x; i < tb * half; i += blockDim.x) {
-      int const j = i / half;
-      int const s = i - j * half;
-      if (j >= nt) continue;
-      float2 const z = SP[j * LDL + (s >> 1)];
-      float const y = (s & 1) ? z.y : z.x;
-      Tb[(int64_t)j * nconv + tw * half + s] = carry[i] + y * scl[j] * ttl[s];
+    for (int i = tid; i < TG * M1; i += NT) {
+      int const n1 = i / TG;
+      int const j = i - n1 * TG;
+      float2 *zz = Z + j * LDL + n1 * M2;
+      float2 v[M2];
+#pragma unroll
+      for (int k2 = 0; k2 < M2; ++k2) v[k2] = zz[k2];
+      agdfft::fft<M2, +1>(v);
+#pragma unroll
+      for (int n2 = 0; n2 < M2; ++n2) zz[n2] = v[n2];
     }
     __syncthreads();
-    // second half: new carry
-    for (int i = threadIdx.x; i < tb * half; i += blockDim.x) {
-      int const j = i / half;
-      int const s = i - j * half + half;
-      float2 const z = SP[j * LDL + (s >> 1)];
-      float const y = (s & 1) ? z.y : z.x;
-      carry[i] = y * scl[j] * ttl[s];
+    // overlap-add: samples [0, half) complete this window's first half
+    int const p0 = fft2_pos<M1, M2>(s >> 1);
+    int const p1 = fft2_pos<M1, M2>((s + half) >> 1);
+#pragma unroll
+    for (int r = 0; r < NJ; ++r) {
+      int const j = j0 + JS * r;
+      float2 const z0 = Z[j * LDL + p0];
+      float2 const z1 = Z[j * LDL + p1];
+      float const y0 = (s & 1) ? z0.y : z0.x;
+      float const y1 = (s & 1) ? z1.y : z1.x;
+      float const sc = SC[j];
+      Tb[j * nconv + tw * half + s] = CA[j * half + s] + y0 * sc * tap0;
+      CA[j * half + s] = y1 * sc * tap1;
     }
-    __syncthreads();
   }
-  for (int i = threadIdx.x; i < tb * half; i += blockDim.x) {
-    int const j = i / half;
-    int const s = i - j * half;
-    int const t = ntw * half + s;
-    if (j < nt && t < nconv) Tb[(int64_t)j * nconv + t] = carry[i];
-  }
-  int const tail0 = (ntw + 1) * half;
-  for (int j = 0; j < nt; ++j)
-    for (int t = tail0 + threadIdx.x; t < nconv; t += blockDim.x)
-      Tb[(int64_t)j * nconv + t] = 0.f;
+  int const tl = ntw * half + s;
+#pragma unroll
+  for (int r = 0; r < NJ; ++r)
+    if (tl < nconv) Tb[(j0 + JS * r) * nconv + tl] = CA[(j0 + JS * r) * half + s];
+  // zero the capacity beyond the data portion (wavelet input padding)
+  for (int j = 0; j < TG; ++j)
+    for (int t = (ntw + 1) * half + tid; t < nconv; t += NT) Tb[(int64_t)j * nconv + t] = 0.f;
 }
 
-// Adjoint time transform for even Wt: record (or wavelet-adjoint traces)
-// -> windowed, tapered time windows -> R2C (x2 interior bins) -> V.
+// Adjoint time transform: record (or wavelet-adjoint traces) -> windowed,
+// tapered time windows -> R2C (x2 interior bins) -> V tiles.
 template <int M1, int M2>
-__global__ __launch_bounds__(NTHR) void k_adj_time_t(
-    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0, int tb,
+__global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
+    AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float const *__restrict__ tapers, float const *__restrict__ tscale,
     AgdEntry const *__restrict__ went, float const *__restrict__ rec,
     float const *__restrict__ Tin, float2 *__restrict__ V) {
-  extern __shared__ float2 sm[];
   constexpr int M = M1 * M2;
-  constexpr int Pt = 2 * M;
+  constexpr int NT = time_threads<M>();
+  constexpr int Ft = M + 1;
   constexpr int LDL = M + 1;
+  constexpr int Pt = 2 * M;
+  constexpr int half = 4 * M / 5;
+  constexpr int Wt = 2 * half;
+  constexpr int JS = NT / half;
+  constexpr int NJ = TG / JS;
+  constexpr int kmax2 = (Pt - 1) / 2;
+  static_assert(NT % half == 0 && TG % JS == 0, "sample mapping");
+  extern __shared__ float2 sm[];
+  float2 *Z = sm;             // [TG][LDL]
+  float2 *TW = Z + TG * LDL;  // exp(-2 pi i m / M)
+  float2 *RP = TW + M;        // exp(-2 pi i k / Pt), k <= M
+  int64_t *erec = (int64_t *)(RP + Ft);
+  int *ets = (int *)(erec + TG);
+  int *en = ets + TG;
   int const bid = block0 + blockIdx.x;
   AgdWin const &w = wins[blocks[2 * bid]];
   int const t0 = blocks[2 * bid + 1];
-  int const ntr = w.ntr;
-  int const nt = min(tb, ntr - t0);
-  int const Wt = w.Wt, half = w.half, nconv = w.nconv, ntw = w.ntw;
-  float2 *Z = sm;
-  float2 *SP = Z + tb * LDL;
-  float *ttl = (float *)(SP + tb * LDL);  // Wt
-  float *scl = ttl + Wt;                  // tb
-  int64_t *erec = (int64_t *)(scl + 16);  // tb record starts (8-byte aligned: Wt even)
-  int *ets = (int *)(erec + 16);          // tb trace starts
-  int *en = ets + 16;                     // tb lengths
-  float2 const *rootsM = roots + plans[w.tplan].roots_off;
-  float2 const *rootsP = roots + plans[w.tplan_full].roots_off;
-  for (int i = threadIdx.x; i < Wt; i += blockDim.x) ttl[i] = tapers[w.tt_off + i];
-  for (int i = threadIdx.x; i < tb; i += blockDim.x) {
-    int const g = w.trace0 + t0 + i;
-    scl[i] = i < nt ? tscale[g] : 0.f;
-    if (i < nt) {
-      AgdEntry const e = went[g];
-      erec[i] = e.rec;
-      ets[i] = e.trace_start;
-      en[i] = e.n;
-    } else {
-      erec[i] = 0;
-      ets[i] = 0;
-      en[i] = 0;
+  int const nconv = w.nconv, ntw = w.ntw;
+  int const tid = threadIdx.x;
+  {
+    float2 const *rM = roots + plans[w.tplan].roots_off;
+    float2 const *rP = roots + plans[w.tplan_full].roots_off;
+    for (int m = tid; m < M; m += NT) TW[m] = rM[m];
+    for (int k = tid; k < Ft; k += NT) RP[k] = rP[k];
+    if (tid < TG) {
+      AgdEntry const e = went[w.trace0 + t0 + tid];
+      erec[tid] = e.rec;
+      ets[tid] = e.trace_start;
+      en[tid] = e.n;
     }
   }
   __syncthreads();
+  int const s = tid % half;
+  int const j0 = tid / half;
+  float const tap0 = tapers[w.tt_off + s];
+  float const tap1 = tapers[w.tt_off + half + s];
+  float const *tin = Tin ? Tin + w.t_off + (int64_t)t0 * nconv : nullptr;
+  auto sample = [&](int j, int t) -> float {
+    if (tin != nullptr) return tin[(int64_t)j * nconv + t];
+    int const i2 = t - ets[j];
+    return (i2 >= 0 && i2 < en[j]) ? rec[erec[j] + i2] : 0.f;
+  };
+  float scl[NJ], cur[NJ], nxt[NJ];
+#pragma unroll
+  for (int r = 0; r < NJ; ++r) {
+    int const j = j0 + JS * r;
+    scl[r] = tscale[w.trace0 + t0 + j];
+    cur[r] = sample(j, s);
+    nxt[r] = sample(j, half + s);
+  }
+  float2 *vt = V + w.u_off + (int64_t)(t0 / TG) * Ft * TG;
+  int64_t const tw_step = (int64_t)(w.ntr / TG) * Ft * TG;
   for (int tw = 0; tw < ntw; ++tw) {
-    // windowed samples into the packed real buffer (zero padded to Pt)
-    for (int i = threadIdx.x; i < tb * Pt; i += blockDim.x) {
-      int const j = i / Pt;
-      int const s = i - j * Pt;
-      float x = 0.f;
-      if (j < nt && s < Wt) {
-        int const t = tw * half + s;
-        if (Tin != nullptr) {
-          x = Tin[w.t_off + (int64_t)(t0 + j) * nconv + t];
-        } else {
-          int const i2 = t - ets[j];
-          x = (i2 >= 0 && i2 < en[j]) ? rec[erec[j] + i2] : 0.f;
-        }
-        x *= scl[j] * ttl[s];
-      }
-      ((float *)(Z + j * LDL))[s] = x;
+#pragma unroll
+    for (int r = 0; r < NJ; ++r) {
+      float *zr = (float *)(Z + (j0 + JS * r) * LDL);
+      zr[s] = cur[r] * scl[r] * tap0;
+      zr[s + half] = nxt[r] * scl[r] * tap1;
+      cur[r] = nxt[r];
+    }
+    for (int i = tid; i < TG * (Pt - Wt); i += NT) {
+      int const j = i / (Pt - Wt);
+      ((float *)(Z + j * LDL))[Wt + (i - j * (Pt - Wt))] = 0.f;
+    }
+    if (tw + 1 < ntw) {
+#pragma unroll
+      for (int r = 0; r < NJ; ++r) nxt[r] = sample(j0 + JS * r, (tw + 2) * half + s);
     }
     __syncthreads();
-    fft2phase<M1, M2, -1>(Z, SP, tb, rootsM);
-    constexpr int kmax2 = (Pt - 1) / 2;
-    float2 *vout = V + w.u_off + (int64_t)tw * LDL * ntr + t0;
-    for (int i = threadIdx.x; i < LDL * tb; i += blockDim.x) {
-      int const k = i / tb;
-      int const j = i - k * tb;
-      if (j >= nt) continue;
-      float2 const zk = SP[j * LDL + (k % M)];
-      float2 zc = SP[j * LDL + ((M - k) % M)];
+    for (int i = tid; i < TG * M2; i += NT) {
+      int const k2 = i / TG;
+      int const j = i - k2 * TG;
+      float2 *zz = Z + j * LDL + k2;
+      float2 v[M1];
+#pragma unroll
+      for (int k1 = 0; k1 < M1; ++k1) v[k1] = zz[k1 * M2];
+      agdfft::fft<M1, -1>(v);
+#pragma unroll
+      for (int n1 = 0; n1 < M1; ++n1)
+        zz[n1 * M2] = (n1 == 0 || k2 == 0) ? v[n1] : agdfft::c_mul(v[n1], TW[(n1 * k2) % M]);
+    }
+    __syncthreads();
+    for (int i = tid; i < TG * M1; i += NT) {
+      int const n1 = i / TG;
+      int const j = i - n1 * TG;
+      float2 *zz = Z + j * LDL + n1 * M2;
+      float2 v[M2];
+#pragma unroll
+      for (int k2 = 0; k2 < M2; ++k2) v[k2] = zz[k2];
+      agdfft::fft<M2, -1>(v);
+#pragma unroll
+      for (int n2 = 0; n2 < M2; ++n2) zz[n2] = v[n2];
+    }
+    __syncthreads();
+    // half spectrum of the real window, x2 on the interior bins
+    float2 *vo = vt + (int64_t)tw * tw_step;
+    for (int i = tid; i < Ft * TG; i += NT) {
+      int const k = i / TG;
+      int const j = i - k * TG;
+      int const kk = (k == M) ? 0 : k;
+      int const kc = (k == 0) ? 0 : M - k;
+      float2 const zk = Z[j * LDL + fft2_pos<M1, M2>(kk)];
+      float2 zc = Z[j * LDL + fft2_pos<M1, M2>(kc)];
       zc.y = -zc.y;
       float2 const e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
       float2 const o = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
-      float2 X = agdfft::c_add(e, agdfft::c_mul(o, rootsP[k]));
+      float2 X = agdfft::c_add(e, agdfft::c_mul(o, RP[k]));
       if (k == 0 || k == M) X.y = 0.f;
       if (k >= 1 && k <= kmax2) {
         X.x *= 2.f;
         X.y *= 2.f;
       }
-      vout[(int64_t)k * ntr + j] = X;
+      vo[i] = X;
     }
     __syncthreads();
   }
@@ -1235,7 +1309,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_time_t(
 #define SPATIAL_SIZES(X) \
   X(1, 10) X(1, 20) X(1, 40) X(10, 10) X(10, 20) X(10, 40) X(20, 10) X(20, 20) \
   X(20, 40) X(40, 10) X(40, 20) X(40, 40)
-#define TIME_SIZES(X) X(8, 5) X(8, 10) X(16, 10) X(16, 20)
+#define TIME_SIZES(X) X(5, 8) X(10, 8) X(10, 16) X(20, 16)
 
 extern "C" {
 
@@ -1365,9 +1439,10 @@ int agdg_fwd_time(AgdDev const *g) {
     bool done = false;
 #define FT_CASE(a, b)                                                                   \
     if (!done && c.p0 == a && c.p1 == b) {                                              \
-      if (launch("fwd_time", k_fwd_time_t<a, b>, dim3(c.count), dim3(c.pad), c.lds,       \
-                 g->win, g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots,  \
-                 g->tapers, g->tscale, (float2 const *)g->U, g->T))                     \
+      if (launch("fwd_time", k_fwd_time_t<a, b>, dim3(c.count),                          \
+                 dim3(time_threads<a * b>()), fwd_time_lds<a * b>(), g->win, g->tblocks,   \
+                 c.begin, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
+                 (float2 const *)g->U, g->T))                                           \
         return 1;                                                                       \
       done = true;                                                                      \
     }
@@ -1409,10 +1484,10 @@ int agdg_adj_time(AgdDev const *g, int from_record) {
     bool done = false;
 #define AT_CASE(a, b)                                                                   \
     if (!done && c.p0 == a && c.p1 == b) {                                              \
-      if (launch("adj_time", k_adj_time_t<a, b>, dim3(c.count), dim3(c.pad), c.lds,       \
-                 g->win, g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots,  \
-                 g->tapers, g->tscale, g->went, (float const *)g->r, Tin,               \
-                 (float2 *)g->V))                                                       \
+      if (launch("adj_time", k_adj_time_t<a, b>, dim3(c.count),                          \
+                 dim3(time_threads<a * b>()), adj_time_lds<a * b>(), g->win, g->tblocks,   \
+                 c.begin, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
+                 g->went, (float const *)g->r, Tin, (float2 *)g->V))                    \
         return 1;                                                                       \
       done = true;                                                                      \
     }

@@ -8,9 +8,12 @@
  *             spatial coefficients contiguously, row-major, last space dim
  *             fastest). Mathematically the same coefficients as the
  *             reference's [ntw][P_space...][Ft] (fk.c:256-262), transposed.
- *   inter U   complex float, per window [ntw][Ft][ntr]: spatial transform of
- *             X cropped to the window's traces (forward), or time transform
- *             of the traces (adjoint). ntr = prod(W_space).
+ *   inter U   complex float, per window [ntw][ntr/ug][Ft][ug]: spatial
+ *             transform of X cropped to the window's traces (forward), or
+ *             time transform of the traces (adjoint); ntr = prod(W_space).
+ *             ug = 16 (tiles of 16 traces, one contiguous [Ft][16] block per
+ *             time window) for windows served by the specialised kernels,
+ *             else ug = ntr ([ntw][Ft][ntr]).
  *   traces T  float, per window [ntr][nconv] (the reference's temp_1 trace
  *             buffer, fk.c:330-331).
  *   record    float, all (channel, interval) continuous records concatenated
@@ -50,7 +53,7 @@ typedef struct {
   int wM;                /* wavelet convolution length M */
   int trace0;            /* first global window-trace index */
   int patch;
-  int pad;
+  int ug;                /* trace group of the U / V layout (16 or ntr) */
   int64_t x_off;         /* model offset (complex) */
   int64_t u_off;         /* intermediate offset (complex) */
   int64_t t_off;         /* trace buffer offset (float) */
