@@ -655,6 +655,8 @@ struct AgdProblem {
   int64_t *d_o_off;
   int *d_o_nt, *d_o_ptr;
   int64_t *d_o_src;
+  int64_t *d_wsrc;     /* per global window trace: its patch trace's data offset */
+  int *d_wnt;          /* per global window trace: its patch's n_times */
   int n_out;
   float stepsize;
   int have_data;
@@ -1063,15 +1065,17 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
     AgdWin const *W = &p->win[w];
     int a, b, found = 0;
     AgdClass *cl = (AgdClass *)sc.p;
+    int const tiled = (W->ug != W->ntr);
     spatial_key(W, &a, &b);
     for (c = 0; c < (int)sc.n; c++)
-      if (cl[c].p0 == a && cl[c].p1 == b) found = 1;
+      if (cl[c].p0 == a && cl[c].p1 == b && (!a || cl[c].tb == tiled)) found = 1;
     if (!found) {
       AgdClass *n = (AgdClass *)vec_push(&sc);
       if (n == NULL) return 1;
       memset(n, 0, sizeof(*n));
       n->p0 = a;
       n->p1 = b;
+      n->tb = a ? tiled : 0; /* spatial classes: U / V tiled by TIME_TILE traces */
     }
     time_key(W, &a, &b);
     found = 0;
@@ -1093,7 +1097,7 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
       AgdWin const *W = &p->win[w];
       int a, b, nk, tw, k0;
       spatial_key(W, &a, &b);
-      if (a != cl->p0 || b != cl->p1) continue;
+      if (a != cl->p0 || b != cl->p1 || (a && (W->ug != W->ntr) != cl->tb)) continue;
       if (a) {
         /* one column transform per thread of a 256-thread block (P1 per
          * plane), within a 48 KB LDS tile of W0 rows of P1 + 1 per plane;
@@ -1324,6 +1328,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
       W->ntw = W->nfk / W->half - 1; /* fk.c:24-26 */
       Pall *= W->Pt;
       scaler = 1.0f / sqrtf((float)Pall);
+      W->scaler = scaler;
       for (d = 0; d < nsp; d++) {
         W->splan[d] = planset_get(&ps, W->P[d]);
         if (W->splan[d] < 0) goto err;
@@ -1613,6 +1618,29 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->d_o_ptr = (int *)dev_upload(p, o_ptr, ((size_t)p->n_out + 1) * sizeof(int));
     p->d_o_src = (int64_t *)dev_upload(p, o_src, (size_t)(ncontrib ? ncontrib : 1) * sizeof(int64_t));
     p->d_data = (float *)dev_alloc(p, (size_t)p->data_len * 4 + 16);
+    {
+      /* window traces of patch data (warm start) */
+      int64_t *wsrc = (int64_t *)malloc(((size_t)p->n_wtr + 1) * sizeof(int64_t));
+      int *wnt = (int *)malloc(((size_t)p->n_wtr + 1) * sizeof(int));
+      if (wsrc == NULL || wnt == NULL) {
+        free(wsrc);
+        free(wnt);
+        goto err;
+      }
+      for (w = 0; w < p->n_win; w++) {
+        AgdWin const *W = &p->win[w];
+        for (t = 0; t < W->ntr; t++) {
+          int const gt = W->trace0 + t;
+          wsrc[gt] = p->data_off[W->patch] + (int64_t)p->wpatch_trace[gt] * p->nt[W->patch];
+          wnt[gt] = p->nt[W->patch];
+        }
+      }
+      p->d_wsrc = (int64_t *)dev_upload(p, wsrc, (size_t)p->n_wtr * sizeof(int64_t));
+      p->d_wnt = (int *)dev_upload(p, wnt, (size_t)p->n_wtr * sizeof(int));
+      free(wsrc);
+      free(wnt);
+      if (!p->d_wsrc || !p->d_wnt) goto err;
+    }
     free(o_off);
     free(o_nt);
     free(o_ptr);
@@ -1807,9 +1835,9 @@ static int adjoint_update(AgdProblem *p, float alpha, float lamb, int mode) {
   int const fuse = p->fuse && mode == 0;
   if (p->use_wavelets) {
     if (agdg_wavelet(&p->dev, 1)) return 1;
-    if (agdg_adj_time(&p->dev, 0)) return 1;
+    if (agdg_adj_time(&p->dev, p->dev.T2, 0)) return 1;
   } else {
-    if (agdg_adj_time(&p->dev, 1)) return 1;
+    if (agdg_adj_time(&p->dev, NULL, 0)) return 1;
   }
   if (agdg_adj_spatial(&p->dev, alpha, lamb, mode, fuse)) return 1;
   p->u_valid = fuse;
@@ -1840,8 +1868,51 @@ static float g_last_lambda0 = 0.0f;
 EXPORT float agdx_last_lambda0(void) { return g_last_lambda0; }
 
 /* agdeblend.c:526-629 */
+/* Warm-start model s R2C(pad(y_w)) of a data estimate y (patch layout)
+ * into dev.X: the window traces of y (placed as set_output places them),
+ * the plain time R2C (scaler, no tapers, no x2) and the spatial DFTs. */
+static int warm_model(AgdProblem *p, float const *const *y) {
+  int a;
+  for (a = 0; a < p->n_patches; a++) {
+    size_t const n = (size_t)p->ntr[a] * p->nt[a];
+    if (y[a] == NULL) {
+      fprintf(stderr, "ERROR: warm-start data of patch %d is NULL\n", a);
+      return 1;
+    }
+    if (agdg_h2d(p->d_data + p->data_off[a], y[a], n * 4)) return 1;
+  }
+  if (agdg_window_traces(&p->dev, p->d_wsrc, p->d_wnt, p->d_data, p->dev.T)) return 1;
+  if (agdg_adj_time(&p->dev, p->dev.T, 1)) return 1;
+  if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, 0)) return 1;
+  p->u_valid = 0;
+  return 0;
+}
+
+EXPORT int agdx_warm_model(AgdProblem *p, float const *const *y) {
+  if (p->host_only) return 1;
+  if (warm_model(p, y)) return 1;
+  return agdg_sync();
+}
+
+static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float const *x0,
+                    float const *const *warm);
+
 EXPORT int agdx_run(AgdProblem *p, float f, int n_its, int print_freq,
                     float const *x0) {
+  return run_impl(p, f, n_its, print_freq, x0, NULL);
+}
+
+EXPORT int agdx_run_warm(AgdProblem *p, float f, int n_its, int print_freq,
+                         float const *const *warm) {
+  if (warm == NULL) {
+    fprintf(stderr, "ERROR: warm-start data is NULL\n");
+    return 1;
+  }
+  return run_impl(p, f, n_its, print_freq, NULL, warm);
+}
+
+static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float const *x0,
+                    float const *const *warm) {
   float const decay = expf(logf(0.001f / f) / (float)n_its);
   int const first_linear_it =
       (int)((logf(decay) * ((float)n_its - 1.0f) + 1.0f) / logf(decay));
@@ -1894,7 +1965,9 @@ EXPORT int agdx_run(AgdProblem *p, float f, int n_its, int print_freq,
   g_last_lambda0 = lamb;
   slope *= lamb;
   lamb *= f;
-  if (x0 == NULL) {
+  if (warm != NULL) {
+    if (warm_model(p, warm)) return 1;
+  } else if (x0 == NULL) {
     if (agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
   }
   p->u_valid = 0;

@@ -471,7 +471,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_time(
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float const *__restrict__ tapers, float const *__restrict__ tscale,
     AgdEntry const *__restrict__ went, float const *__restrict__ rec,
-    float const *__restrict__ Tin, float2 *__restrict__ V) {
+    float const *__restrict__ Tin, float2 *__restrict__ V, int plain) {
   extern __shared__ float2 sm[];
   int const bid = block0 + blockIdx.x;
   AgdWin const w = wins[blocks[2 * bid]];
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_time(
           int const i2 = t - e.trace_start;
           x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
         }
-        v = x * tscale[g] * tt[s];
+        v = plain ? x * w.scaler : x * tscale[g] * tt[s];
       }
       if (even) {
         float *Af = (float *)A;
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_time(
         X = R[j * M + k];
         if (k == 0) X.y = 0.f;
       }
-      if (k >= 1 && k <= kmax2) {
+      if (!plain && k >= 1 && k <= kmax2) {
         X.x *= 2.f;
         X.y *= 2.f;
       }
@@ -811,6 +811,25 @@ __global__ __launch_bounds__(NTHR) void k_wavelet(
 // ---------------------------------------------------------------------------
 // Output assembly and blend extraction
 // ---------------------------------------------------------------------------
+// Window traces of patch data (the inverse placement of set_output,
+// model.c:1478-1484): trace buffer sample t of a window trace is data sample
+// t - W_t/2 of its patch trace, zero outside the data. One block per global
+// window trace.
+__global__ __launch_bounds__(NTHR) void k_window_traces(
+    AgdWin const *__restrict__ wins, int const *__restrict__ trace_win,
+    int64_t const *__restrict__ src_off, int const *__restrict__ src_nt,
+    float const *__restrict__ data, float *__restrict__ T) {
+  int const g = blockIdx.x;
+  AgdWin const &w = wins[trace_win[g]];
+  float const *src = data + src_off[g];
+  int const nt = src_nt[g];
+  float *dst = T + w.t_off + (int64_t)(g - w.trace0) * w.nconv;
+  for (int t = threadIdx.x; t < w.nconv; t += blockDim.x) {
+    int const ts = t - w.half;
+    dst[t] = (ts >= 0 && ts < nt) ? src[ts] : 0.f;
+  }
+}
+
 __global__ __launch_bounds__(NTHR) void k_output(
     float const *__restrict__ T, int64_t const *__restrict__ o_off,
     int const *__restrict__ o_nt, int const *__restrict__ o_ptr,
@@ -848,6 +867,8 @@ __global__ __launch_bounds__(NTHR) void k_extract(
 // holding the W0 trace rows of each plane (padded rows of P1 + 1); every
 // FFT is a register codelet.
 // ---------------------------------------------------------------------------
+constexpr int TG = 16;  // traces per U / V tile of the specialised kernels
+
 template <int P>
 __host__ __device__ constexpr int win_len() {
   return P == 1 ? 1 : P * 4 / 5;
@@ -876,12 +897,12 @@ __device__ __forceinline__ void rows_fft(float2 *sm, int nk) {
 // for the specialised time kernels; ug = ntr: [Ft][ntr]); the element
 // order below walks each tile's nk planes contiguously. Utw points at the
 // task's time window, k0 is its first plane.
-template <int P0, int P1, bool TO_LDS>
-__device__ __forceinline__ void planes_copy(float2 *sm, float2 *Utw, int k0, int Ft, int ug,
-                                            int nk) {
+template <int P0, int P1, bool TILED, bool TO_LDS>
+__device__ __forceinline__ void planes_copy(float2 *sm, float2 *Utw, int k0, int Ft, int nk) {
   constexpr int LD = P1 + 1;
   constexpr int W0 = win_len<P0>();
   constexpr int W1 = win_len<P1>();
+  constexpr int ug = TILED ? TG : W0 * W1;
   int const span = nk * ug;
   for (int i = threadIdx.x; i < nk * W0 * W1; i += blockDim.x) {
     int const g = i / span;
@@ -898,7 +919,7 @@ __device__ __forceinline__ void planes_copy(float2 *sm, float2 *Utw, int k0, int
   }
 }
 
-template <int P0, int P1>
+template <int P0, int P1, bool TILED>
 __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     float2 const *__restrict__ X, float2 *__restrict__ U) {
@@ -924,8 +945,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
   __syncthreads();
   rows_fft<P0, P1, +1, P1, W1>(sm, nk);
   __syncthreads();
-  planes_copy<P0, P1, false>(sm, U + w.u_off + (int64_t)t.tw * w.Ft * (W0 * W1), t.k0, w.Ft,
-                             w.ug, nk);
+  planes_copy<P0, P1, TILED, false>(sm, U + w.u_off + (int64_t)t.tw * w.Ft * (W0 * W1), t.k0,
+                                    w.Ft, nk);
 }
 
 // Adjoint spatial transform of V and the model update, by MODE:
@@ -935,7 +956,7 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
 //   3  x <- g
 // With FUSE (mode 0) the updated model's forward spatial transform, the
 // next iteration's first stage, is written to U.
-template <int P0, int P1, int MODE, bool FUSE>
+template <int P0, int P1, bool TILED, int MODE, bool FUSE>
 __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     float2 const *__restrict__ V, float2 *__restrict__ X, float2 *__restrict__ U,
@@ -949,7 +970,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
   AgdWin const &w = wins[t.win];
   int const nk = t.nk;
   int64_t const uoff = w.u_off + (int64_t)t.tw * w.Ft * (W0 * W1);
-  planes_copy<P0, P1, true>(sm, const_cast<float2 *>(V) + uoff, t.k0, w.Ft, w.ug, nk);
+  planes_copy<P0, P1, TILED, true>(sm, const_cast<float2 *>(V) + uoff, t.k0, w.Ft, nk);
   __syncthreads();
   rows_fft<P0, P1, -1, W1, P1>(sm, nk);
   __syncthreads();
@@ -1009,7 +1030,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
     __syncthreads();
     rows_fft<P0, P1, +1, P1, W1>(sm, nk);
     __syncthreads();
-    planes_copy<P0, P1, false>(sm, U + uoff, t.k0, w.Ft, w.ug, nk);
+    planes_copy<P0, P1, TILED, false>(sm, U + uoff, t.k0, w.Ft, nk);
   }
 }
 
@@ -1026,8 +1047,6 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
 // traces), and the next time window's input is loaded into registers while
 // the current one is transformed.
 // ---------------------------------------------------------------------------
-constexpr int TG = 16;  // traces per tile
-
 template <int M>
 __host__ __device__ constexpr int time_threads() {
   return M >= 160 ? 256 : 128;
@@ -1178,7 +1197,7 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float const *__restrict__ tapers, float const *__restrict__ tscale,
     AgdEntry const *__restrict__ went, float const *__restrict__ rec,
-    float const *__restrict__ Tin, float2 *__restrict__ V) {
+    float const *__restrict__ Tin, float2 *__restrict__ V, int plain) {
   constexpr int M = M1 * M2;
   constexpr int NT = time_threads<M>();
   constexpr int Ft = M + 1;
@@ -1217,8 +1236,9 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   __syncthreads();
   int const s = tid % half;
   int const j0 = tid / half;
-  float const tap0 = tapers[w.tt_off + s];
-  float const tap1 = tapers[w.tt_off + half + s];
+  float const tap0 = plain ? 1.f : tapers[w.tt_off + s];
+  float const tap1 = plain ? 1.f : tapers[w.tt_off + half + s];
+  float const two = plain ? 1.f : 2.f;
   float const *tin = Tin ? Tin + w.t_off + (int64_t)t0 * nconv : nullptr;
   auto sample = [&](int j, int t) -> float {
     if (tin != nullptr) return tin[(int64_t)j * nconv + t];
@@ -1229,7 +1249,7 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
 #pragma unroll
   for (int r = 0; r < NJ; ++r) {
     int const j = j0 + JS * r;
-    scl[r] = tscale[w.trace0 + t0 + j];
+    scl[r] = plain ? w.scaler : tscale[w.trace0 + t0 + j];
     cur[r] = sample(j, s);
     nxt[r] = sample(j, half + s);
   }
@@ -1292,8 +1312,8 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
       float2 X = agdfft::c_add(e, agdfft::c_mul(o, RP[k]));
       if (k == 0 || k == M) X.y = 0.f;
       if (k >= 1 && k <= kmax2) {
-        X.x *= 2.f;
-        X.y *= 2.f;
+        X.x *= two;
+        X.y *= two;
       }
       vo[i] = X;
     }
@@ -1310,6 +1330,61 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   X(1, 10) X(1, 20) X(1, 40) X(10, 10) X(10, 20) X(10, 40) X(20, 10) X(20, 20) \
   X(20, 40) X(40, 10) X(40, 20) X(40, 40)
 #define TIME_SIZES(X) X(5, 8) X(10, 8) X(10, 16) X(20, 16)
+
+// the specialised kernels may use up to the CU's 160 KB of LDS: set once per
+// kernel instance, on its first launch
+static int allow_lds(void const *f) {
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  (void)hipGetLastError();
+  return 1;
+}
+
+template <int P0, int P1, bool TILED, int MODE, bool FUSE>
+static int launch_adj_spatial(AgdDev const *g, AgdClass const &c, float alpha, float lamb) {
+  static int const lds_ok = allow_lds((void const *)k_adj_spatial_t<P0, P1, TILED, MODE, FUSE>);
+  (void)lds_ok;
+  return launch("adj_spatial", k_adj_spatial_t<P0, P1, TILED, MODE, FUSE>, dim3(c.count),
+                dim3(NTHR), c.lds, g->win, g->tasks, c.begin, (float2 const *)g->V,
+                (float2 *)g->X, (float2 *)g->U, alpha, lamb, g->scalar);
+}
+
+template <int P0, int P1, bool TILED>
+static int adj_spatial_layout(AgdDev const *g, AgdClass const &c, float alpha, float lamb,
+                              int mode, int fuse) {
+  switch (mode) {
+    case 0:
+      return fuse ? launch_adj_spatial<P0, P1, TILED, 0, true>(g, c, alpha, lamb)
+                  : launch_adj_spatial<P0, P1, TILED, 0, false>(g, c, alpha, lamb);
+    case 1: return launch_adj_spatial<P0, P1, TILED, 1, false>(g, c, alpha, lamb);
+    case 2: return launch_adj_spatial<P0, P1, TILED, 2, false>(g, c, alpha, lamb);
+    default: return launch_adj_spatial<P0, P1, TILED, 3, false>(g, c, alpha, lamb);
+  }
+}
+
+template <int P0, int P1>
+static int adj_spatial_sized(AgdDev const *g, AgdClass const &c, float alpha, float lamb,
+                             int mode, int fuse) {
+  if (fuse && mode != 0) {
+    fprintf(stderr, "ERROR: fused update requested for mode %d\n", mode);
+    return 1;
+  }
+  return c.tb ? adj_spatial_layout<P0, P1, true>(g, c, alpha, lamb, mode, fuse)
+              : adj_spatial_layout<P0, P1, false>(g, c, alpha, lamb, mode, fuse);
+}
+
+template <int P0, int P1, bool TILED>
+static int launch_fwd_spatial(AgdDev const *g, AgdClass const &c, float const *X) {
+  static int const lds_ok = allow_lds((void const *)k_fwd_spatial_t<P0, P1, TILED>);
+  (void)lds_ok;
+  return launch("fwd_spatial", k_fwd_spatial_t<P0, P1, TILED>, dim3(c.count), dim3(NTHR),
+                c.lds, g->win, g->tasks, c.begin, (float2 const *)X, (float2 *)g->U);
+}
+
+template <int P0, int P1>
+static int fwd_spatial_sized(AgdDev const *g, AgdClass const &c, float const *X) {
+  return c.tb ? launch_fwd_spatial<P0, P1, true>(g, c, X)
+              : launch_fwd_spatial<P0, P1, false>(g, c, X);
+}
 
 extern "C" {
 
@@ -1337,27 +1412,12 @@ int agdg_init(void) {
                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     (void)hipFuncSetAttribute((void const *)k_wavelet,
                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-#define ATTR_S(a, b)                                                              \
-    (void)hipFuncSetAttribute((void const *)k_fwd_spatial_t<a, b>,                \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
-    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 0, false>,      \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
-    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 0, true>,       \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
-    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 1, false>,      \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
-    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 2, false>,      \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
-    (void)hipFuncSetAttribute((void const *)k_adj_spatial_t<a, b, 3, false>,      \
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 #define ATTR_T(a, b)                                                              \
     (void)hipFuncSetAttribute((void const *)k_fwd_time_t<a, b>,                   \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
     (void)hipFuncSetAttribute((void const *)k_adj_time_t<a, b>,                   \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    SPATIAL_SIZES(ATTR_S)
     TIME_SIZES(ATTR_T)
-#undef ATTR_S
 #undef ATTR_T
     (void)hipGetLastError();
   }
@@ -1415,12 +1475,10 @@ int agdg_fwd_spatial(AgdDev const *g, float const *X) {
   for (int ci = 0; ci < g->n_sclass; ++ci) {
     AgdClass const &c = g->sclass[ci];
     bool done = false;
-#define FWD_CASE(a, b)                                                                 \
-    if (!done && c.p0 == a && c.p1 == b) {                                             \
-      if (launch("fwd_spatial", k_fwd_spatial_t<a, b>, dim3(c.count), dim3(NTHR), c.lds, \
-                 g->win, g->tasks, c.begin, (float2 const *)X, (float2 *)g->U))        \
-        return 1;                                                                      \
-      done = true;                                                                     \
+#define FWD_CASE(a, b)                                                    \
+    if (!done && c.p0 == a && c.p1 == b) {                                \
+      if (fwd_spatial_sized<a, b>(g, c, X)) return 1;                     \
+      done = true;                                                        \
     }
     SPATIAL_SIZES(FWD_CASE)
 #undef FWD_CASE
@@ -1477,8 +1535,7 @@ int agdg_stepsize_fold(AgdDev const *g) {
                 g->rec_len, g->scalar + 1);
 }
 
-int agdg_adj_time(AgdDev const *g, int from_record) {
-  float const *Tin = from_record ? (float const *)nullptr : (float const *)g->T2;
+int agdg_adj_time(AgdDev const *g, float const *Tin, int plain) {
   for (int ci = 0; ci < g->n_tclass; ++ci) {
     AgdClass const &c = g->tclass[ci];
     bool done = false;
@@ -1487,7 +1544,7 @@ int agdg_adj_time(AgdDev const *g, int from_record) {
       if (launch("adj_time", k_adj_time_t<a, b>, dim3(c.count),                          \
                  dim3(time_threads<a * b>()), adj_time_lds<a * b>(), g->win, g->tblocks,   \
                  c.begin, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
-                 g->went, (float const *)g->r, Tin, (float2 *)g->V))                    \
+                 g->went, (float const *)g->r, Tin, (float2 *)g->V, plain))             \
         return 1;                                                                       \
       done = true;                                                                      \
     }
@@ -1496,37 +1553,13 @@ int agdg_adj_time(AgdDev const *g, int from_record) {
     if (!done &&
         launch("adj_time", k_adj_time, dim3(c.count), dim3(NTHR), g->lds_time, g->win,
                g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots, g->tapers,
-               g->tscale, g->went, (float const *)g->r, Tin, (float2 *)g->V))
+               g->tscale, g->went, (float const *)g->r, Tin, (float2 *)g->V, plain))
       return 1;
   }
   return 0;
 }
 
 }  // extern "C"
-
-template <int P0, int P1, int MODE, bool FUSE>
-static int launch_adj_spatial(AgdDev const *g, AgdClass const &c, float alpha, float lamb) {
-  return launch("adj_spatial", k_adj_spatial_t<P0, P1, MODE, FUSE>, dim3(c.count),
-                dim3(NTHR), c.lds, g->win, g->tasks, c.begin, (float2 const *)g->V,
-                (float2 *)g->X, (float2 *)g->U, alpha, lamb, g->scalar);
-}
-
-template <int P0, int P1>
-static int adj_spatial_sized(AgdDev const *g, AgdClass const &c, float alpha, float lamb,
-                             int mode, int fuse) {
-  if (fuse && mode != 0) {
-    fprintf(stderr, "ERROR: fused update requested for mode %d\n", mode);
-    return 1;
-  }
-  switch (mode) {
-    case 0:
-      return fuse ? launch_adj_spatial<P0, P1, 0, true>(g, c, alpha, lamb)
-                  : launch_adj_spatial<P0, P1, 0, false>(g, c, alpha, lamb);
-    case 1: return launch_adj_spatial<P0, P1, 1, false>(g, c, alpha, lamb);
-    case 2: return launch_adj_spatial<P0, P1, 2, false>(g, c, alpha, lamb);
-    default: return launch_adj_spatial<P0, P1, 3, false>(g, c, alpha, lamb);
-  }
-}
 
 extern "C" {
 
@@ -1597,6 +1630,12 @@ int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec) {
                 gt->tile, gt->tile_ptr, gt->tile_ent, gt->ent, src,
                 (float const *)nullptr, gt->taper, gt->taper_len, gt->mute,
                 (float const *)nullptr, rec);
+}
+
+int agdg_window_traces(AgdDev const *g, int64_t const *src_off, int const *src_nt,
+                       float const *data, float *T) {
+  return launch("window_traces", k_window_traces, dim3(g->n_wtr), dim3(NTHR), 0, g->win,
+                g->trace_win, src_off, src_nt, data, T);
 }
 
 int agdg_extract(AgdEntry const *ent, int n_ent, int const *trace_len, float const *rec,

@@ -54,6 +54,8 @@ typedef struct {
   int trace0;            /* first global window-trace index */
   int patch;
   int ug;                /* trace group of the U / V layout (16 or ntr) */
+  float scaler;          /* 1 / sqrt(prod P) (fk.c:157-163) */
+  int spare;
   int64_t x_off;         /* model offset (complex) */
   int64_t u_off;         /* intermediate offset (complex) */
   int64_t t_off;         /* trace buffer offset (float) */
@@ -76,7 +78,8 @@ typedef struct {
   int p0, p1;   /* spatial: padded sizes (p0 = 1 when one space dim);
                    time: p0 = M1, p1 = M2 with M1*M2 = Pt/2 */
   int begin, count;
-  int tb;       /* time classes: traces per block */
+  int tb;       /* time classes: traces per block; specialised spatial
+                   classes: 1 when U / V are tiled by 16 traces */
   int pad;      /* time classes: threads per block */
   size_t lds;
 } AgdClass;

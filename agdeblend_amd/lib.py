@@ -54,6 +54,8 @@ def load():
     fn("agdx_forward", ci, vp, vp, vp)
     fn("agdx_adjoint", ci, vp, vp, vp)
     fn("agdx_run", ci, vp, cf, ci, ci, vp)
+    fn("agdx_run_warm", ci, vp, cf, ci, ci, vp)
+    fn("agdx_warm_model", ci, vp, vp)
     fn("agdx_get_output", ci, vp, vp)
     fn("agdx_finish", ci, vp)
     fn("agdx_get_model", ci, vp, vp)
@@ -71,7 +73,8 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_create", "agdx_create_host", "agdx_destroy", "agdx_model_len",
            "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
-           "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_get_output",
+           "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
+           "agdx_warm_model", "agdx_get_output",
            "agdx_finish",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
            "agdx_timing_get", "agdx_bytes"]
@@ -206,6 +209,19 @@ class Problem:
             x0p = _p(x0)
         if self.L.agdx_run(self.h, float(initial_factor), int(n_its), int(print_freq), x0p):
             raise RuntimeError("agdx_run failed")
+
+    def warm_model(self, y):
+        """Device model set to the warm-start model of data estimate y."""
+        yt = PtrTable([np.asarray(a, np.float32).reshape(-1) for a in y], np.float32)
+        if self.L.agdx_warm_model(self.h, yt.ptr):
+            raise RuntimeError("agdx_warm_model failed")
+
+    def run_warm(self, initial_factor, n_its, warm, print_freq=0):
+        """Inversion started from the warm-start model of data estimate warm."""
+        yt = PtrTable([np.asarray(a, np.float32).reshape(-1) for a in warm], np.float32)
+        if self.L.agdx_run_warm(self.h, float(initial_factor), int(n_its), int(print_freq),
+                                yt.ptr):
+            raise RuntimeError("agdx_run_warm failed")
 
     def iterate(self, n, lamb):
         if self.L.agdx_iterate(self.h, int(n), float(lamb)):

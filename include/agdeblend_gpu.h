@@ -84,6 +84,18 @@ int agdx_adjoint(AgdProblem *p, float const *rec, float *g);
  * Leaves the final model on the device. */
 int agdx_run(AgdProblem *p, float initial_factor, int n_its, int print_freq,
              float const *x0);
+/* Warm-start model of a data estimate y (patches in the layout of
+ * agd_deblend's data): per window and time window s R2C(pad(y_w)), no
+ * tapers, no x2, so that the forward operator gives taper . y_w and, where
+ * the window tapers sum to one, the output of the model is y. Written to
+ * the device model. (The reference starts every inversion from the
+ * thresholded first update, agdeblend.c:584-590: this is the hook a
+ * learned or other warm start plugs into.) */
+int agdx_warm_model(AgdProblem *p, float const *const *y);
+/* agdx_run starting from agdx_warm_model(warm) instead of the thresholded
+ * first update; lambda0 and the schedule are those of the zero start. */
+int agdx_run_warm(AgdProblem *p, float initial_factor, int n_its, int print_freq,
+                  float const *const *warm);
 /* Forward of the current model assembled into the device copy of the patch
  * data (set_output, model.c:1425-1515), without copying it to the host. */
 int agdx_finish(AgdProblem *p);

@@ -1345,13 +1345,65 @@ err:
   return 1;
 }
 
+/* Warm-start model of a data estimate y (patch layout): per window and
+ * time window, x = s R2C(pad(y_w)), with the window's samples taken like
+ * set_output places them (trace buffer sample t <-> data time t - W_t/2,
+ * model.c:1478-1484), no tapers and no x2. Then F x = taper . y_w, and
+ * where the window tapers sum to one the output of x is y. Live-window
+ * data only; wavelets are not deconvolved. (No reference counterpart: the
+ * reference always starts from the thresholded first update,
+ * agdeblend.c:584-590.) */
+static void model_from_data(omodel const *m, real const *const *y,
+                            int const *const *shape, ocpx *x) {
+  int const nth = n_threads();
+  int w;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth)
+#endif
+  {
+    oscratch sc;
+    scratch_alloc(m, &sc);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (w = 0; w < m->n_win; w++) {
+      owin const *wi = &m->win[w];
+      int const nd = wi->ndim;
+      int const wt = wi->W[nd - 1];
+      int const half = wt / 2;
+      int const Pt = wi->P[nd - 1];
+      int const pp = wi->patch;
+      int const nt = m->nt[pp];
+      int tw, t, s;
+      memset(sc.r, 0, (size_t)wi->ntw * wi->npad * sizeof(real));
+      for (t = 0; t < wi->ntr; t++) {
+        int const pt = window_patch_trace(wi, shape[pp], t);
+        int const zt = fk_padded_trace(wi, t);
+        real const *src = y[pp] + (size_t)pt * nt;
+        for (tw = 0; tw < wi->ntw; tw++) {
+          real *z = sc.r + (size_t)tw * wi->npad + (size_t)zt * Pt;
+          for (s = 0; s < wt; s++) {
+            int const ts = tw * half + s - half;
+            z[s] = (ts >= 0 && ts < nt) ? src[ts] * wi->scaler : (real)0;
+          }
+        }
+      }
+      for (tw = 0; tw < wi->ntw; tw++)
+        offt_r2c_nd(nd, wi->P, sc.r + (size_t)tw * wi->npad,
+                    x + wi->model_off + (size_t)tw * wi->nfk_el, sc.f);
+    }
+    scratch_free(&sc);
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Exported entry points                                                     */
 /* ------------------------------------------------------------------------ */
 
-/* agdeblend.c:511-649. Optional x0 (model_len complex) replaces the zero
- * start and skips the initial step when non-NULL (warm-start hook); stats
- * receives the first threshold and stepsize when non-NULL. */
+/* agdeblend.c:511-649. stats receives the first threshold and stepsize
+ * when non-NULL. With warm (patch data, the layout of data) the start model
+ * is model_from_data(warm) instead of the thresholded first update; the
+ * threshold schedule still starts from the first update's max |x|. */
 /* Threshold schedule (agdeblend.c:526-533, :623-627): lamb decays by
  * `decay` per iteration until first_linear_it, then falls linearly to zero
  * at the last iteration; slope is per unit of the first threshold. */
@@ -1370,7 +1422,8 @@ static int deblend_impl(int np, int const *volumes, int const *n_dims,
                         int const *const *ch, int const *const *tt,
                         int const *wl, int const *const *widx,
                         real const *const *wav, real f, int n_its,
-                        int print_freq, real *const *data, double *stats) {
+                        int print_freq, real *const *data, double *stats,
+                        real const *const *warm) {
   real decay, slope;
   int first_linear_it;
   omodel *m;
@@ -1401,7 +1454,11 @@ static int deblend_impl(int np, int const *volumes, int const *n_dims,
   }
   slope *= lamb;
   lamb *= f;
-  model_shrink(m, lamb, x);
+  if (warm != NULL) {
+    model_from_data(m, warm, shape, x);
+  } else {
+    model_shrink(m, lamb, x);
+  }
   if (print_freq > 0) {
     double acc = 0;
     size_t i;
@@ -1444,7 +1501,19 @@ EXPORT int OSYM(deblend)(int np, int const *volumes, int const *n_dims,
                          real const *const *wav, real f, int n_its,
                          int print_freq, real *const *data, double *stats) {
   return deblend_impl(np, volumes, n_dims, wshape, coords, shape, shot, ch, tt,
-                      wl, widx, wav, f, n_its, print_freq, data, stats);
+                      wl, widx, wav, f, n_its, print_freq, data, stats, NULL);
+}
+
+EXPORT int OSYM(deblend_warm)(int np, int const *volumes, int const *n_dims,
+                              int const *const *wshape, int const *const *coords,
+                              int const *const *shape, long const *const *shot,
+                              int const *const *ch, int const *const *tt,
+                              int const *wl, int const *const *widx,
+                              real const *const *wav, real f, int n_its,
+                              int print_freq, real *const *data, double *stats,
+                              real const *const *warm) {
+  return deblend_impl(np, volumes, n_dims, wshape, coords, shape, shot, ch, tt,
+                      wl, widx, wav, f, n_its, print_freq, data, stats, warm);
 }
 
 /* agdeblend.c:651-732 */
@@ -1677,6 +1746,11 @@ EXPORT void OSYM(model_blend_input)(void *h, real const *const *data,
 EXPORT void OSYM(model_output)(void *h, real const *x, int const *const *shape,
                                real *const *data) {
   model_output((omodel *)h, (ocpx const *)x, shape, data);
+}
+
+EXPORT void OSYM(model_from_data)(void *h, real const *const *y,
+                                  int const *const *shape, real *x) {
+  model_from_data((omodel *)h, y, shape, (ocpx *)x);
 }
 
 /* FK forward / adjoint of window w alone (out: ntr*nconv real) */

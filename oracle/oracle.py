@@ -48,6 +48,9 @@ def _declare(L, key):
         setattr(L, name, f)
 
     fn("deblend", ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, real, ci, ci, vp, vp)
+    fn("deblend_warm", ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, real, ci, ci, vp, vp,
+       vp)
+    fn("model_from_data", None, vp, vp, vp, vp)
     fn("blend", ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, ci, vp, vp, vp, vp, vp, vp)
     fn("model_new", vp, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp)
     fn("model_free", None, vp)
@@ -217,9 +220,10 @@ class _DeblendArgs:
 def deblend(volumes, n_dims, window_shapes, coords, shapes, shottimes, channels,
             trace_types, data, initial_factor=1.0, n_its=100, print_freq=0,
             wavelet_lengths=None, wavelet_idxs=None, wavelets=None,
-            double=False, threads=None):
+            double=False, threads=None, warm=None):
     """Run the oracle's agd_deblend. Returns (list of output arrays, stats)
-    where stats = (initial threshold lambda0, stepsize)."""
+    where stats = (initial threshold lambda0, stepsize). warm: optional data
+    estimate (patch arrays) whose model_from_data replaces the start model."""
     dt = _rdt(double)
     L = lib(double)
     if threads is not None:
@@ -229,7 +233,12 @@ def deblend(volumes, n_dims, window_shapes, coords, shapes, shottimes, channels,
                         wavelets, double)
     d = PtrTable([np.array(x, dtype=dt, copy=True) for x in data], dt, writable=True)
     stats = np.zeros(2, np.float64)
-    rc = L.deblend(*args.head(), initial_factor, n_its, print_freq, d.ptr, _p(stats))
+    if warm is not None:
+        wt = PtrTable([np.asarray(x, dtype=dt).reshape(-1) for x in warm], dt)
+        rc = L.deblend_warm(*args.head(), initial_factor, n_its, print_freq, d.ptr,
+                            _p(stats), wt.ptr)
+    else:
+        rc = L.deblend(*args.head(), initial_factor, n_its, print_freq, d.ptr, _p(stats))
     if rc != 0:
         raise RuntimeError("oracle deblend failed")
     return [a.reshape(np.shape(x)) for a, x in zip(d.np, data)], stats
@@ -376,6 +385,14 @@ class Model:
                        writable=True)
         self.L.model_output(self.h, _p(x), sh.ptr, out.ptr)
         return [o.reshape(s) for o, s in zip(out.np, shapes)]
+
+    def from_data(self, y, shapes):
+        """Warm-start model s R2C(pad(y_w)) of patch data y."""
+        yt = PtrTable([np.asarray(a, self.dt).reshape(-1) for a in y], self.dt)
+        sh = PtrTable(shapes, C_INT)
+        x = self.zeros_model()
+        self.L.model_from_data(self.h, yt.ptr, sh.ptr, _p(x))
+        return x
 
     def window_fk_forward(self, w, xw):
         wi = self.windows[w]

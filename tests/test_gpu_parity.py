@@ -90,7 +90,11 @@ def _spec_geometry(shape, windows, seed, interval=None):
 SPEC = [("spec_20x40_320", _spec_geometry((16, 64, 512), (16, 32, 256), 11)),
         ("spec_40x40_640", _spec_geometry((32, 64, 1024), (32, 32, 512), 12)),
         ("spec_10x40_320", _spec_geometry((8, 64, 512), (8, 32, 256), 13)),
-        ("spec_1x40_320", _spec_geometry((128, 1024), (32, 256), 14))]
+        ("spec_1x40_320", _spec_geometry((128, 1024), (32, 256), 14)),
+        # specialised spatial size with a generic time size: untiled U / V
+        ("spec_20x40_t250", _spec_geometry((16, 64, 500), (16, 32, 200), 15)),
+        # specialised spatial size whose windows hold 8 traces: untiled
+        ("spec_1x10_320", _spec_geometry((64, 512), (8, 256), 16))]
 
 
 def _geometries():
@@ -258,4 +262,68 @@ def test_blend_input_record_matches_oracle():
     ref = om.blend_input(prob["truth"], g["shottimes"], g["channels"], g["trace_types"])
     got = pm.set_data(prob["truth"], want_record=True)
     np.testing.assert_array_equal(got, ref)
+    pm.close()
+
+
+# ---------------------------------------------------------------------------
+# Warm-start hook: x0 = s R2C(pad(y_w)) per window (no tapers, no x2)
+# ---------------------------------------------------------------------------
+WARM = [g for g in GEOS if g[0].startswith(("spec_", "c0_", "c1_", "random"))]
+
+
+@pytest.mark.parametrize("name,geo", WARM, ids=[g[0] for g in WARM])
+def test_warm_model_matches_oracle(name, geo):
+    kw = _kw(geo)
+    om = O.Model(double=True, **kw)
+    if om.n_windows == 0 or om.record_len == 0:
+        pytest.skip("empty geometry")
+    rng = np.random.default_rng(zlib.crc32(("warm" + name).encode()))
+    shapes = [tuple(int(v) for v in s) for s in geo["shapes"]]
+    y = [rng.standard_normal(s) for s in shapes]
+    ref = om.from_data(y, [list(s) for s in shapes])
+    pm = L.Problem(**kw)
+    pm.warm_model([a.astype(np.float32) for a in y])
+    got = _to_oracle_layout(om, pm.model().astype(np.complex128))
+    rel = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30)
+    assert rel <= 2e-6, rel
+    pm.close()
+
+
+@pytest.mark.parametrize("name", ["spec_20x40_320", "spec_40x40_640", "spec_1x40_320"])
+def test_warm_model_output_is_the_data(name):
+    """Windows of nominal length: the tapers sum to one, so assembling the
+    output of the warm-start model of y gives y back."""
+    geo = dict(SPEC)[name]
+    kw = _kw(geo)
+    shapes = [tuple(int(v) for v in s) for s in geo["shapes"]]
+    rng = np.random.default_rng(3)
+    y = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    pm = L.Problem(**kw)
+    pm.set_data(y)
+    pm.warm_model(y)
+    out = pm.output()
+    for a, b in zip(out, y):
+        assert np.abs(a - b).max() <= 2e-5 * np.abs(b).max(), np.abs(a - b).max()
+    pm.close()
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_warm_deblend_matches_fp64_oracle(cfg):
+    prob = synth.SMALL[cfg]()
+    g = prob["geometry"]
+    kw = _kw(g)
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    # a data estimate between the blended input and the truth
+    warm = [0.5 * (np.asarray(a, np.float64) + b) for a, b in zip(blended, prob["truth"])]
+    n_its = 8
+    ref, _ = O.deblend(data=blended, initial_factor=1.0, n_its=n_its, double=True, warm=warm,
+                       **kw)
+    pm = L.Problem(**kw)
+    pm.set_data(blended)
+    pm.run_warm(1.0, n_its, [a.astype(np.float32) for a in warm])
+    out = pm.output()
+    num = sum(float(np.sum((np.asarray(a, np.float64) - b) ** 2)) for a, b in zip(out, ref))
+    den = sum(float(np.sum(np.asarray(b, np.float64) ** 2)) for b in ref)
+    rel = (num / den) ** 0.5
+    assert rel <= 1e-4, rel
     pm.close()

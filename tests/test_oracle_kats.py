@@ -754,3 +754,6 @@ def test_output_partition_of_unity(shape, wshape):
             x[o:o + blk] = X.reshape(-1)
     out = m.output(x, [shape])[0]
     assert np.allclose(out, y, rtol=0, atol=1e-12), np.abs(out - y).max()
+    # the oracle's warm-start model is the same construction
+    xo = m.from_data([y], [shape])
+    assert np.allclose(xo, x, rtol=0, atol=1e-11 * np.abs(x).max()), np.abs(xo - x).max()
