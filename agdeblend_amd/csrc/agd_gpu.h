@@ -66,6 +66,17 @@ typedef struct {
   float *d;                   /* blended input record */
   float *r;                   /* forward / residual record */
   unsigned int *scalar;       /* device scalars (max bits etc.) */
+  /* window range [w0, w1) of the transform launches (all windows by
+   * default); wtrace0 (host) holds each window's first global trace and
+   * n_wtr at index n_win */
+  int w0, w1;
+  int const *wtrace0;
+  /* device threshold of captured iterations: when lamb_dev is set, the
+   * update kernels read their threshold from lamb_dev[0], which
+   * agdg_next_lamb advances through lamb_sched */
+  float *lamb_dev;
+  float *lamb_sched;
+  int *lamb_idx;
   double *partials;           /* reduction partials */
   int n_partials;
 } AgdDev;
@@ -134,6 +145,14 @@ int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec);
 int agdg_extract(AgdEntry const *ent, int n_ent, int const *trace_len,
                  float const *rec, float *out);
 /* timing of the library's kernels (per kernel name accumulators) */
+/* lamb_dev[0] = lamb_sched[*lamb_idx]; *lamb_idx += 1 */
+int agdg_next_lamb(AgdDev const *g);
+/* stream capture of the library stream into an executable graph */
+int agdg_graph_begin(void);
+int agdg_graph_end(void **exec);
+int agdg_graph_launch(void *exec);
+void agdg_graph_destroy(void *exec);
+int agdg_timing_enabled(void);
 int agdg_timing(int enable);
 int agdg_timing_get(char const *name, double *ms, long *count);
 

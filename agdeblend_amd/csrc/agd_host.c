@@ -666,6 +666,11 @@ struct AgdProblem {
   AgdClass *sclass, *tclass;
   int n_sclass, n_tclass;
   int fuse;      /* adjoint kernels also write the next forward's stage 1 */
+  int *wtrace0;  /* per window: first global window trace; n_wtr at n_win */
+  int *chunk;    /* window chunk boundaries [n_chunk + 1] */
+  int n_chunk;
+  float *lamb_cur;  /* device threshold read by captured iterations */
+  int sched_cap;    /* capacity of dev.lamb_sched (floats) */
   int u_valid;   /* U holds the forward spatial transform of X */
 };
 
@@ -702,8 +707,12 @@ EXPORT void agdx_destroy(AgdProblem *p) {
   free(p->wcoord);
   free(p->wpatch_trace);
   free(p->dcoord);
+  for (i = 0; i < p->n_sclass; i++) free((void *)p->sclass[i].wpos);
+  for (i = 0; i < p->n_tclass; i++) free((void *)p->tclass[i].wpos);
   free(p->sclass);
   free(p->tclass);
+  free(p->wtrace0);
+  free(p->chunk);
   blendcfg_free(&p->cfg);
   free(p);
 }
@@ -1092,10 +1101,14 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
   }
   for (c = 0; c < (int)sc.n; c++) {
     AgdClass *cl = (AgdClass *)sc.p + c;
+    int *wpos = (int *)malloc(((size_t)p->n_win + 1) * sizeof(int));
+    if (wpos == NULL) return 1;
+    cl->wpos = wpos;
     cl->begin = (int)tasks->n;
     for (w = 0; w < p->n_win; w++) {
       AgdWin const *W = &p->win[w];
       int a, b, nk, tw, k0;
+      wpos[w] = (int)tasks->n - cl->begin;
       spatial_key(W, &a, &b);
       if (a != cl->p0 || b != cl->p1 || (a && (W->ug != W->ntr) != cl->tb)) continue;
       if (a) {
@@ -1133,14 +1146,17 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
       }
     }
     cl->count = (int)tasks->n - cl->begin;
+    wpos[p->n_win] = cl->count;
   }
   for (c = 0; c < (int)tc.n; c++) {
     AgdClass *cl = (AgdClass *)tc.p + c;
     int tb;
-    /* traces per block: as many as fit the LDS budget, up to tb_max
-     * (specialised kernels: AGD_TIME_TB, default 8, with AGD_TIME_THREADS
-     * threads, default 256) */
+    /* traces per block: one tile for the specialised kernels, else as
+     * many as fit the LDS budget, up to 16 */
     int tb_max = cl->p0 ? TIME_TILE : 16;
+    int *wpos = (int *)malloc(((size_t)p->n_win + 1) * sizeof(int));
+    if (wpos == NULL) return 1;
+    cl->wpos = wpos;
     cl->pad = 0;
     for (tb = tb_max; tb >= 1; tb /= 2) {
       size_t need = 0;
@@ -1170,6 +1186,7 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
     for (w = 0; w < p->n_win; w++) {
       AgdWin const *W = &p->win[w];
       int a, b, t0;
+      wpos[w] = (int)tblocks->n - cl->begin;
       time_key(W, &a, &b);
       if (a != cl->p0 || b != cl->p1) continue;
       for (t0 = 0; t0 < W->ntr; t0 += tb) {
@@ -1180,6 +1197,7 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
       }
     }
     cl->count = (int)tblocks->n - cl->begin;
+    wpos[p->n_win] = cl->count;
   }
   p->sclass = (AgdClass *)sc.p;
   p->n_sclass = (int)sc.n;
@@ -1488,6 +1506,33 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->dev.n_sclass = p->n_sclass;
     p->dev.tclass = p->tclass;
     p->dev.n_tclass = p->n_tclass;
+    /* window ranges: all windows by default; chunks of windows whose
+     * intermediates U and V together stay below AGD_CHUNK_MB (unset or 0:
+     * one chunk) for the chunked iteration order */
+    p->wtrace0 = (int *)malloc(((size_t)p->n_win + 1) * sizeof(int));
+    p->chunk = (int *)malloc(((size_t)p->n_win + 2) * sizeof(int));
+    if (p->wtrace0 == NULL || p->chunk == NULL) goto err;
+    for (w = 0; w < p->n_win; w++) p->wtrace0[w] = p->win[w].trace0;
+    p->wtrace0[p->n_win] = p->n_wtr;
+    p->dev.wtrace0 = p->wtrace0;
+    p->dev.w0 = 0;
+    p->dev.w1 = p->n_win;
+    {
+      char const *e = getenv("AGD_CHUNK_MB");
+      double const cap = e ? atof(e) * 1e6 : 0.0;
+      double acc = 0.0;
+      p->n_chunk = 0;
+      p->chunk[0] = 0;
+      for (w = 0; w < p->n_win; w++) {
+        double const b = 2.0 * 8.0 * (double)p->win[w].ntw * p->win[w].Ft * p->win[w].ntr;
+        if (cap > 0.0 && acc > 0.0 && acc + b > cap) {
+          p->chunk[++p->n_chunk] = w;
+          acc = 0.0;
+        }
+        acc += b;
+      }
+      p->chunk[++p->n_chunk] = p->n_win;
+    }
     p->dev.tblocks = (int const *)dev_upload(p, tblocks.p, tblocks.n * 2 * sizeof(int));
     p->dev.n_tblocks = (int)tblocks.n;
     p->dev.lds_time = lds_t;
@@ -1504,6 +1549,11 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->dev.d = (float *)dev_alloc(p, (size_t)p->cfg.rec_len * 4 + 16);
     p->dev.r = (float *)dev_alloc(p, (size_t)p->cfg.rec_len * 4 + 16);
     p->dev.scalar = (unsigned int *)dev_alloc(p, 64);
+    p->sched_cap = 1 << 16;
+    p->dev.lamb_sched = (float *)dev_alloc(p, (size_t)p->sched_cap * sizeof(float));
+    p->dev.lamb_idx = (int *)dev_alloc(p, 64);
+    p->lamb_cur = (float *)dev_alloc(p, 64);
+    p->dev.lamb_dev = NULL;
     p->dev.n_partials = 1024;
     p->dev.partials = (double *)dev_alloc(p, 1024 * sizeof(double));
     p->bytes_M = (double)xo * 8.0;
@@ -1521,7 +1571,8 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
         !p->dev.tscale || !p->dev.tconst || !p->dev.went || !p->dev.trace_win ||
         !p->dev.tasks || !p->dev.tblocks || !p->dev.wsrc_off || !p->dev.X ||
         !p->dev.U || !p->dev.V || !p->dev.T || !p->dev.T2 || !p->dev.d || !p->dev.r ||
-        !p->dev.scalar || !p->dev.partials || !p->dev.tile_ptr || !p->dev.tile_ent ||
+        !p->dev.scalar || !p->dev.lamb_sched || !p->dev.lamb_idx || !p->lamb_cur ||
+        !p->dev.partials || !p->dev.tile_ptr || !p->dev.tile_ent ||
         !p->dev.mute || !p->dev.wspec)
       goto err;
   }
@@ -1897,6 +1948,108 @@ EXPORT int agdx_warm_model(AgdProblem *p, float const *const *y) {
 static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float const *x0,
                     float const *const *warm);
 
+static int forward_record(AgdProblem *p, int resid);
+static int adjoint_update(AgdProblem *p, float alpha, float lamb, int mode);
+
+/* One iteration in window-chunk order, starting from the residual record of
+ * the current model: for each chunk of windows, the adjoint update with
+ * threshold lamb and then (unless last) the chunk's forward traces for the
+ * next iteration, whose intermediates are still in cache; finally the
+ * residual record of the new model. The same kernels over the same
+ * elements as forward_record + adjoint_update, in another launch order. */
+static int chunked_iteration(AgdProblem *p, float alpha, float lamb, int last) {
+  int k, rc = 0;
+  for (k = 0; k < p->n_chunk && rc == 0; k++) {
+    p->dev.w0 = p->chunk[k];
+    p->dev.w1 = p->chunk[k + 1];
+    if (adjoint_update(p, alpha, lamb, 0)) rc = 1;
+    if (rc == 0 && !last) {
+      if (!p->fuse && agdg_fwd_spatial(&p->dev, p->dev.X)) rc = 1;
+      if (rc == 0 && agdg_fwd_time(&p->dev)) rc = 1;
+      if (rc == 0 && p->use_wavelets && agdg_wavelet(&p->dev, 0)) rc = 1;
+    }
+  }
+  p->dev.w0 = 0;
+  p->dev.w1 = p->n_win;
+  if (rc == 0 && !last && agdg_blend_traces(&p->dev, 1)) rc = 1;
+  p->u_valid = p->fuse;
+  return rc;
+}
+
+/* n iterations with thresholds lamb[0..n): one iteration captured into a
+ * graph whose update kernels read the threshold that agdg_next_lamb copies
+ * from the device schedule, launched n times (chunked: in window-chunk
+ * order, none of them the last iteration) */
+static int run_captured(AgdProblem *p, float alpha, float const *lamb, int n, int chunked) {
+  void *exec = NULL;
+  int zero = 0;
+  int rc = 0;
+  if (agdg_h2d(p->dev.lamb_sched, lamb, (size_t)n * sizeof(float))) return 1;
+  if (agdg_h2d(p->dev.lamb_idx, &zero, sizeof(zero))) return 1;
+  if (agdg_graph_begin()) return 1;
+  p->dev.lamb_dev = p->lamb_cur;
+  if (agdg_next_lamb(&p->dev)) rc = 1;
+  else if (chunked) rc = chunked_iteration(p, alpha, 0.0f, 0);
+  else if (forward_record(p, 1) || adjoint_update(p, alpha, 0.0f, 0)) rc = 1;
+  p->dev.lamb_dev = NULL;
+  {
+    /* end the capture even after a failed launch */
+    int const rc2 = agdg_graph_end(&exec);
+    if (rc || rc2) {
+      agdg_graph_destroy(exec);
+      return 1;
+    }
+  }
+  for (int i = 0; i < n && rc == 0; i++) rc = agdg_graph_launch(exec);
+  if (rc == 0) rc = agdg_sync();
+  agdg_graph_destroy(exec);
+  return rc;
+}
+
+/* Iterations 1 .. n_its - 1 with thresholds sched[it], from the current
+ * model (forward not yet computed). */
+static int run_loop(AgdProblem *p, float alpha, float const *sched, int n_its,
+                    int print_freq) {
+  int it, rc = 0;
+  /* from the second iteration on, every iteration launches the same
+   * kernels with the same arguments except the threshold: capture one
+   * into a graph that reads its threshold from the device schedule, and
+   * replay it (not while printing residuals or timing kernels) */
+  int const graph = n_its > 4 && n_its - 2 <= p->sched_cap && print_freq <= 0 &&
+                    !agdg_timing_enabled() && getenv("AGD_NO_GRAPH") == NULL;
+  int const chunked = p->n_chunk > 1;
+  if (chunked && n_its > 1 && forward_record(p, 1)) rc = 1;
+  for (it = 1; it < n_its && rc == 0; it++) {
+    if (chunked) {
+      if (graph && it == 2) {
+        /* iterations 2 .. n_its - 2 replayed, the last one eager */
+        rc = run_captured(p, alpha, sched + 2, n_its - 3, 1);
+        it = n_its - 2;
+        continue;
+      }
+      if (print_freq > 0 && it % print_freq == 0) {
+        double r2 = 0.0;
+        if (agdg_norm2(&p->dev, p->dev.r, p->cfg.rec_len, &r2)) rc = 1;
+        else printf("Iteration %d Residual %f\n", it, sqrt(r2));
+      }
+      if (rc == 0) rc = chunked_iteration(p, alpha, sched[it], it == n_its - 1);
+      continue;
+    }
+    if (graph && it == 2) {
+      rc = run_captured(p, alpha, sched + 2, n_its - 2, 0);
+      break;
+    }
+    if (forward_record(p, 1)) rc = 1;
+    if (rc == 0 && print_freq > 0 && it % print_freq == 0) {
+      double r2 = 0.0;
+      if (agdg_norm2(&p->dev, p->dev.r, p->cfg.rec_len, &r2)) rc = 1;
+      else printf("Iteration %d Residual %f\n", it, sqrt(r2));
+    }
+    if (rc == 0 && adjoint_update(p, alpha, sched[it], 0)) rc = 1;
+  }
+  return rc;
+}
+
 EXPORT int agdx_run(AgdProblem *p, float f, int n_its, int print_freq,
                     float const *x0) {
   return run_impl(p, f, n_its, print_freq, x0, NULL);
@@ -1976,28 +2129,34 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
     if (agdg_norm2(&p->dev, p->dev.d, p->cfg.rec_len, &r2)) return 1;
     printf("Iteration 0 Residual %f\n", sqrt(r2));
   }
-  for (it = 1; it < n_its; it++) {
-    float next;
-    if (forward_record(p, 1)) return 1;
-    if (print_freq > 0 && it % print_freq == 0) {
-      double r2 = 0.0;
-      if (agdg_norm2(&p->dev, p->dev.r, p->cfg.rec_len, &r2)) return 1;
-      printf("Iteration %d Residual %f\n", it, sqrt(r2));
+  {
+    /* thresholds of iterations 1..n_its-1, in the reference's float
+     * arithmetic (agdeblend.c:623-627) */
+    float *sched = (float *)malloc((size_t)(n_its > 1 ? n_its : 1) * sizeof(float));
+    int rc = 0;
+    if (sched == NULL) return 1;
+    for (it = 1; it < n_its; it++) {
+      if (it < first_linear_it) lamb = lamb * decay;
+      else lamb = (float)(n_its - 1 - it) * slope;
+      sched[it] = lamb;
     }
-    if (it < first_linear_it) next = lamb * decay;
-    else next = (float)(n_its - 1 - it) * slope;
-    lamb = next;
-    if (adjoint_update(p, alpha, lamb, 0)) return 1;
+    rc = run_loop(p, alpha, sched, n_its, print_freq);
+    free(sched);
+    if (rc) return 1;
   }
   return agdg_sync();
 }
 
 EXPORT int agdx_iterate(AgdProblem *p, int n, float lamb) {
-  int it;
-  for (it = 0; it < n; it++) {
-    if (forward_record(p, 1)) return 1;
-    if (adjoint_update(p, p->stepsize, lamb, 0)) return 1;
-  }
+  float *sched;
+  int it, rc;
+  if (n <= 0) return agdg_sync();
+  sched = (float *)malloc(((size_t)n + 1) * sizeof(float));
+  if (sched == NULL) return 1;
+  for (it = 0; it <= n; it++) sched[it] = lamb;
+  rc = run_loop(p, p->stepsize, sched, n + 1, 0);
+  free(sched);
+  if (rc) return 1;
   return agdg_sync();
 }
 

@@ -556,7 +556,9 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float2 const *__restrict__ V, float2 *__restrict__ X, float alpha,
-    float lamb, int mode, unsigned int *__restrict__ maxbits) {
+    float lamb, float const *__restrict__ lamb_dev, int mode,
+    unsigned int *__restrict__ maxbits) {
+  if (lamb_dev != nullptr) lamb = lamb_dev[0];
   extern __shared__ float2 sm[];
   AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const w = wins[t.win];
@@ -715,13 +717,13 @@ __global__ void k_norm2(float const *__restrict__ v, int64_t n,
 // multiplied in the reference, which scales them by N (wavelet.c:273).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NTHR) void k_wavelet(
-    AgdWin const *__restrict__ wins, int const *__restrict__ trace_win,
+    AgdWin const *__restrict__ wins, int const *__restrict__ trace_win, int trace_first,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float2 const *__restrict__ wspec, int const *__restrict__ wsrc_off,
     int adjoint, AgdEntry const *__restrict__ went, float const *__restrict__ rec,
     float *__restrict__ T, float *__restrict__ T2) {
   extern __shared__ float2 sm[];
-  int const g = blockIdx.x;
+  int const g = trace_first + blockIdx.x;
   AgdWin const w = wins[trace_win[g]];
   int const tr = g - w.trace0;
   int const N = w.nconv;
@@ -811,6 +813,18 @@ __global__ __launch_bounds__(NTHR) void k_wavelet(
 // ---------------------------------------------------------------------------
 // Output assembly and blend extraction
 // ---------------------------------------------------------------------------
+// Threshold of the next iteration of a captured loop: lamb_dev[0] =
+// sched[idx], idx += 1 (one lane; vector stores only).
+__global__ void k_next_lamb(float const *__restrict__ sched, int *__restrict__ idx,
+                            float *__restrict__ lamb_dev) {
+  int const lane = threadIdx.x;
+  if (lane == 0) {
+    int const i = idx[lane];
+    lamb_dev[lane] = sched[i];
+    idx[lane] = i + 1;
+  }
+}
+
 // Window traces of patch data (the inverse placement of set_output,
 // model.c:1478-1484): trace buffer sample t of a window trace is data sample
 // t - W_t/2 of its patch trace, zero outside the data. One block per global
@@ -960,12 +974,14 @@ template <int P0, int P1, bool TILED, int MODE, bool FUSE>
 __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     float2 const *__restrict__ V, float2 *__restrict__ X, float2 *__restrict__ U,
-    float alpha, float lamb, unsigned int *__restrict__ maxbits) {
+    float alpha, float lamb, float const *__restrict__ lamb_dev,
+    unsigned int *__restrict__ maxbits) {
   extern __shared__ float2 sm[];
   constexpr int LD = P1 + 1;
   constexpr int S = P0 * P1;
   constexpr int W0 = win_len<P0>();
   constexpr int W1 = win_len<P1>();
+  if (lamb_dev != nullptr) lamb = lamb_dev[0];
   AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const &w = wins[t.win];
   int const nk = t.nk;
@@ -1331,6 +1347,13 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   X(20, 40) X(40, 10) X(40, 20) X(40, 40)
 #define TIME_SIZES(X) X(5, 8) X(10, 8) X(10, 16) X(20, 16)
 
+// units of class c within the launch's window range: [*first, *first + n)
+static inline int class_range(AgdDev const *g, AgdClass const &c, int *first) {
+  int const a = c.wpos[g->w0], b = c.wpos[g->w1];
+  *first = c.begin + a;
+  return b - a;
+}
+
 // the specialised kernels may use up to the CU's 160 KB of LDS: set once per
 // kernel instance, on its first launch
 static int allow_lds(void const *f) {
@@ -1343,9 +1366,13 @@ template <int P0, int P1, bool TILED, int MODE, bool FUSE>
 static int launch_adj_spatial(AgdDev const *g, AgdClass const &c, float alpha, float lamb) {
   static int const lds_ok = allow_lds((void const *)k_adj_spatial_t<P0, P1, TILED, MODE, FUSE>);
   (void)lds_ok;
-  return launch("adj_spatial", k_adj_spatial_t<P0, P1, TILED, MODE, FUSE>, dim3(c.count),
-                dim3(NTHR), c.lds, g->win, g->tasks, c.begin, (float2 const *)g->V,
-                (float2 *)g->X, (float2 *)g->U, alpha, lamb, g->scalar);
+  int first;
+  int const n = class_range(g, c, &first);
+  if (n <= 0) return 0;
+  return launch("adj_spatial", k_adj_spatial_t<P0, P1, TILED, MODE, FUSE>, dim3(n),
+                dim3(NTHR), c.lds, g->win, g->tasks, first, (float2 const *)g->V,
+                (float2 *)g->X, (float2 *)g->U, alpha, lamb, (float const *)g->lamb_dev,
+                g->scalar);
 }
 
 template <int P0, int P1, bool TILED>
@@ -1376,8 +1403,11 @@ template <int P0, int P1, bool TILED>
 static int launch_fwd_spatial(AgdDev const *g, AgdClass const &c, float const *X) {
   static int const lds_ok = allow_lds((void const *)k_fwd_spatial_t<P0, P1, TILED>);
   (void)lds_ok;
-  return launch("fwd_spatial", k_fwd_spatial_t<P0, P1, TILED>, dim3(c.count), dim3(NTHR),
-                c.lds, g->win, g->tasks, c.begin, (float2 const *)X, (float2 *)g->U);
+  int first;
+  int const n = class_range(g, c, &first);
+  if (n <= 0) return 0;
+  return launch("fwd_spatial", k_fwd_spatial_t<P0, P1, TILED>, dim3(n), dim3(NTHR), c.lds,
+                g->win, g->tasks, first, (float2 const *)X, (float2 *)g->U);
 }
 
 template <int P0, int P1>
@@ -1482,9 +1512,11 @@ int agdg_fwd_spatial(AgdDev const *g, float const *X) {
     }
     SPATIAL_SIZES(FWD_CASE)
 #undef FWD_CASE
-    if (!done &&
-        launch("fwd_spatial", k_fwd_spatial, dim3(c.count), dim3(NTHR), g->lds_spatial,
-               g->win, g->tasks, c.begin, g->plans, (float2 const *)g->roots,
+    int first;
+    int const n = class_range(g, c, &first);
+    if (!done && n > 0 &&
+        launch("fwd_spatial", k_fwd_spatial, dim3(n), dim3(NTHR), g->lds_spatial,
+               g->win, g->tasks, first, g->plans, (float2 const *)g->roots,
                (float2 const *)X, (float2 *)g->U))
       return 1;
   }
@@ -1495,20 +1527,23 @@ int agdg_fwd_time(AgdDev const *g) {
   for (int ci = 0; ci < g->n_tclass; ++ci) {
     AgdClass const &c = g->tclass[ci];
     bool done = false;
+    int first;
+    int const n = class_range(g, c, &first);
 #define FT_CASE(a, b)                                                                   \
     if (!done && c.p0 == a && c.p1 == b) {                                              \
-      if (launch("fwd_time", k_fwd_time_t<a, b>, dim3(c.count),                          \
+      if (n > 0 &&                                                                      \
+          launch("fwd_time", k_fwd_time_t<a, b>, dim3(n),                                 \
                  dim3(time_threads<a * b>()), fwd_time_lds<a * b>(), g->win, g->tblocks,   \
-                 c.begin, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
+                 first, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
                  (float2 const *)g->U, g->T))                                           \
         return 1;                                                                       \
       done = true;                                                                      \
     }
     TIME_SIZES(FT_CASE)
 #undef FT_CASE
-    if (!done &&
-        launch("fwd_time", k_fwd_time, dim3(c.count), dim3(NTHR), g->lds_time, g->win,
-               g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots, g->tapers,
+    if (!done && n > 0 &&
+        launch("fwd_time", k_fwd_time, dim3(n), dim3(NTHR), g->lds_time, g->win,
+               g->tblocks, first, c.tb, g->plans, (float2 const *)g->roots, g->tapers,
                g->tscale, (float2 const *)g->U, g->T))
       return 1;
   }
@@ -1539,20 +1574,23 @@ int agdg_adj_time(AgdDev const *g, float const *Tin, int plain) {
   for (int ci = 0; ci < g->n_tclass; ++ci) {
     AgdClass const &c = g->tclass[ci];
     bool done = false;
+    int first;
+    int const n = class_range(g, c, &first);
 #define AT_CASE(a, b)                                                                   \
     if (!done && c.p0 == a && c.p1 == b) {                                              \
-      if (launch("adj_time", k_adj_time_t<a, b>, dim3(c.count),                          \
+      if (n > 0 &&                                                                      \
+          launch("adj_time", k_adj_time_t<a, b>, dim3(n),                                 \
                  dim3(time_threads<a * b>()), adj_time_lds<a * b>(), g->win, g->tblocks,   \
-                 c.begin, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
+                 first, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
                  g->went, (float const *)g->r, Tin, (float2 *)g->V, plain))             \
         return 1;                                                                       \
       done = true;                                                                      \
     }
     TIME_SIZES(AT_CASE)
 #undef AT_CASE
-    if (!done &&
-        launch("adj_time", k_adj_time, dim3(c.count), dim3(NTHR), g->lds_time, g->win,
-               g->tblocks, c.begin, c.tb, g->plans, (float2 const *)g->roots, g->tapers,
+    if (!done && n > 0 &&
+        launch("adj_time", k_adj_time, dim3(n), dim3(NTHR), g->lds_time, g->win,
+               g->tblocks, first, c.tb, g->plans, (float2 const *)g->roots, g->tapers,
                g->tscale, g->went, (float const *)g->r, Tin, (float2 *)g->V, plain))
       return 1;
   }
@@ -1579,9 +1617,13 @@ int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fus
         fprintf(stderr, "ERROR: fused update requested for a generic window size\n");
         return 1;
       }
-      if (launch("adj_spatial", k_adj_spatial, dim3(c.count), dim3(NTHR), g->lds_spatial,
-                 g->win, g->tasks, c.begin, g->plans, (float2 const *)g->roots,
-                 (float2 const *)g->V, (float2 *)g->X, alpha, lamb, mode, g->scalar))
+      int first;
+      int const n = class_range(g, c, &first);
+      if (n > 0 &&
+          launch("adj_spatial", k_adj_spatial, dim3(n), dim3(NTHR), g->lds_spatial,
+                 g->win, g->tasks, first, g->plans, (float2 const *)g->roots,
+                 (float2 const *)g->V, (float2 *)g->X, alpha, lamb,
+                 (float const *)g->lamb_dev, mode, g->scalar))
         return 1;
     }
   }
@@ -1598,8 +1640,11 @@ int agdg_shrink(AgdDev const *g, int64_t n_complex, float lamb) {
 
 int agdg_wavelet(AgdDev const *g, int adjoint) {
   if (!g->use_wavelets) return 0;
-  return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet, dim3(g->n_wtr),
-                dim3(NTHR), g->lds_wavelet, g->win, g->trace_win,
+  int const first = g->wtrace0[g->w0];
+  int const n = g->wtrace0[g->w1] - first;
+  if (n <= 0) return 0;
+  return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet, dim3(n),
+                dim3(NTHR), g->lds_wavelet, g->win, g->trace_win, first,
                 g->plans, (float2 const *)g->roots, (float2 const *)g->wspec,
                 g->wsrc_off, adjoint, g->went, (float const *)g->r, g->T, g->T2);
 }
@@ -1631,6 +1676,38 @@ int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec) {
                 (float const *)nullptr, gt->taper, gt->taper_len, gt->mute,
                 (float const *)nullptr, rec);
 }
+
+int agdg_next_lamb(AgdDev const *g) {
+  return launch("next_lamb", k_next_lamb, dim3(1), dim3(64), 0, (float const *)g->lamb_sched,
+                g->lamb_idx, g->lamb_dev);
+}
+
+int agdg_graph_begin(void) {
+  return hip_check(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal),
+                   "hipStreamBeginCapture");
+}
+
+int agdg_graph_end(void **exec) {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t ge = nullptr;
+  *exec = nullptr;
+  if (hip_check(hipStreamEndCapture(g_stream, &graph), "hipStreamEndCapture")) return 1;
+  hipError_t const e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (hip_check(e, "hipGraphInstantiate")) return 1;
+  *exec = (void *)ge;
+  return 0;
+}
+
+int agdg_graph_launch(void *exec) {
+  return hip_check(hipGraphLaunch((hipGraphExec_t)exec, g_stream), "hipGraphLaunch");
+}
+
+void agdg_graph_destroy(void *exec) {
+  if (exec != nullptr) (void)hipGraphExecDestroy((hipGraphExec_t)exec);
+}
+
+int agdg_timing_enabled(void) { return g_timing; }
 
 int agdg_window_traces(AgdDev const *g, int64_t const *src_off, int const *src_nt,
                        float const *data, float *T) {

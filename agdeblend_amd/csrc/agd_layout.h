@@ -80,8 +80,11 @@ typedef struct {
   int begin, count;
   int tb;       /* time classes: traces per block; specialised spatial
                    classes: 1 when U / V are tiled by 16 traces */
-  int pad;      /* time classes: threads per block */
+  int pad;
   size_t lds;
+  int const *wpos; /* host array [n_win + 1]: first unit of the class that
+                      belongs to window w or later (units are in window
+                      order), relative to begin */
 } AgdClass;
 
 /* A unit of work for the spatial (per time-frequency chunk) kernels */
