@@ -109,8 +109,8 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) {
   return make_float2(a.x - b.x, a.y - b.y);
 }
 // roots[m] = exp(-2 pi i m / n); sign > 0 conjugates
-__device__ __forceinline__ float2 rootf(float2 const *roots, int n, long e, int sign) {
-  float2 w = roots[(int)(e % n)];
+__device__ __forceinline__ float2 rootf(float2 const *roots, int n, int e, int sign) {
+  float2 w = roots[e % n];
   if (sign > 0) w.y = -w.y;
   return w;
 }
@@ -146,10 +146,13 @@ struct LinesND {
 
 // One Stockham pass (radix R, Ns = product of the radices already applied)
 // over nlines lines of length n; element j of line l at base(l) + j*stride.
-template <class L>
-__device__ void fft_pass(float2 const *__restrict__ src, float2 *__restrict__ dst,
-                         int n, int R, int Ns, int sign, float2 const *roots,
-                         int nlines, int stride, L const &lines) {
+// Radices with a register codelet (2, 3, 4, 5, 7, 8) are compile-time; each
+// butterfly loads R - 1 twiddles exp(sign 2 pi i r k / (Ns R)) from the
+// roots table of n (exponent r k n / (Ns R) < n, 32-bit).
+template <int R, int S, class L>
+__device__ void fft_pass_r(float2 const *__restrict__ src, float2 *__restrict__ dst, int n,
+                           int Ns, float2 const *__restrict__ roots, int nlines, int stride,
+                           L const &lines) {
   int const nr = n / R;
   int const total = nlines * nr;
   int const tw_step = n / (Ns * R);
@@ -161,61 +164,81 @@ __device__ void fft_pass(float2 const *__restrict__ src, float2 *__restrict__ ds
     int const ob = b + ((j / Ns) * Ns * R + k) * stride;
     float2 const *in = src + b + j * stride;
     int const istr = nr * stride;
-    if (R == 2) {
-      float2 v0 = in[0];
-      float2 v1 = in[istr];
-      if (k) v1 = cmul(v1, rootf(roots, n, (long)k * tw_step, sign));
-      dst[ob] = cadd(v0, v1);
-      dst[ob + Ns * stride] = csub(v0, v1);
-    } else if (R == 4) {
-      float2 v0 = in[0];
-      float2 v1 = in[istr];
-      float2 v2 = in[2 * istr];
-      float2 v3 = in[3 * istr];
-      if (k) {
-        v1 = cmul(v1, rootf(roots, n, (long)k * tw_step, sign));
-        v2 = cmul(v2, rootf(roots, n, (long)2 * k * tw_step, sign));
-        v3 = cmul(v3, rootf(roots, n, (long)3 * k * tw_step, sign));
-      }
-      float2 const s02 = cadd(v0, v2), d02 = csub(v0, v2);
-      float2 const s13 = cadd(v1, v3), d13 = csub(v1, v3);
-      // -i*d13 (sign -1) or +i*d13 (sign +1)
-      float2 const j13 = sign < 0 ? make_float2(d13.y, -d13.x) : make_float2(-d13.y, d13.x);
-      dst[ob] = cadd(s02, s13);
-      dst[ob + Ns * stride] = cadd(d02, j13);
-      dst[ob + 2 * Ns * stride] = csub(s02, s13);
-      dst[ob + 3 * Ns * stride] = csub(d02, j13);
-    } else if (R == 3) {
-      float2 v0 = in[0];
-      float2 v1 = in[istr];
-      float2 v2 = in[2 * istr];
-      if (k) {
-        v1 = cmul(v1, rootf(roots, n, (long)k * tw_step, sign));
-        v2 = cmul(v2, rootf(roots, n, (long)2 * k * tw_step, sign));
-      }
-      float2 const t1 = cadd(v1, v2);
-      float2 const t2 = make_float2(v0.x - 0.5f * t1.x, v0.y - 0.5f * t1.y);
-      float const s3 = (sign < 0 ? -1.f : 1.f) * 0.866025403784438647f;
-      float2 const dd = csub(v1, v2);
-      float2 const t3 = make_float2(-s3 * dd.y, s3 * dd.x);  // i*s3*(v1-v2)
-      dst[ob] = cadd(v0, t1);
-      dst[ob + Ns * stride] = cadd(t2, t3);
-      dst[ob + 2 * Ns * stride] = csub(t2, t3);
-    } else {
-      // direct butterfly for any other radix (5, 7, larger primes)
-      int const rstep = n / R;
-      for (int q = 0; q < R; ++q) {
-        float2 acc = make_float2(0.f, 0.f);
-        for (int r = 0; r < R; ++r) {
-          float2 v = in[r * istr];
-          long const e = (long)r * k * tw_step + (long)((q * r) % R) * rstep;
-          if (e % n) v = cmul(v, rootf(roots, n, e, sign));
-          acc = cadd(acc, v);
-        }
-        dst[ob + q * Ns * stride] = acc;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[r * istr];
+    if (k) {
+      int const e1 = k * tw_step;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = roots[r * e1];
+        if (S > 0) w.y = -w.y;
+        v[r] = agdfft::c_mul(v[r], w);
       }
     }
+    agdfft::butterfly<R, S>(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[ob + r * Ns * stride] = v[r];
   }
+}
+
+// any other radix: direct DFT of the R inputs with table roots
+template <int S, class L>
+__device__ void fft_pass_any(float2 const *__restrict__ src, float2 *__restrict__ dst, int n,
+                             int R, int Ns, float2 const *__restrict__ roots, int nlines,
+                             int stride, L const &lines) {
+  int const nr = n / R;
+  int const total = nlines * nr;
+  int const tw_step = n / (Ns * R);
+  int const rstep = n / R;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int const l = i / nr;
+    int const j = i - l * nr;
+    int const k = j % Ns;
+    int const b = lines.base(l);
+    int const ob = b + ((j / Ns) * Ns * R + k) * stride;
+    float2 const *in = src + b + j * stride;
+    int const istr = nr * stride;
+    for (int q = 0; q < R; ++q) {
+      float2 acc = make_float2(0.f, 0.f);
+      for (int r = 0; r < R; ++r) {
+        float2 v = in[r * istr];
+        int const e = (r * k * tw_step + ((q * r) % R) * rstep) % n;
+        if (e) {
+          float2 w = roots[e];
+          if (S > 0) w.y = -w.y;
+          v = agdfft::c_mul(v, w);
+        }
+        acc = agdfft::c_add(acc, v);
+      }
+      dst[ob + q * Ns * stride] = acc;
+    }
+  }
+}
+
+template <int S, class L>
+__device__ void fft_pass_s(float2 const *__restrict__ src, float2 *__restrict__ dst, int n,
+                           int R, int Ns, float2 const *roots, int nlines, int stride,
+                           L const &lines) {
+  switch (R) {
+    case 2: fft_pass_r<2, S>(src, dst, n, Ns, roots, nlines, stride, lines); break;
+    case 3: fft_pass_r<3, S>(src, dst, n, Ns, roots, nlines, stride, lines); break;
+    case 4: fft_pass_r<4, S>(src, dst, n, Ns, roots, nlines, stride, lines); break;
+    case 5: fft_pass_r<5, S>(src, dst, n, Ns, roots, nlines, stride, lines); break;
+    case 7: fft_pass_r<7, S>(src, dst, n, Ns, roots, nlines, stride, lines); break;
+    case 8: fft_pass_r<8, S>(src, dst, n, Ns, roots, nlines, stride, lines); break;
+    default: fft_pass_any<S>(src, dst, n, R, Ns, roots, nlines, stride, lines); break;
+  }
+}
+
+template <class L>
+__device__ void fft_pass(float2 const *__restrict__ src, float2 *__restrict__ dst,
+                         int n, int R, int Ns, int sign, float2 const *roots,
+                         int nlines, int stride, L const &lines) {
+  if (sign < 0)
+    fft_pass_s<-1>(src, dst, n, R, Ns, roots, nlines, stride, lines);
+  else
+    fft_pass_s<+1>(src, dst, n, R, Ns, roots, nlines, stride, lines);
 }
 
 // Full FFT along the lines; returns the buffer holding the result. Ends
