@@ -1125,7 +1125,7 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
   extern __shared__ float2 sm[];
   float2 *IN = sm;            // [Ft][TG] tile
   float2 *Z = IN + Ft * TG;   // [TG][LDL]
-  float2 *TW = Z + TG * LDL;  // exp(+2 pi i m / M)
+  float2 *TW = Z + TG * LDL;  // twiddles [M2][M1]
   float2 *RP = TW + M;        // exp(+2 pi i k / Pt), k <= M
   float *CA = (float *)(RP + Ft);  // [TG][half] overlap carry
   float *SC = CA + TG * half;      // [TG] per-trace scale
@@ -1137,7 +1137,11 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
   {
     float2 const *rM = roots + plans[w.tplan].roots_off;
     float2 const *rP = roots + plans[w.tplan_full].roots_off;
-    for (int m = tid; m < M; m += NT) TW[m] = make_float2(rM[m].x, -rM[m].y);
+    // TW[k2 M1 + n1] = exp(+2 pi i n1 k2 / M)
+    for (int m = tid; m < M; m += NT) {
+      float2 const r = rM[((m % M1) * (m / M1)) % M];
+      TW[m] = make_float2(r.x, -r.y);
+    }
     for (int k = tid; k < Ft; k += NT) RP[k] = make_float2(rP[k].x, -rP[k].y);
     for (int i = tid; i < TG * half; i += NT) CA[i] = 0.f;
     if (tid < TG) SC[tid] = tscale[w.trace0 + t0 + tid];
@@ -1187,9 +1191,10 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
       }
       agdfft::fft<M1, +1>(v);
       float2 *zz = Z + j * LDL + k2;
+      float2 const *tw = TW + k2 * M1;
 #pragma unroll
       for (int n1 = 0; n1 < M1; ++n1)
-        zz[n1 * M2] = (n1 == 0 || k2 == 0) ? v[n1] : agdfft::c_mul(v[n1], TW[(n1 * k2) % M]);
+        zz[n1 * M2] = (n1 == 0) ? v[n1] : agdfft::c_mul(v[n1], tw[n1]);
     }
     __syncthreads();
     for (int i = tid; i < TG * M1; i += NT) {
@@ -1250,7 +1255,7 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   static_assert(NT % half == 0 && TG % JS == 0, "sample mapping");
   extern __shared__ float2 sm[];
   float2 *Z = sm;             // [TG][LDL]
-  float2 *TW = Z + TG * LDL;  // exp(-2 pi i m / M)
+  float2 *TW = Z + TG * LDL;  // twiddles [M2][M1]
   float2 *RP = TW + M;        // exp(-2 pi i k / Pt), k <= M
   int64_t *erec = (int64_t *)(RP + Ft);
   int *ets = (int *)(erec + TG);
@@ -1263,7 +1268,8 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   {
     float2 const *rM = roots + plans[w.tplan].roots_off;
     float2 const *rP = roots + plans[w.tplan_full].roots_off;
-    for (int m = tid; m < M; m += NT) TW[m] = rM[m];
+    // TW[k2 M1 + n1] = exp(-2 pi i n1 k2 / M)
+    for (int m = tid; m < M; m += NT) TW[m] = rM[((m % M1) * (m / M1)) % M];
     for (int k = tid; k < Ft; k += NT) RP[k] = rP[k];
     if (tid < TG) {
       AgdEntry const e = went[w.trace0 + t0 + tid];
@@ -1319,9 +1325,10 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
 #pragma unroll
       for (int k1 = 0; k1 < M1; ++k1) v[k1] = zz[k1 * M2];
       agdfft::fft<M1, -1>(v);
+      float2 const *tw = TW + k2 * M1;
 #pragma unroll
       for (int n1 = 0; n1 < M1; ++n1)
-        zz[n1 * M2] = (n1 == 0 || k2 == 0) ? v[n1] : agdfft::c_mul(v[n1], TW[(n1 * k2) % M]);
+        zz[n1 * M2] = (n1 == 0) ? v[n1] : agdfft::c_mul(v[n1], tw[n1]);
     }
     __syncthreads();
     for (int i = tid; i < TG * M1; i += NT) {
@@ -1336,25 +1343,36 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
       for (int n2 = 0; n2 < M2; ++n2) zz[n2] = v[n2];
     }
     __syncthreads();
-    // half spectrum of the real window, x2 on the interior bins
+    // half spectrum of the real window, x2 on the interior bins; the pair
+    // (k, M - k) shares its two inputs
     float2 *vo = vt + (int64_t)tw * tw_step;
-    for (int i = tid; i < Ft * TG; i += NT) {
+    for (int i = tid; i < (M / 2 + 1) * TG; i += NT) {
       int const k = i / TG;
       int const j = i - k * TG;
-      int const kk = (k == M) ? 0 : k;
       int const kc = (k == 0) ? 0 : M - k;
-      float2 const zk = Z[j * LDL + fft2_pos<M1, M2>(kk)];
-      float2 zc = Z[j * LDL + fft2_pos<M1, M2>(kc)];
-      zc.y = -zc.y;
-      float2 const e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
-      float2 const o = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
-      float2 X = agdfft::c_add(e, agdfft::c_mul(o, RP[k]));
-      if (k == 0 || k == M) X.y = 0.f;
-      if (k >= 1 && k <= kmax2) {
-        X.x *= two;
-        X.y *= two;
+      float2 const za = Z[j * LDL + fft2_pos<M1, M2>(k)];
+      float2 const zb = Z[j * LDL + fft2_pos<M1, M2>(kc)];
+      // X[k] from (za, conj zb), X[M - k] from (zb, conj za)
+      float2 const e1 = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+      float2 const o1 = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
+      float2 X1 = agdfft::c_add(e1, agdfft::c_mul(o1, RP[k]));
+      float2 X2 = agdfft::c_add(make_float2(e1.x, -e1.y),
+                                agdfft::c_mul(make_float2(o1.x, -o1.y), RP[M - k]));
+      if (k == 0) {
+        // DC, and Nyquist (RP[M] = -1), both real
+        X1.y = 0.f;
+        X2 = make_float2(za.x - za.y, 0.f);
       }
-      vo[i] = X;
+      if (k >= 1 && k <= kmax2) {
+        X1.x *= two;
+        X1.y *= two;
+      }
+      if (M - k >= 1 && M - k <= kmax2) {
+        X2.x *= two;
+        X2.y *= two;
+      }
+      vo[k * TG + j] = X1;
+      if (k != M - k) vo[(M - k) * TG + j] = X2;
     }
     __syncthreads();
   }
