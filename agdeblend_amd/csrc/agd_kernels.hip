@@ -1257,6 +1257,8 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   float2 *Z = sm;             // [TG][LDL]
   float2 *TW = Z + TG * LDL;  // twiddles [M2][M1]
   float2 *RP = TW + M;        // exp(-2 pi i k / Pt), k <= M
+  // per trace: offset of its samples in src, index of its first sample
+  // and count (the record through the blend entries, or a Tin trace)
   int64_t *erec = (int64_t *)(RP + Ft);
   int *ets = (int *)(erec + TG);
   int *en = ets + TG;
@@ -1272,10 +1274,16 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
     for (int m = tid; m < M; m += NT) TW[m] = rM[((m % M1) * (m / M1)) % M];
     for (int k = tid; k < Ft; k += NT) RP[k] = rP[k];
     if (tid < TG) {
-      AgdEntry const e = went[w.trace0 + t0 + tid];
-      erec[tid] = e.rec;
-      ets[tid] = e.trace_start;
-      en[tid] = e.n;
+      if (Tin != nullptr) {
+        erec[tid] = w.t_off + (int64_t)(t0 + tid) * nconv;
+        ets[tid] = 0;
+        en[tid] = nconv;
+      } else {
+        AgdEntry const e = went[w.trace0 + t0 + tid];
+        erec[tid] = e.rec;
+        ets[tid] = e.trace_start;
+        en[tid] = e.n;
+      }
     }
   }
   __syncthreads();
@@ -1284,11 +1292,15 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   float const tap0 = plain ? 1.f : tapers[w.tt_off + s];
   float const tap1 = plain ? 1.f : tapers[w.tt_off + half + s];
   float const two = plain ? 1.f : 2.f;
-  float const *tin = Tin ? Tin + w.t_off + (int64_t)t0 * nconv : nullptr;
+  // one uniform source; each sample load is unconditional (index clamped
+  // into the trace's range) and zeroed outside it, so that the loads of a
+  // time window stay in flight together
+  float const *const src = Tin != nullptr ? Tin : rec;
   auto sample = [&](int j, int t) -> float {
-    if (tin != nullptr) return tin[(int64_t)j * nconv + t];
     int const i2 = t - ets[j];
-    return (i2 >= 0 && i2 < en[j]) ? rec[erec[j] + i2] : 0.f;
+    bool const ok = (unsigned)i2 < (unsigned)en[j];
+    float const x = src[erec[j] + (ok ? i2 : 0)];
+    return ok ? x : 0.f;
   };
   float scl[NJ], cur[NJ], nxt[NJ];
 #pragma unroll
