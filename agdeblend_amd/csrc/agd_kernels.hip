@@ -454,26 +454,43 @@ __global__ __launch_bounds__(NTHR) void k_blend(
     if (s >= rec_len) break;
     float acc = 0.f;
     float cnt = 0.f;
-    for (int q = e0; q < e1; ++q) {
-      int const id = tile_ent[q];
-      AgdEntry const e = ent[id];
-      int64_t const i = s - e.rec;
-      if (i < 0 || i >= e.n) continue;
-      if (mode == 4) {
-        acc += cst[id];
-        continue;
+    // four entries at a time: their loads are issued together (clamped
+    // indices, values outside an entry's range unused), then added in the
+    // entries' order, so the sums are those of the serial loop
+    for (int q = e0; q < e1; q += 4) {
+      float v[4], wg[4];
+      bool ok[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int const qq = min(q + u, e1 - 1);
+        int const id = tile_ent[qq];
+        AgdEntry const e = ent[id];
+        int64_t const i = s - e.rec;
+        ok[u] = (q + u < e1) && i >= 0 && i < e.n;
+        int const ii = ok[u] ? (int)i : 0;
+        wg[u] = 1.f;
+        if (mode == 4) {
+          v[u] = cst[id];
+        } else {
+          v[u] = src[e.src + e.trace_start + ii];
+          if (mode == 1) {
+            int const to_end = e.n - 1 - ii;
+            int const edge = ii < to_end ? ii : to_end;
+            wg[u] = (taper_len > 1 && edge >= 0 && edge < taper_len) ? taper[edge] : 1.f;
+          }
+        }
       }
-      float const v = src[e.src + e.trace_start + i];
-      if (mode == 1) {
-        int const to_end = e.n - 1 - (int)i;
-        int const edge = (int)i < to_end ? (int)i : to_end;
-        float const wgt = (taper_len > 1 && edge < taper_len) ? taper[edge] : 1.f;
-        acc += v * wgt;
-        cnt += wgt;
-      } else if (mode == 2) {
-        acc = v;
-      } else {
-        acc += v;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!ok[u]) continue;
+        if (mode == 1) {
+          acc += v[u] * wg[u];
+          cnt += wg[u];
+        } else if (mode == 2) {
+          acc = v[u];
+        } else {
+          acc += v[u];
+        }
       }
     }
     if (mode == 1 && cnt > 0.f) acc /= cnt;
@@ -940,19 +957,32 @@ __device__ __forceinline__ void planes_copy(float2 *sm, float2 *Utw, int k0, int
   constexpr int W0 = win_len<P0>();
   constexpr int W1 = win_len<P1>();
   constexpr int ug = TILED ? TG : W0 * W1;
+  constexpr int NB = 8;  // loads issued together per thread
   int const span = nk * ug;
-  for (int i = threadIdx.x; i < nk * W0 * W1; i += blockDim.x) {
-    int const g = i / span;
-    int const rem = i - g * span;
-    int const k = rem / ug;
-    int const tr = g * ug + (rem - k * ug);
-    int const row = tr / W1;
-    float2 *l = sm + (k * W0 + row) * LD + (tr - row * W1);
-    float2 *u = Utw + ((int64_t)g * Ft + k0) * ug + rem;
-    if (TO_LDS)
-      *l = *u;
-    else
-      *u = *l;
+  int const total = nk * W0 * W1;
+  for (int base = 0; base < total; base += NB * NTHR) {
+    float2 v[NB];
+    int lp[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int const i = min(base + (int)threadIdx.x + b * NTHR, total - 1);
+      int const g = i / span;
+      int const rem = i - g * span;
+      int const k = rem / ug;
+      int const tr = g * ug + (rem - k * ug);
+      int const row = tr / W1;
+      lp[b] = (k * W0 + row) * LD + (tr - row * W1);
+      float2 *u = Utw + ((int64_t)g * Ft + k0) * ug + rem;
+      if (TO_LDS)
+        v[b] = *u;
+      else if (base + (int)threadIdx.x + b * NTHR < total)
+        *u = sm[lp[b]];
+    }
+    if (TO_LDS) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (base + (int)threadIdx.x + b * NTHR < total) sm[lp[b]] = v[b];
+    }
   }
 }
 
