@@ -65,6 +65,24 @@ def load():
     fn("agdx_timing_get", ci, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
        ctypes.POINTER(ctypes.c_long))
     fn("agdx_bytes", ctypes.c_double, vp, ci)
+    i64 = ctypes.c_int64
+    fn("agdu_conv3", ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, ci, vp)
+    fn("agdu_flip_weights", ci, vp, vp, ci, ci, vp)
+    fn("agdu_wgrad_slabs", i64, ci, ci, ci, ci)
+    fn("agdu_conv3_wgrad", ci, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, vp)
+    fn("agdu_bias_grad", ci, vp, vp, vp, i64, ci, vp)
+    fn("agdu_maxpool", ci, vp, vp, vp, ci, ci, ci, ci, ci, vp)
+    fn("agdu_maxpool_bwd", ci, vp, vp, vp, ci, ci, ci, ci, ci, vp)
+    fn("agdu_upsample", ci, vp, vp, ci, ci, ci, ci, ci, ci, ci, vp)
+    fn("agdu_upsample_bwd", ci, vp, vp, ci, ci, ci, ci, ci, ci, ci, vp)
+    fn("agdu_copy_channels", ci, vp, vp, i64, ci, ci, ci, ci, ci, ci, vp, vp)
+    fn("agdu_pack_input", ci, vp, vp, i64, ci, cf, vp)
+    fn("agdu_unpack_output", ci, vp, vp, vp, i64, ci, cf, vp)
+    fn("agdu_l2_loss", ci, vp, vp, vp, vp, i64, ci, cf, vp)
+    fn("agdu_loss_parts", i64, i64)
+    fn("agdu_adam", ci, vp, vp, vp, vp, vp, ci, cf, cf, cf, cf, ci, vp)
+    fn("agdu_to_bf16", ci, vp, vp, i64, vp)
+    fn("agdu_from_bf16", ci, vp, vp, i64, vp)
     _lib = L
     return L
 
@@ -77,7 +95,12 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_warm_model", "agdx_get_output",
            "agdx_finish",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
-           "agdx_timing_get", "agdx_bytes"]
+           "agdx_timing_get", "agdx_bytes",
+           "agdu_conv3", "agdu_flip_weights", "agdu_wgrad_slabs", "agdu_conv3_wgrad",
+           "agdu_bias_grad", "agdu_maxpool", "agdu_maxpool_bwd", "agdu_upsample",
+           "agdu_upsample_bwd", "agdu_copy_channels", "agdu_pack_input",
+           "agdu_unpack_output", "agdu_l2_loss", "agdu_loss_parts", "agdu_adam",
+           "agdu_to_bf16", "agdu_from_bf16"]
 
 
 def _p(a):

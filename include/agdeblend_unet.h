@@ -1,0 +1,73 @@
+/*
+ * Kernels of the warm-start network of libagdeblend.so (MI355X): a 3-D
+ * U-Net over (space, space, time) patches of pseudo-deblended data, trained
+ * on the GPU by these kernels, whose prediction becomes the starting model
+ * of the inversion through agdx_warm_model / agdx_run_warm. No counterpart
+ * in the reference (which always starts from the thresholded first
+ * update); used by agdeblend_amd.warmstart.
+ *
+ * Tensors are device pointers. Activations: [n][d][h][w][c] bf16 with c a
+ * multiple of 16; weights: [27][co][ci] bf16 (tap = (kd * 3 + kh) * 3 + kw);
+ * masters, moments and gradients: fp32. stream: a hipStream_t or NULL.
+ * Return 0 on success, 1 on failure (message on stderr).
+ */
+#ifndef AGDEBLEND_UNET_H
+#define AGDEBLEND_UNET_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* y = conv3x3x3(x, wt) (+ bias if mode & 1), ReLU if mode & 2, times
+ * (gate > 0) if mode & 4 (gate has y's layout) */
+int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
+               int N, int D, int H, int W, int CI, int CO, int mode, void *stream);
+/* weights of the input gradient: wtt[tap][ci][co] = wt[26 - tap][co][ci] */
+int agdu_flip_weights(void const *wt, void *wtt, int CI, int CO, void *stream);
+/* number of voxel slabs of the weight gradient (part holds
+ * slabs * 27 * CI * CO floats) */
+int64_t agdu_wgrad_slabs(int N, int D, int H, int W);
+/* dw[tap][co][ci] = sum_v dy[v][co] x[v + tap][ci] */
+int agdu_conv3_wgrad(void const *x, void const *dy, float *part, float *dw, int N, int D,
+                     int H, int W, int CI, int CO, void *stream);
+/* db[co] = sum_v dy[v][co] (part: ceil(nvox / 4096) * CO floats) */
+int agdu_bias_grad(void const *dy, float *part, float *db, int64_t nvox, int CO,
+                   void *stream);
+/* 2x2x2 max pooling, arg: winning offset per output element */
+int agdu_maxpool(void const *x, void *y, void *arg, int N, int D, int H, int W, int C,
+                 void *stream);
+int agdu_maxpool_bwd(void const *dy, void const *arg, void *dx, int N, int D, int H, int W,
+                     int C, void *stream);
+/* nearest x2 upsampling into channels [c0, c0 + C) of a Cy-channel tensor */
+int agdu_upsample(void const *x, void *y, int N, int D, int H, int W, int C, int Cy, int c0,
+                  void *stream);
+int agdu_upsample_bwd(void const *dy, void *dx, int N, int D, int H, int W, int C, int Cy,
+                      int c0, void *stream);
+/* y[v][cy0 + c] = x[v][cx0 + c] (mode 1: added to y; mode 2: zero where
+ * gate[v][cy0 + c] <= 0) */
+int agdu_copy_channels(void const *x, void *y, int64_t nvox, int Cx, int cx0, int Cy,
+                       int cy0, int C, int mode, void const *gate, void *stream);
+/* fp32 field -> channel 0 of a C-channel bf16 tensor (times scale), and back */
+int agdu_pack_input(float const *src, void *x, int64_t nvox, int C, float scale,
+                    void *stream);
+int agdu_unpack_output(void const *y, float *dst, float const *base, int64_t nvox, int C,
+                       float scale, void *stream);
+/* L2 loss of channel 0 against t * tscale: per-block partial sums of the
+ * squared error in part (agdu_loss_parts(nvox) floats) and the gradient
+ * 2 (y - t) / nvox in dy */
+int agdu_l2_loss(void const *y, float const *t, void *dy, float *part, int64_t nvox, int C,
+                 float tscale, void *stream);
+int64_t agdu_loss_parts(int64_t nvox);
+/* Adam step number step (1-based) on fp32 masters; refreshes the bf16 copy */
+int agdu_adam(float *w, float *m, float *v, float const *g, void *wb, int len, float lr,
+              float b1, float b2, float eps, int step, void *stream);
+int agdu_to_bf16(float const *a, void *b, int64_t n, void *stream);
+int agdu_from_bf16(void const *b, float *a, int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -83,7 +83,7 @@ def test_tables_match_oracle(name, geo):
 
 def _header_functions():
     names = []
-    for h in ("agdeblend.h", "agdeblend_gpu.h"):
+    for h in ("agdeblend.h", "agdeblend_gpu.h", "agdeblend_unet.h"):
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         names += re.findall(r"\b(agd\w+|agdeblend_\w+|agdx_\w+)\s*\(", src)

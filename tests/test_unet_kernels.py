@@ -1,0 +1,195 @@
+"""The warm-start network's kernels (agdu_*) against an FP64 numpy
+reference on small tensors. BF16 storage: inputs are rounded to bf16 before
+the reference sees them; outputs are compared to within bf16 rounding of
+FP32-accumulated sums."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests import unet_ref as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _lib():
+    from agdeblend_amd import lib as L
+    return L.load()
+
+
+def _bf(a):
+    return torch.tensor(np.asarray(a, np.float32), device="cuda").to(torch.bfloat16)
+
+
+def _np(t):
+    return t.float().cpu().numpy().astype(np.float64)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _close(got, ref, rtol=2e-2):
+    scale = max(np.abs(ref).max(), 1e-30)
+    err = np.abs(got - ref).max() / scale
+    assert err <= rtol, err
+
+
+def test_mfma_operand_layout_via_1x1_conv():
+    """A convolution whose only non-zero tap is the centre is a plain
+    matrix product; checks the lane layout of the matrix-core operands."""
+    rng = np.random.default_rng(0)
+    x = R.bf16_round(rng.standard_normal((1, 1, 1, 64, 16)))
+    w = np.zeros((27, 16, 16))
+    w[13] = R.bf16_round(rng.standard_normal((16, 16)))
+    y = torch.empty((1, 1, 1, 64, 16), dtype=torch.bfloat16, device="cuda")
+    x_d = _bf(x)
+    w_d = _bf(w)
+    assert _lib().agdu_conv3(_p(x_d), _p(w_d), None, None, _p(y), 1, 1, 1, 64, 16, 16, 0,
+                             None) == 0
+    torch.cuda.synchronize()
+    ref = np.einsum("ndhwi,oi->ndhwo", x, w[13])
+    _close(_np(y), ref, 1e-2)
+
+
+@pytest.mark.parametrize("ci,co", [(16, 16), (16, 32), (32, 64), (96, 32), (48, 16)])
+def test_conv3_forward(ci, co):
+    rng = np.random.default_rng(ci * 7 + co)
+    shape = (2, 4, 6, 64, ci)
+    x = R.bf16_round(rng.standard_normal(shape))
+    w = R.bf16_round(rng.standard_normal((27, co, ci)) / np.sqrt(27 * ci))
+    b = rng.standard_normal(co).astype(np.float32)
+    y = torch.empty(shape[:-1] + (co,), dtype=torch.bfloat16, device="cuda")
+    bt = torch.tensor(b, device="cuda")
+    L = _lib()
+    x_d = _bf(x)
+    w_d = _bf(w)
+    assert L.agdu_conv3(_p(x_d), _p(w_d), _p(bt), None, _p(y), 2, 4, 6, 64, ci, co, 3,
+                        None) == 0
+    torch.cuda.synchronize()
+    _close(_np(y), R.conv3(x, w, b.astype(np.float64), relu=True))
+
+
+@pytest.mark.parametrize("ci,co", [(16, 32), (32, 64), (96, 32)])
+def test_conv3_input_gradient_with_relu_mask(ci, co):
+    rng = np.random.default_rng(ci + 3 * co)
+    shape = (1, 4, 4, 64, ci)
+    dy = R.bf16_round(rng.standard_normal(shape[:-1] + (co,)))
+    w = R.bf16_round(rng.standard_normal((27, co, ci)) / np.sqrt(27 * co))
+    gate = R.bf16_round(rng.standard_normal(shape))
+    L = _lib()
+    wtt = torch.empty((27, ci, co), dtype=torch.bfloat16, device="cuda")
+    w_d = _bf(w)
+    assert L.agdu_flip_weights(_p(w_d), _p(wtt), ci, co, None) == 0
+    dx = torch.empty(shape, dtype=torch.bfloat16, device="cuda")
+    dy_d = _bf(dy)
+    gate_d = _bf(gate)
+    assert L.agdu_conv3(_p(dy_d), _p(wtt), None, _p(gate_d), _p(dx), *shape[:4], co, ci,
+                        4, None) == 0
+    torch.cuda.synchronize()
+    ref = R.conv3_input_grad(dy, w) * (gate > 0)
+    _close(_np(dx), ref)
+
+
+@pytest.mark.parametrize("ci,co", [(16, 16), (32, 64), (96, 32)])
+def test_conv3_weight_and_bias_gradients(ci, co):
+    rng = np.random.default_rng(5 * ci + co)
+    shape = (2, 4, 4, 64, ci)
+    x = R.bf16_round(rng.standard_normal(shape))
+    dy = R.bf16_round(rng.standard_normal(shape[:-1] + (co,)))
+    L = _lib()
+    ns = L.agdu_wgrad_slabs(*shape[:4])
+    part = torch.empty(ns * 27 * ci * co, dtype=torch.float32, device="cuda")
+    dw = torch.empty((27, co, ci), dtype=torch.float32, device="cuda")
+    x_d = _bf(x)
+    dy_d = _bf(dy)
+    assert L.agdu_conv3_wgrad(_p(x_d), _p(dy_d), _p(part), _p(dw), *shape[:4], ci, co,
+                              None) == 0
+    nvox = int(np.prod(shape[:4]))
+    bpart = torch.empty(((nvox + 4095) // 4096) * co, dtype=torch.float32, device="cuda")
+    db = torch.empty(co, dtype=torch.float32, device="cuda")
+    dy_d = _bf(dy)
+    assert L.agdu_bias_grad(_p(dy_d), _p(bpart), _p(db), nvox, co, None) == 0
+    torch.cuda.synchronize()
+    _close(dw.cpu().numpy().astype(np.float64), R.conv3_weight_grad(x, dy), 1e-4)
+    _close(db.cpu().numpy().astype(np.float64), dy.sum(axis=(0, 1, 2, 3)), 1e-4)
+
+
+def test_pool_upsample_and_their_gradients():
+    rng = np.random.default_rng(11)
+    shape = (2, 4, 6, 8, 16)
+    # distinct values within every 2x2x2 cell (integers, exact in bf16), so
+    # the maximum of a cell is unique
+    base = rng.integers(-100, 100, (2, 2, 3, 4, 16, 1))
+    cell = rng.permuted(np.broadcast_to(np.arange(8), (2, 2, 3, 4, 16, 8)), axis=-1) + base
+    x = cell.reshape(2, 2, 3, 4, 16, 2, 2, 2).transpose(0, 1, 5, 2, 6, 3, 7, 4)
+    x = x.reshape(shape).astype(np.float64)
+    L = _lib()
+    y = torch.empty((2, 2, 3, 4, 16), dtype=torch.bfloat16, device="cuda")
+    arg = torch.empty((2, 2, 3, 4, 16), dtype=torch.uint8, device="cuda")
+    x_d = _bf(x)
+    assert L.agdu_maxpool(_p(x_d), _p(y), _p(arg), *shape, None) == 0
+    dy = R.bf16_round(rng.standard_normal((2, 2, 3, 4, 16)))
+    dx = torch.empty(shape, dtype=torch.bfloat16, device="cuda")
+    dy_d = _bf(dy)
+    assert L.agdu_maxpool_bwd(_p(dy_d), _p(arg), _p(dx), *shape, None) == 0
+    up = torch.empty((2, 8, 12, 16, 32), dtype=torch.bfloat16, device="cuda")
+    up.zero_()
+    x_d = _bf(x)
+    assert L.agdu_upsample(_p(x_d), _p(up), *shape, 32, 16, None) == 0
+    dup = R.bf16_round(rng.standard_normal((2, 8, 12, 16, 32)))
+    dxu = torch.empty(shape, dtype=torch.bfloat16, device="cuda")
+    dup_d = _bf(dup)
+    assert L.agdu_upsample_bwd(_p(dup_d), _p(dxu), *shape, 32, 16, None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(y), R.maxpool(x))
+    # gradient lands on the maximum of each 2x2x2 cell
+    xr = x.reshape(2, 2, 2, 3, 2, 4, 2, 16)
+    is_max = xr == R.maxpool(x)[:, :, None, :, None, :, None, :]
+    ref = (is_max * dy[:, :, None, :, None, :, None, :]).reshape(shape)
+    assert np.array_equal(_np(dx), ref)
+    uref = R.upsample(x)
+    assert np.array_equal(_np(up)[..., 16:], uref) and not _np(up)[..., :16].any()
+    sref = dup[..., 16:].reshape(2, 4, 2, 6, 2, 8, 2, 16).sum(axis=(2, 4, 6))
+    _close(_np(dxu), sref, 1e-2)
+
+
+def test_loss_and_adam():
+    rng = np.random.default_rng(12)
+    nvox, C = 5000, 16
+    y = R.bf16_round(rng.standard_normal((nvox, C)))
+    t = rng.standard_normal(nvox).astype(np.float32)
+    L = _lib()
+    npart = L.agdu_loss_parts(nvox)
+    part = torch.empty(npart, dtype=torch.float32, device="cuda")
+    dy = torch.empty((nvox, C), dtype=torch.bfloat16, device="cuda")
+    tt = torch.tensor(t, device="cuda")
+    y_d = _bf(y)
+    assert L.agdu_l2_loss(_p(y_d), _p(tt), _p(dy), _p(part), nvox, C, 0.5, None) == 0
+    torch.cuda.synchronize()
+    e = y[:, 0] - t.astype(np.float64) * 0.5
+    assert abs(float(part.sum()) - float(np.sum(e * e))) <= 1e-4 * float(np.sum(e * e))
+    _close(_np(dy)[:, 0], 2 * e / nvox, 1e-2)
+    assert not _np(dy)[:, 1:].any()
+    # Adam, two steps, against the closed form
+    n = 1000
+    w0 = rng.standard_normal(n).astype(np.float32)
+    g = rng.standard_normal(n).astype(np.float32)
+    w = torch.tensor(w0, device="cuda")
+    m = torch.zeros(n, device="cuda")
+    v = torch.zeros(n, device="cuda")
+    gt = torch.tensor(g, device="cuda")
+    wb = torch.empty(n, dtype=torch.bfloat16, device="cuda")
+    for step in (1, 2):
+        assert L.agdu_adam(_p(w), _p(m), _p(v), _p(gt), _p(wb), n, 1e-2, 0.9, 0.999, 1e-8,
+                           step, None) == 0
+    torch.cuda.synchronize()
+    wr, mr, vr = w0.astype(np.float64), 0.0, 0.0
+    for step in (1, 2):
+        mr = 0.9 * mr + 0.1 * g
+        vr = 0.999 * vr + 0.001 * g.astype(np.float64) ** 2
+        wr = wr - 1e-2 * (mr / (1 - 0.9 ** step)) / (np.sqrt(vr / (1 - 0.999 ** step)) + 1e-8)
+    assert np.abs(w.cpu().numpy() - wr).max() <= 1e-5
+    assert np.array_equal(wb.float().cpu().numpy(), R.bf16_round(w.cpu().numpy()).astype(np.float32))

@@ -1,0 +1,58 @@
+"""Pins the FP64 numpy reference of the warm-start network (the checker of
+the agdu_* kernels) against independent routines: scipy's N-d correlation
+for the convolution, the adjoint identities <A x, y> = <x, A^T y> for the
+input and weight gradients, pooling and upsampling, and finite differences
+of the loss."""
+import numpy as np
+import pytest
+from scipy import signal
+
+from tests import unet_ref as R
+
+
+def test_conv3_matches_scipy_correlate():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((1, 3, 4, 5, 2))
+    w = rng.standard_normal((27, 3, 2))
+    b = rng.standard_normal(3)
+    y = R.conv3(x, w, b)
+    k = w.reshape(3, 3, 3, 3, 2)  # [kd][kh][kw][co][ci]
+    for co in range(3):
+        ref = b[co] + sum(signal.correlate(x[0, ..., ci], k[..., co, ci], mode="same",
+                                           method="direct") for ci in range(2))
+        assert np.allclose(y[0, ..., co], ref, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(R.conv3(x, w, b, relu=True), np.maximum(y, 0))
+
+
+def test_gradients_are_adjoints():
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((2, 3, 4, 5, 3))
+    w = rng.standard_normal((27, 4, 3))
+    dy = rng.standard_normal((2, 3, 4, 5, 4))
+    lhs = np.sum(R.conv3(x, w) * dy)
+    assert np.isclose(lhs, np.sum(x * R.conv3_input_grad(dy, w)), rtol=1e-12)
+    assert np.isclose(lhs, np.sum(w * R.conv3_weight_grad(x, dy)), rtol=1e-12)
+
+
+def test_pool_upsample_and_loss():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((1, 4, 2, 6, 2))
+    p = R.maxpool(x)
+    assert p.shape == (1, 2, 1, 3, 2)
+    assert p[0, 1, 0, 2, 1] == x[0, 2:4, 0:2, 4:6, 1].max()
+    u = rng.standard_normal((1, 8, 4, 12, 2))
+    lhs = np.sum(R.upsample(x) * u)
+    pooled_sum = u.reshape(1, 4, 2, 2, 2, 6, 2, 2).sum(axis=(2, 4, 6))
+    assert np.isclose(lhs, np.sum(x * pooled_sum), rtol=1e-12)
+
+
+@pytest.mark.parametrize("v", [1.0, -3.5, 0.1, 65504.0, 1.00390625, 1.01171875])
+def test_bf16_round_to_nearest_even(v):
+    got = R.bf16_round(np.array([v]))[0]
+    # brute force: the nearest of all bf16 values with that exponent, ties even
+    u = np.array([v], np.float32).view(np.uint32)[0]
+    lo = np.array([u & 0xFFFF0000], np.uint32).view(np.float32)[0]
+    hi = np.array([(u & 0xFFFF0000) + 0x10000], np.uint32).view(np.float32)[0]
+    dl, dh = abs(float(v) - float(lo)), abs(float(hi) - float(v))
+    want = lo if dl < dh else hi if dh < dl else (lo if ((u >> 16) & 1) == 0 else hi)
+    assert got == float(want)
