@@ -1922,6 +1922,14 @@ EXPORT float agdx_last_lambda0(void) { return g_last_lambda0; }
 /* Warm-start model s R2C(pad(y_w)) of a data estimate y (patch layout)
  * into dev.X: the window traces of y (placed as set_output places them),
  * the plain time R2C (scaler, no tapers, no x2) and the spatial DFTs. */
+static int warm_model_flat(AgdProblem *p, float const *y_dev) {
+  if (agdg_window_traces(&p->dev, p->d_wsrc, p->d_wnt, y_dev, p->dev.T)) return 1;
+  if (agdg_adj_time(&p->dev, p->dev.T, 1)) return 1;
+  if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, 0)) return 1;
+  p->u_valid = 0;
+  return 0;
+}
+
 static int warm_model(AgdProblem *p, float const *const *y) {
   int a;
   for (a = 0; a < p->n_patches; a++) {
@@ -1932,11 +1940,17 @@ static int warm_model(AgdProblem *p, float const *const *y) {
     }
     if (agdg_h2d(p->d_data + p->data_off[a], y[a], n * 4)) return 1;
   }
-  if (agdg_window_traces(&p->dev, p->d_wsrc, p->d_wnt, p->d_data, p->dev.T)) return 1;
-  if (agdg_adj_time(&p->dev, p->dev.T, 1)) return 1;
-  if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, 0)) return 1;
-  p->u_valid = 0;
-  return 0;
+  return warm_model_flat(p, p->d_data);
+}
+
+EXPORT float const *agdx_data_dev(AgdProblem *p) { return p->host_only ? NULL : p->d_data; }
+
+EXPORT int64_t agdx_data_len(AgdProblem *p) { return p->data_len; }
+
+EXPORT int agdx_warm_model_dev(AgdProblem *p, float const *y_dev) {
+  if (p->host_only || y_dev == NULL) return 1;
+  if (warm_model_flat(p, y_dev)) return 1;
+  return agdg_sync();
 }
 
 EXPORT int agdx_warm_model(AgdProblem *p, float const *const *y) {
@@ -1946,7 +1960,7 @@ EXPORT int agdx_warm_model(AgdProblem *p, float const *const *y) {
 }
 
 static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float const *x0,
-                    float const *const *warm);
+                    float const *const *warm, float const *warm_dev);
 
 static int forward_record(AgdProblem *p, int resid);
 static int adjoint_update(AgdProblem *p, float alpha, float lamb, int mode);
@@ -2052,7 +2066,7 @@ static int run_loop(AgdProblem *p, float alpha, float const *sched, int n_its,
 
 EXPORT int agdx_run(AgdProblem *p, float f, int n_its, int print_freq,
                     float const *x0) {
-  return run_impl(p, f, n_its, print_freq, x0, NULL);
+  return run_impl(p, f, n_its, print_freq, x0, NULL, NULL);
 }
 
 EXPORT int agdx_run_warm(AgdProblem *p, float f, int n_its, int print_freq,
@@ -2061,11 +2075,20 @@ EXPORT int agdx_run_warm(AgdProblem *p, float f, int n_its, int print_freq,
     fprintf(stderr, "ERROR: warm-start data is NULL\n");
     return 1;
   }
-  return run_impl(p, f, n_its, print_freq, NULL, warm);
+  return run_impl(p, f, n_its, print_freq, NULL, warm, NULL);
+}
+
+EXPORT int agdx_run_warm_dev(AgdProblem *p, float f, int n_its, int print_freq,
+                             float const *warm_dev) {
+  if (warm_dev == NULL) {
+    fprintf(stderr, "ERROR: warm-start data is NULL\n");
+    return 1;
+  }
+  return run_impl(p, f, n_its, print_freq, NULL, NULL, warm_dev);
 }
 
 static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float const *x0,
-                    float const *const *warm) {
+                    float const *const *warm, float const *warm_dev) {
   float const decay = expf(logf(0.001f / f) / (float)n_its);
   int const first_linear_it =
       (int)((logf(decay) * ((float)n_its - 1.0f) + 1.0f) / logf(decay));
@@ -2120,6 +2143,8 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
   lamb *= f;
   if (warm != NULL) {
     if (warm_model(p, warm)) return 1;
+  } else if (warm_dev != NULL) {
+    if (warm_model_flat(p, warm_dev)) return 1;
   } else if (x0 == NULL) {
     if (agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
   }

@@ -58,7 +58,7 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // ---------------------------------------------------------------------------
 // Convolution (forward, or input gradient with flipped weights)
 // ---------------------------------------------------------------------------
-// grid: (ceil(W / 64), D * H, N); block: 256 = 4 waves, each 16 voxels along w.
+// grid: (ceil(W / 64), H, N * D); block: 256 = 4 waves, each 16 voxels along w.
 // mode bits: 1 = add bias, 2 = ReLU on the output, 4 = multiply the output
 // by (gate > 0) (the ReLU mask of the layer below, for input gradients).
 template <int CI, int CO>
@@ -70,9 +70,9 @@ __global__ __launch_bounds__(UT) void k_conv3(
   constexpr int NT = CO / 16;
   int const lane = threadIdx.x & 63;
   int const wave = threadIdx.x >> 6;
-  int const n = blockIdx.z;
-  int const d = blockIdx.y / H;
-  int const h = blockIdx.y - d * H;
+  int const n = blockIdx.z / D;
+  int const d = blockIdx.z - n * D;
+  int const h = blockIdx.y;
   int const w0 = blockIdx.x * 64 + wave * 16;
   int const i = lane & 15;   // A row: voxel; B / D column: channel in the tile
   int const kq = lane >> 4;  // k quarter: channels 4 kq .. 4 kq + 3
@@ -424,11 +424,11 @@ extern "C" {
 
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
-  if (N < 1 || D < 1 || H < 1 || W < 1 || (int64_t)D * H > 65535 || N > 65535) {
+  if (N < 1 || D < 1 || H < 1 || W < 1 || H > 65535 || (int64_t)N * D > 65535) {
     fprintf(stderr, "ERROR: convolution shape %d x %d x %d x %d unsupported\n", N, D, H, W);
     return 1;
   }
-  dim3 const grid((unsigned)((W + 63) / 64), (unsigned)(D * H), (unsigned)N);
+  dim3 const grid((unsigned)((W + 63) / 64), (unsigned)H, (unsigned)(N * D));
 #define CC(a, b)                                                                      \
   if (CI == a && CO == b)                                                             \
     return ulaunch("conv3", k_conv3<a, b>, grid, dim3(UT), stream, (uint16_t const *)x, \

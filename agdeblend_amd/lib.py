@@ -56,6 +56,10 @@ def load():
     fn("agdx_run", ci, vp, cf, ci, ci, vp)
     fn("agdx_run_warm", ci, vp, cf, ci, ci, vp)
     fn("agdx_warm_model", ci, vp, vp)
+    fn("agdx_data_dev", vp, vp)
+    fn("agdx_data_len", ctypes.c_int64, vp)
+    fn("agdx_warm_model_dev", ci, vp, vp)
+    fn("agdx_run_warm_dev", ci, vp, cf, ci, ci, vp)
     fn("agdx_get_output", ci, vp, vp)
     fn("agdx_finish", ci, vp)
     fn("agdx_get_model", ci, vp, vp)
@@ -92,7 +96,8 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
-           "agdx_warm_model", "agdx_get_output",
+           "agdx_warm_model", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
+           "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
            "agdx_timing_get", "agdx_bytes",
@@ -245,6 +250,22 @@ class Problem:
         if self.L.agdx_run_warm(self.h, float(initial_factor), int(n_its), int(print_freq),
                                 yt.ptr):
             raise RuntimeError("agdx_run_warm failed")
+
+    def data_dev(self):
+        """Device address of the patch data set by set_data (flat, patch
+        order), and its length in floats."""
+        return int(self.L.agdx_data_dev(self.h) or 0), int(self.L.agdx_data_len(self.h))
+
+    def warm_model_dev(self, y_ptr):
+        if self.L.agdx_warm_model_dev(self.h, ctypes.c_void_p(y_ptr)):
+            raise RuntimeError("agdx_warm_model_dev failed")
+
+    def run_warm_dev(self, initial_factor, n_its, y_ptr, print_freq=0):
+        """run_warm with the estimate already on the device (flat layout of
+        data_dev)."""
+        if self.L.agdx_run_warm_dev(self.h, float(initial_factor), int(n_its), int(print_freq),
+                                    ctypes.c_void_p(y_ptr)):
+            raise RuntimeError("agdx_run_warm_dev failed")
 
     def iterate(self, n, lamb):
         if self.L.agdx_iterate(self.h, int(n), float(lamb)):

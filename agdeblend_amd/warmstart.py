@@ -1,0 +1,288 @@
+"""Learned warm start: a 3-level 3-D U-Net that maps pseudo-deblended data
+(blended record cut back into traces at each shot time) to an estimate of
+the unblended data, used as the initial model of the inversion through
+agdx_run_warm_dev.
+
+Every operation of the network runs in the agdu_* HIP kernels of
+libagdeblend.so (BF16 activations and weights, FP32 accumulation on the
+matrix cores, FP32 master weights and Adam moments); PyTorch only provides
+device memory and the stream.
+
+Network (channels-last tensors [n][d][h][w][c], 3x3x3 convolutions):
+  e1a 16->16, e1b 16->16 | 2x2x2 max pool | e2a 16->32, e2b 32->32 | pool |
+  b1 32->64, b2 64->64 | upsample, concat e2b -> 96 | d2a 96->32, d2b 32->32 |
+  upsample, concat e1b -> 48 | d1a 48->16, d1b 16->16 | head 16->16 (linear)
+ReLU after every layer but the head. The input field sits in channel 0 of
+16; the head's channel 0 is the prediction of (unblended - input) * scale,
+so the estimate is input + head[0] / scale.
+
+The hook into the inversion is the reference's initial model
+(agdeblend.c:584-590): instead of the thresholded first update, the model
+starts from the FK coefficients of the estimate.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import lib as L
+
+LAYERS = (("e1a", 16, 16), ("e1b", 16, 16), ("e2a", 16, 32), ("e2b", 32, 32),
+          ("b1", 32, 64), ("b2", 64, 64), ("d2a", 96, 32), ("d2b", 32, 32),
+          ("d1a", 48, 16), ("d1b", 16, 16), ("head", 16, 16))
+CH = 16  # channels of the input and output tensors
+
+
+def _check(rc, what):
+    if rc:
+        raise RuntimeError("%s failed" % what)
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class UNet:
+    """Weights, optimizer state and activation buffers on one HIP device."""
+
+    def __init__(self, seed=0, device="cuda", head_scale=0.1):
+        if not torch.cuda.is_available():
+            raise RuntimeError("the warm-start network needs a HIP device")
+        self.L = L.load()
+        self.dev = torch.device(device)
+        self.offs = {}
+        n = 0
+        for name, ci, co in LAYERS:
+            self.offs[name] = (n, n + 27 * ci * co)
+            n += 27 * ci * co + co
+        self.n_params = n
+        rng = np.random.default_rng(seed)
+        w = np.zeros(n, np.float32)
+        for name, ci, co in LAYERS:
+            a, b = self.offs[name]
+            std = np.sqrt(2.0 / (27 * ci)) * (head_scale if name == "head" else 1.0)
+            w[a:b] = rng.standard_normal(b - a) * std
+        self.scale = 1.0
+        self._set_params(w)
+        self.m = torch.zeros(n, device=self.dev)
+        self.v = torch.zeros(n, device=self.dev)
+        self.g = torch.zeros(n, device=self.dev)
+        self.step_count = 0
+        self._bufs = {}
+
+    # -- parameters ---------------------------------------------------------
+    def _set_params(self, w):
+        self.w = torch.tensor(np.asarray(w, np.float32), device=self.dev)
+        self.wb = torch.empty(self.n_params, dtype=torch.bfloat16, device=self.dev)
+        self.wt = torch.empty(self.n_params, dtype=torch.bfloat16, device=self.dev)
+        _check(self.L.agdu_to_bf16(_vp(self.w), _vp(self.wb), self.n_params, self._s()),
+               "agdu_to_bf16")
+
+    def _s(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def _wptr(self, name, buf=None):
+        a, _ = self.offs[name]
+        buf = self.wb if buf is None else buf
+        return ctypes.c_void_p(buf.data_ptr() + 2 * a)
+
+    def _bptr(self, name, buf=None):
+        _, b = self.offs[name]
+        buf = self.w if buf is None else buf
+        return ctypes.c_void_p(buf.data_ptr() + 4 * b)
+
+    def params(self):
+        """{name: (w [27][co][ci], b [co])} as float32 numpy arrays."""
+        w = self.w.cpu().numpy()
+        out = {}
+        for name, ci, co in LAYERS:
+            a, b = self.offs[name]
+            out[name] = (w[a:b].reshape(27, co, ci).copy(), w[b:b + co].copy())
+        return out
+
+    def grads(self):
+        g = self.g.cpu().numpy()
+        out = {}
+        for name, ci, co in LAYERS:
+            a, b = self.offs[name]
+            out[name] = (g[a:b].reshape(27, co, ci).copy(), g[b:b + co].copy())
+        return out
+
+    def save(self, path):
+        np.savez(path, w=self.w.cpu().numpy(), scale=np.float32(self.scale),
+                 layers=np.array([("%s:%d:%d" % l) for l in LAYERS]))
+
+    def load(self, path):
+        z = np.load(path, allow_pickle=False)
+        layers = [str(s) for s in z["layers"]]
+        if layers != ["%s:%d:%d" % l for l in LAYERS] or z["w"].size != self.n_params:
+            raise ValueError("%s does not hold this network's weights" % path)
+        self._set_params(z["w"])
+        self.scale = float(z["scale"])
+        return self
+
+    # -- buffers --------------------------------------------------------------
+    def _buffers(self, shape, train):
+        key = (shape, train)
+        if key in self._bufs:
+            return self._bufs[key]
+        N, D, H, W = shape
+        if D % 4 or H % 4 or W % 4:
+            raise ValueError("patch sizes must be multiples of 4, got %s" % (shape,))
+        bf = torch.bfloat16
+
+        def t(lv, c, dtype=bf):
+            f = 2 ** lv
+            return torch.empty((N, D // f, H // f, W // f, c), dtype=dtype, device=self.dev)
+
+        B = dict(x=t(0, 16), e1a=t(0, 16), e1b=t(0, 16), p1=t(1, 16), a1=t(1, 16, torch.uint8),
+                 e2a=t(1, 32), e2b=t(1, 32), p2=t(2, 32), a2=t(2, 32, torch.uint8),
+                 b1=t(2, 64), b2=t(2, 64), c2=t(1, 96), d2a=t(1, 32), d2b=t(1, 32),
+                 c1=t(0, 48), d1a=t(0, 16), d1b=t(0, 16), head=t(0, 16))
+        if train:
+            B.update(g16a=t(0, 16), g16b=t(0, 16), g48=t(0, 48), gh32a=t(1, 32),
+                     gh32b=t(1, 32), gh96=t(1, 96), gh16=t(1, 16), gq64a=t(2, 64),
+                     gq64b=t(2, 64), gq32=t(2, 32))
+            nvox = N * D * H * W
+            # weight-gradient partials: the largest slabs x 27 x ci x co of
+            # the three levels (48x16 at full size, 96x32 and 64x64 below)
+            nw = 0
+            for lv, cico in ((0, 48 * 16), (1, 96 * 32), (2, 64 * 64)):
+                f = 2 ** lv
+                nsl = int(self.L.agdu_wgrad_slabs(N, D // f, H // f, W // f))
+                nw = max(nw, nsl * 27 * cico)
+            B["wpart"] = torch.empty(nw, dtype=torch.float32, device=self.dev)
+            B["bpart"] = torch.empty(((nvox + 4095) // 4096) * 64, dtype=torch.float32,
+                                     device=self.dev)
+            B["lpart"] = torch.empty(int(self.L.agdu_loss_parts(nvox)), dtype=torch.float32,
+                                     device=self.dev)
+        self._bufs[key] = B
+        return B
+
+    def release(self):
+        self._bufs.clear()
+
+    # -- forward ----------------------------------------------------------------
+    def _conv(self, name, x, y, dims, mode):
+        ci, co = x.shape[-1], y.shape[-1]
+        _check(self.L.agdu_conv3(_vp(x), self._wptr(name), self._bptr(name), None, _vp(y),
+                                 *dims, ci, co, mode, self._s()), "agdu_conv3 " + name)
+
+    def forward(self, field, B):
+        """field: float32 device tensor [N, D, H, W] (unscaled input data)."""
+        N, D, H, W = field.shape
+        d0, d1, d2 = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
+        s = self._s()
+        nvox = N * D * H * W
+        _check(self.L.agdu_pack_input(_vp(field), _vp(B["x"]), nvox, CH, self.scale, s),
+               "agdu_pack_input")
+        self._conv("e1a", B["x"], B["e1a"], d0, 3)
+        self._conv("e1b", B["e1a"], B["e1b"], d0, 3)
+        _check(self.L.agdu_maxpool(_vp(B["e1b"]), _vp(B["p1"]), _vp(B["a1"]), *d0, 16, s),
+               "agdu_maxpool")
+        self._conv("e2a", B["p1"], B["e2a"], d1, 3)
+        self._conv("e2b", B["e2a"], B["e2b"], d1, 3)
+        _check(self.L.agdu_maxpool(_vp(B["e2b"]), _vp(B["p2"]), _vp(B["a2"]), *d1, 32, s),
+               "agdu_maxpool")
+        self._conv("b1", B["p2"], B["b1"], d2, 3)
+        self._conv("b2", B["b1"], B["b2"], d2, 3)
+        _check(self.L.agdu_upsample(_vp(B["b2"]), _vp(B["c2"]), *d2, 64, 96, 0, s),
+               "agdu_upsample")
+        nv1 = nvox // 8
+        _check(self.L.agdu_copy_channels(_vp(B["e2b"]), _vp(B["c2"]), nv1, 32, 0, 96, 64, 32, 0,
+                                         None, s), "agdu_copy_channels")
+        self._conv("d2a", B["c2"], B["d2a"], d1, 3)
+        self._conv("d2b", B["d2a"], B["d2b"], d1, 3)
+        _check(self.L.agdu_upsample(_vp(B["d2b"]), _vp(B["c1"]), *d1, 32, 48, 0, s),
+               "agdu_upsample")
+        _check(self.L.agdu_copy_channels(_vp(B["e1b"]), _vp(B["c1"]), nvox, 16, 0, 48, 32, 16,
+                                         0, None, s), "agdu_copy_channels")
+        self._conv("d1a", B["c1"], B["d1a"], d0, 3)
+        self._conv("d1b", B["d1a"], B["d1b"], d0, 3)
+        self._conv("head", B["d1b"], B["head"], d0, 1)
+        return B["head"]
+
+    def predict(self, field, out=None):
+        """Estimate of the unblended data from the pseudo-deblended field
+        (float32 device tensor [N, D, H, W]): field + head[0] / scale."""
+        B = self._buffers(tuple(field.shape), False)
+        head = self.forward(field, B)
+        if out is None:
+            out = torch.empty_like(field)
+        nvox = field.numel()
+        _check(self.L.agdu_unpack_output(_vp(head), _vp(out), _vp(field), nvox, CH,
+                                         1.0 / self.scale, self._s()), "agdu_unpack_output")
+        return out
+
+    # -- backward ---------------------------------------------------------------
+    def _layer_bwd(self, name, x, dy, dims, dx=None, gate=None):
+        """weight and bias gradient of layer name (input x, pre-activation
+        gradient dy); the input gradient times (gate > 0) into dx"""
+        ci, co = x.shape[-1], dy.shape[-1]
+        a, b = self.offs[name]
+        s = self._s()
+        B = self._cur
+        _check(self.L.agdu_conv3_wgrad(_vp(x), _vp(dy), _vp(B["wpart"]),
+                                       ctypes.c_void_p(self.g.data_ptr() + 4 * a), *dims, ci,
+                                       co, s), "agdu_conv3_wgrad " + name)
+        nvox = dims[0] * dims[1] * dims[2] * dims[3]
+        _check(self.L.agdu_bias_grad(_vp(dy), _vp(B["bpart"]),
+                                     ctypes.c_void_p(self.g.data_ptr() + 4 * b), nvox, co, s),
+               "agdu_bias_grad " + name)
+        if dx is not None:
+            _check(self.L.agdu_conv3(_vp(dy), self._wptr(name, self.wt), None, _vp(gate),
+                                     _vp(dx), *dims, co, ci, 4, s), "agdu_conv3 bwd " + name)
+
+    def train_step(self, field, target, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        """One Adam step on a batch: field and target float32 device tensors
+        [N, D, H, W] (pseudo-deblended input, unblended truth). Returns a
+        device tensor holding the batch's summed squared error (of the
+        scaled residual); the mean loss is that over N*D*H*W."""
+        N, D, H, W = field.shape
+        d0, d1, d2 = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
+        B = self._cur = self._buffers(tuple(field.shape), True)
+        s = self._s()
+        nvox = N * D * H * W
+        head = self.forward(field, B)
+        # residual target (target - field), scaled: the loss kernel takes
+        # t * tscale, so pass (target - field) through a scratch field
+        resid = B.get("resid")
+        if resid is None:
+            resid = B["resid"] = torch.empty_like(field)
+        torch.sub(target, field, out=resid)
+        dh = B["g16a"]
+        _check(self.L.agdu_l2_loss(_vp(head), _vp(resid), _vp(dh), _vp(B["lpart"]), nvox, CH,
+                                   self.scale, s), "agdu_l2_loss")
+        # flipped weights of every layer for the input gradients
+        for name, ci, co in LAYERS:
+            _check(self.L.agdu_flip_weights(self._wptr(name), self._wptr(name, self.wt), ci, co,
+                                            s), "agdu_flip_weights")
+        g16a, g16b, g48 = B["g16a"], B["g16b"], B["g48"]
+        self._layer_bwd("head", B["d1b"], dh, d0, g16b, B["d1b"])
+        self._layer_bwd("d1b", B["d1a"], g16b, d0, g16a, B["d1a"])
+        self._layer_bwd("d1a", B["c1"], g16a, d0, g48, B["c1"])
+        _check(self.L.agdu_upsample_bwd(_vp(g48), _vp(B["gh32a"]), *d1, 32, 48, 0, s),
+               "agdu_upsample_bwd")
+        self._layer_bwd("d2b", B["d2a"], B["gh32a"], d1, B["gh32b"], B["d2a"])
+        self._layer_bwd("d2a", B["c2"], B["gh32b"], d1, B["gh96"], B["c2"])
+        _check(self.L.agdu_upsample_bwd(_vp(B["gh96"]), _vp(B["gq64a"]), *d2, 64, 96, 0, s),
+               "agdu_upsample_bwd")
+        self._layer_bwd("b2", B["b1"], B["gq64a"], d2, B["gq64b"], B["b1"])
+        self._layer_bwd("b1", B["p2"], B["gq64b"], d2, B["gq32"], B["p2"])
+        # e2b's output gradient: pooled path plus the skip connection
+        _check(self.L.agdu_maxpool_bwd(_vp(B["gq32"]), _vp(B["a2"]), _vp(B["gh32a"]), *d1, 32, s),
+               "agdu_maxpool_bwd")
+        _check(self.L.agdu_copy_channels(_vp(B["gh96"]), _vp(B["gh32a"]), nvox // 8, 96, 64, 32,
+                                         0, 32, 1, None, s), "agdu_copy_channels")
+        self._layer_bwd("e2b", B["e2a"], B["gh32a"], d1, B["gh32b"], B["e2a"])
+        self._layer_bwd("e2a", B["p1"], B["gh32b"], d1, B["gh16"], B["p1"])
+        _check(self.L.agdu_maxpool_bwd(_vp(B["gh16"]), _vp(B["a1"]), _vp(g16b), *d0, 16, s),
+               "agdu_maxpool_bwd")
+        _check(self.L.agdu_copy_channels(_vp(g48), _vp(g16b), nvox, 48, 32, 16, 0, 16, 1, None,
+                                         s), "agdu_copy_channels")
+        self._layer_bwd("e1b", B["e1a"], g16b, d0, g16a, B["e1a"])
+        self._layer_bwd("e1a", B["x"], g16a, d0)
+        self.step_count += 1
+        _check(self.L.agdu_adam(_vp(self.w), _vp(self.m), _vp(self.v), _vp(self.g), _vp(self.wb),
+                                self.n_params, lr, b1, b2, eps, self.step_count, s), "agdu_adam")
+        return B["lpart"]

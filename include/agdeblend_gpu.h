@@ -96,6 +96,17 @@ int agdx_warm_model(AgdProblem *p, float const *const *y);
  * first update; lambda0 and the schedule are those of the zero start. */
 int agdx_run_warm(AgdProblem *p, float initial_factor, int n_its, int print_freq,
                   float const *const *warm);
+/* Device copy of the patch data set by agdx_set_data: all patches
+ * concatenated in patch order, agdx_data_len floats (valid until
+ * agdx_warm_model, agdx_finish or agdx_get_output overwrite it). */
+float const *agdx_data_dev(AgdProblem *p);
+int64_t agdx_data_len(AgdProblem *p);
+/* agdx_warm_model and agdx_run_warm with the estimate already on the
+ * device, in the flat layout of agdx_data_dev; the caller has finished
+ * writing it (device or stream synchronised) before the call. */
+int agdx_warm_model_dev(AgdProblem *p, float const *y_dev);
+int agdx_run_warm_dev(AgdProblem *p, float initial_factor, int n_its, int print_freq,
+                      float const *warm_dev);
 /* Forward of the current model assembled into the device copy of the patch
  * data (set_output, model.c:1425-1515), without copying it to the host. */
 int agdx_finish(AgdProblem *p);

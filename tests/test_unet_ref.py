@@ -56,3 +56,37 @@ def test_bf16_round_to_nearest_even(v):
     dl, dh = abs(float(v) - float(lo)), abs(float(hi) - float(v))
     want = lo if dl < dh else hi if dh < dl else (lo if ((u >> 16) & 1) == 0 else hi)
     assert got == float(want)
+
+
+def _small_net(rng, scale=0.3):
+    P = {}
+    for name, ci, co in R.LAYERS:
+        P[name] = (rng.standard_normal((27, co, ci)) * scale / np.sqrt(27 * ci),
+                   rng.standard_normal(co) * 0.05)
+    return P
+
+
+def test_network_gradients_match_finite_differences():
+    rng = np.random.default_rng(4)
+    P = _small_net(rng)
+    x = np.zeros((1, 4, 4, 8, 16))
+    x[..., 0] = rng.standard_normal((1, 4, 4, 8))
+    t = rng.standard_normal((1, 4, 4, 8))
+    out, A = R.unet_forward(x, P)
+    loss, dout = R.unet_loss(out, t)
+    G = R.unet_backward(dout, P, A)
+    for name, _, _ in R.LAYERS:
+        w, b = P[name]
+        for which, arr in ((0, w), (1, b)):
+            flat = arr.reshape(-1)
+            for k in rng.choice(flat.size, 3, replace=False):
+                old = flat[k]
+                h = 1e-6
+                flat[k] = old + h
+                lp = R.unet_loss(R.unet_forward(x, P)[0], t)[0]
+                flat[k] = old - h
+                lm = R.unet_loss(R.unet_forward(x, P)[0], t)[0]
+                flat[k] = old
+                fd = (lp - lm) / (2 * h)
+                an = G[name][which].reshape(-1)[k]
+                assert abs(fd - an) <= 1e-5 * max(abs(fd), 1e-3), (name, which, fd, an)

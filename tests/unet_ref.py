@@ -71,3 +71,93 @@ def maxpool(x):
 
 def upsample(x):
     return x.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
+
+
+# The warm-start network: name, input channels, output channels. Encoder
+# e1a e1b | pool | e2a e2b | pool | b1 b2 | up, concat e2b | d2a d2b | up,
+# concat e1b | d1a d1b | head (linear). ReLU after every layer but the head.
+LAYERS = (("e1a", 16, 16), ("e1b", 16, 16), ("e2a", 16, 32), ("e2b", 32, 32),
+          ("b1", 32, 64), ("b2", 64, 64), ("d2a", 96, 32), ("d2b", 32, 32),
+          ("d1a", 48, 16), ("d1b", 16, 16), ("head", 16, 16))
+
+
+def unet_forward(x, P):
+    """x [n][d][h][w][16]; P[name] = (w [27][co][ci], b [co]). Returns the
+    output and the saved activations."""
+    A = {"x": x}
+
+    def cv(name, inp, relu=True):
+        w, b = P[name]
+        A[name] = conv3(inp, w, b, relu)
+        return A[name]
+
+    s1 = cv("e1b", cv("e1a", x))
+    p1 = A["p1"] = maxpool(s1)
+    s2 = cv("e2b", cv("e2a", p1))
+    p2 = A["p2"] = maxpool(s2)
+    bt = cv("b2", cv("b1", p2))
+    c2 = A["c2"] = np.concatenate([upsample(bt), s2], axis=-1)
+    u2 = cv("d2b", cv("d2a", c2))
+    c1 = A["c1"] = np.concatenate([upsample(u2), s1], axis=-1)
+    u1 = cv("d1b", cv("d1a", c1))
+    return cv("head", u1, relu=False), A
+
+
+def _pool_bwd(x, dy):
+    """gradient of maxpool at x (first maximum in (d, h, w) order wins)"""
+    n, D, H, W, c = x.shape
+    v = x.reshape(n, D // 2, 2, H // 2, 2, W // 2, 2, c).transpose(0, 1, 3, 5, 7, 2, 4, 6)
+    v = v.reshape(n, D // 2, H // 2, W // 2, c, 8)
+    k = np.argmax(v, axis=-1)
+    g = np.zeros(v.shape)
+    np.put_along_axis(g, k[..., None], dy[..., None], axis=-1)
+    g = g.reshape(n, D // 2, H // 2, W // 2, c, 2, 2, 2).transpose(0, 1, 5, 2, 6, 3, 7, 4)
+    return g.reshape(x.shape)
+
+
+def _up_bwd(dy):
+    n, D, H, W, c = dy.shape
+    return dy.reshape(n, D // 2, 2, H // 2, 2, W // 2, 2, c).sum(axis=(2, 4, 6))
+
+
+def unet_backward(dout, P, A):
+    """Gradients {name: (dw, db)} of sum(dout * output)."""
+    G = {}
+
+    def layer(name, inp, dy, gate):
+        """dy: gradient at the layer's pre-activation; returns the gradient
+        at its input times (gate > 0) when gate is given"""
+        w, _ = P[name]
+        G[name] = (conv3_weight_grad(A[inp] if isinstance(inp, str) else inp, dy),
+                   dy.sum(axis=(0, 1, 2, 3)))
+        if gate is None:
+            return None
+        return conv3_input_grad(dy, w) * (gate > 0)
+
+    du1 = layer("head", A["d1b"], dout, A["d1b"])
+    dd1a = layer("d1b", A["d1a"], du1, A["d1a"])
+    dc1 = layer("d1a", A["c1"], dd1a, A["c1"])
+    c_u2 = A["d2b"].shape[-1]
+    du2 = _up_bwd(dc1[..., :c_u2])
+    dd2a = layer("d2b", A["d2a"], du2, A["d2a"])
+    dc2 = layer("d2a", A["c2"], dd2a, A["c2"])
+    c_b = A["b2"].shape[-1]
+    db2 = _up_bwd(dc2[..., :c_b])
+    db1 = layer("b2", A["b1"], db2, A["b1"])
+    dp2 = layer("b1", A["p2"], db1, A["p2"])
+    ds2 = _pool_bwd(A["e2b"], dp2) + dc2[..., c_b:]
+    de2a = layer("e2b", A["e2a"], ds2, A["e2a"])
+    dp1 = layer("e2a", A["p1"], de2a, A["p1"])
+    ds1 = _pool_bwd(A["e1b"], dp1) + dc1[..., c_u2:]
+    de1a = layer("e1b", A["e1a"], ds1, A["e1a"])
+    layer("e1a", A["x"], de1a, None)
+    return G
+
+
+def unet_loss(out, t):
+    """mean over voxels of (out[..., 0] - t)^2, and its gradient"""
+    e = out[..., 0] - t
+    nvox = e.size
+    dout = np.zeros(out.shape)
+    dout[..., 0] = 2.0 * e / nvox
+    return float(np.sum(e * e)) / nvox, dout
