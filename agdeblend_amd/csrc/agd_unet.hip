@@ -142,89 +142,196 @@ __global__ void k_flip_weights(uint16_t const *__restrict__ wt, uint16_t *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Weight and bias gradients: dW[tap][co][ci] = sum_v dy[v][co] x[v + tap][ci]
+// Weight and bias gradients: dW[tap][co][ci] = sum_v dy[v][co] x[v + tap][ci],
+// db[co] = sum_v dy[v][co]
 // ---------------------------------------------------------------------------
-// grid: (27, nslab); block 256. Each block sums its voxel slab for one tap
-// into part[slab][tap][co][ci] (fp32); threads own (co, ci) pairs.
+// A GEMM over voxels (K) on the matrix cores: for one tap, A[co][v] = dy and
+// B[v][ci] = the shifted input. A block owns one 64-voxel w chunk of RB
+// consecutive (n, d, h) rows and a group of taps (NKH = 3: the nine taps of
+// one kd, grid.y = kd; NKH = 1: the three taps of one (kd, kh)). Per row it
+// stages dy transposed (dys[co][v]) and the input rows transposed once per
+// kw shift (xs[r][kw][ci][v]), so both operands are aligned 8-byte LDS
+// reads; waves split the (tap, co tile, ci tile) products and the four
+// 16-voxel chunks of the row. The block's partial sums (fixed-order
+// reduction over its waves) go to part[block][27][CO][CI] (taps of its
+// group only) and, for grid.y == 0, the bias partials to bpart[block][CO].
+template <int CI, int CO, int NKH>
+struct WgradShape {
+  static constexpr int NTAP = NKH * 3;
+  static constexpr int NTILE = NTAP * (CO / 16) * (CI / 16);
+  static constexpr int TG = NTILE <= 32 ? 1 : (NTILE <= 64 ? 2 : 4);  // waves over tiles
+  static constexpr int NCS = 4 / TG;                                    // chunk groups
+  static constexpr int NPW = (NTILE + TG - 1) / TG;                     // tiles per wave
+  static constexpr int LR = 68;  // LDS row: 64 voxels + 4 (spreads the banks)
+  static constexpr int XS = NKH * 3 * CI * LR;                          // bf16 elements
+  static constexpr int DS = CO * LR;
+  static constexpr int RED = (NCS > 1 ? TG * NPW * 256 : 0) > 256       // floats
+                                 ? (NCS > 1 ? TG * NPW * 256 : 0)
+                                 : 256;
+  static constexpr int LDS_BYTES =
+      (2 * (XS + DS) > 4 * RED ? 2 * (XS + DS) : 4 * RED);
+  static constexpr int NPART = 256 / CO;  // bias: voxel parts per channel
+  static constexpr int VPP = 64 / NPART;
+};
+
+template <int CI, int CO, int NKH>
 __global__ __launch_bounds__(UT) void k_conv3_wgrad(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ dy, float *__restrict__ part,
-    int N, int D, int H, int W, int CI, int CO, int slab) {
-  __shared__ float sx[64 * 96];
-  __shared__ float sd[64 * 64];
-  int const tap = blockIdx.x;
-  int const kd = tap / 9, kh = (tap / 3) % 3, kw = tap % 3;
-  int64_t const nvox = (int64_t)N * D * H * W;
-  int64_t const v0 = (int64_t)blockIdx.y * slab;
-  int64_t const v1 = v0 + slab < nvox ? v0 + slab : nvox;
-  int const npair = CI * CO;
-  float acc[24];
+    float *__restrict__ bpart, int D, int H, int W, int64_t rows, int RB) {
+  using S = WgradShape<CI, CO, NKH>;
+  static_assert(CI % 16 == 0 && (CO == 16 || CO == 32 || CO == 64), "channel counts");
+  constexpr int LR = S::LR;
+  __shared__ __align__(16) unsigned char lds[S::LDS_BYTES];
+  uint16_t *const xs = (uint16_t *)lds;         // [NKH][3][CI][LR]
+  uint16_t *const dys = xs + S::XS;             // [CO][LR]
+  int const tid = threadIdx.x;
+  int const lane = tid & 63, wave = tid >> 6;
+  int const i = lane & 15, kq = lane >> 4;
+  int const tg = wave % S::TG, cs = wave / S::TG;
+  int const nwc = (W + 63) / 64;
+  int const wc = blockIdx.x % nwc;
+  int64_t const rb = blockIdx.x / nwc;
+  int const w0 = wc * 64;
+  int const g = blockIdx.y;
+  int const kd = NKH == 3 ? g : g / 3;
+  int const kh0 = NKH == 3 ? 0 : g % 3;
+  bool const do_bias = bpart != nullptr && g == 0;
+  f32x4 acc[S::NPW];
 #pragma unroll
-  for (int q = 0; q < 24; ++q) acc[q] = 0.f;
-  for (int64_t vb = v0; vb < v1; vb += 64) {
-    int const nv = (int)(v1 - vb < 64 ? v1 - vb : 64);
+  for (int t = 0; t < S::NPW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  int64_t const r0 = rb * RB;
+  int64_t const r1 = r0 + RB < rows ? r0 + RB : rows;
+  for (int64_t row = r0; row < r1; ++row) {
+    int const h = (int)(row % H);
+    int64_t const nd = row / H;
+    int const d = (int)(nd % D);
+    int64_t const n = nd / D;
     __syncthreads();
-    for (int e = threadIdx.x; e < 64 * CI; e += UT) {
-      int const j = e / CI, ci = e - j * CI;
-      float val = 0.f;
-      if (j < nv) {
-        int64_t const v = vb + j;
-        int const w = (int)(v % W);
-        int64_t r = v / W;
-        int const h = (int)(r % H);
-        r /= H;
-        int const d = (int)(r % D);
-        int const n = (int)(r / D);
-        int const dd = d + kd - 1, hh = h + kh - 1, ww = w + kw - 1;
-        if (dd >= 0 && dd < D && hh >= 0 && hh < H && ww >= 0 && ww < W)
-          val = bf2f(x[((((int64_t)n * D + dd) * H + hh) * W + ww) * CI + ci]);
+    // dy row, transposed: item = (voxel pair, 8-channel group)
+    for (int e = tid; e < 32 * (CO / 8); e += UT) {
+      int const v2 = e & 31, c8 = e >> 5;
+      int const wv = w0 + 2 * v2;
+      uint4 p0 = make_uint4(0, 0, 0, 0), p1 = p0;
+      int64_t const base = (row * W + wv) * CO + c8 * 8;
+      if (wv < W) p0 = *(uint4 const *)(dy + base);
+      if (wv + 1 < W) p1 = *(uint4 const *)(dy + base + CO);
+      uint32_t const a0[4] = {p0.x, p0.y, p0.z, p0.w}, a1[4] = {p1.x, p1.y, p1.z, p1.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        uint32_t const lo = (a0[k] & 0xffffu) | (a1[k] << 16);
+        uint32_t const hi = (a0[k] >> 16) | (a1[k] & 0xffff0000u);
+        *(uint32_t *)(dys + (c8 * 8 + 2 * k) * LR + 2 * v2) = lo;
+        *(uint32_t *)(dys + (c8 * 8 + 2 * k + 1) * LR + 2 * v2) = hi;
       }
-      sx[j * CI + ci] = val;
     }
-    for (int e = threadIdx.x; e < 64 * CO; e += UT) {
-      int const j = e / CO, co = e - j * CO;
-      sd[j * CO + co] = j < nv ? bf2f(dy[(vb + j) * CO + co]) : 0.f;
+    // input rows (dd, hh) for the group's kh, once per kw shift
+    int const dd = d + kd - 1;
+    for (int e = tid; e < NKH * 3 * 32 * (CI / 8); e += UT) {
+      int const v2 = e & 31;
+      int const c8 = (e >> 5) % (CI / 8);
+      int const rk = (e >> 5) / (CI / 8);  // r * 3 + kw
+      int const r = rk / 3, kw = rk - r * 3;
+      int const hh = h + kh0 + r - 1;
+      int const wv = w0 + 2 * v2 + kw - 1;
+      bool const okrow = dd >= 0 && dd < D && hh >= 0 && hh < H;
+      uint4 p0 = make_uint4(0, 0, 0, 0), p1 = p0;
+      int64_t const base = (((n * D + dd) * H + hh) * (int64_t)W + wv) * CI + c8 * 8;
+      if (okrow && wv >= 0 && wv < W) p0 = *(uint4 const *)(x + base);
+      if (okrow && wv + 1 >= 0 && wv + 1 < W) p1 = *(uint4 const *)(x + base + CI);
+      uint32_t const a0[4] = {p0.x, p0.y, p0.z, p0.w}, a1[4] = {p1.x, p1.y, p1.z, p1.w};
+      uint16_t *const dst = xs + (rk * CI + c8 * 8) * LR + 2 * v2;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        *(uint32_t *)(dst + (2 * k) * LR) = (a0[k] & 0xffffu) | (a1[k] << 16);
+        *(uint32_t *)(dst + (2 * k + 1) * LR) = (a0[k] >> 16) | (a1[k] & 0xffff0000u);
+      }
     }
     __syncthreads();
+    if (do_bias) {
+      int const co = tid % CO, pt = tid / CO;
 #pragma unroll
-    for (int q = 0; q < 24; ++q) {
-      int const p = threadIdx.x + q * UT;
-      if (p < npair) {
-        int const co = p / CI, ci = p - co * CI;
-        float s = acc[q];
-        for (int j = 0; j < nv; ++j) s += sd[j * CO + co] * sx[j * CI + ci];
-        acc[q] = s;
+      for (int v = 0; v < S::VPP; v += 4) {
+        bf16x4 const q = *(bf16x4 const *)(dys + co * LR + pt * S::VPP + v);
+        bsum += (bf2f((uint16_t)q[0]) + bf2f((uint16_t)q[1])) +
+                (bf2f((uint16_t)q[2]) + bf2f((uint16_t)q[3]));
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < S::TG; ++cc) {
+      int const c = cs + cc * S::NCS;  // 16-voxel chunk
+#pragma unroll
+      for (int j = 0; j < S::NPW; ++j) {
+        int const t = tg + j * S::TG;
+        if (S::NTILE % S::TG != 0 && t >= S::NTILE) break;
+        int const ct = t % (CO / 16);
+        int const it = (t / (CO / 16)) % (CI / 16);
+        int const tap = t / ((CO / 16) * (CI / 16));  // r * 3 + kw
+        bf16x4 const a = *(bf16x4 const *)(dys + (ct * 16 + i) * LR + c * 16 + 4 * kq);
+        bf16x4 const b = *(bf16x4 const *)(xs + (tap * CI + it * 16 + i) * LR + c * 16 + 4 * kq);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
       }
     }
   }
-  float *out = part + ((int64_t)blockIdx.y * 27 + tap) * npair;
+  // bias: fixed-order sum over the voxel parts
+  float *const red = (float *)lds;
+  if (do_bias) {
+    __syncthreads();
+    red[tid] = bsum;
+    __syncthreads();
+    if (tid < CO) {
+      float b = 0.f;
+      for (int pt = 0; pt < S::NPART; ++pt) b += red[pt * CO + tid];
+      bpart[(int64_t)blockIdx.x * CO + tid] = b;
+    }
+  }
+  // fixed-order sum over the chunk groups, then one partial per block
+  for (int sgrp = 1; sgrp < S::NCS; ++sgrp) {
+    __syncthreads();
+    if (cs == sgrp) {
 #pragma unroll
-  for (int q = 0; q < 24; ++q) {
-    int const p = threadIdx.x + q * UT;
-    if (p < npair) out[p] = acc[q];
+      for (int j = 0; j < S::NPW; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[(tg * S::NPW + j) * 256 + r * 64 + lane] = acc[j][r];
+    }
+    __syncthreads();
+    if (cs == 0) {
+#pragma unroll
+      for (int j = 0; j < S::NPW; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[j][r] += red[(tg * S::NPW + j) * 256 + r * 64 + lane];
+    }
+  }
+  if (cs == 0) {
+    float *const out = part + (int64_t)blockIdx.x * 27 * CO * CI;
+#pragma unroll
+    for (int j = 0; j < S::NPW; ++j) {
+      int const t = tg + j * S::TG;
+      if (S::NTILE % S::TG != 0 && t >= S::NTILE) break;
+      int const ct = t % (CO / 16);
+      int const it = (t / (CO / 16)) % (CI / 16);
+      int const tl = t / ((CO / 16) * (CI / 16));
+      int const kh = kh0 + tl / 3, kw = tl % 3;
+      int const tap = (kd * 3 + kh) * 3 + kw;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        out[((int64_t)tap * CO + ct * 16 + 4 * kq + r) * CI + it * 16 + i] = acc[j][r];
+    }
   }
 }
 
-// bias partials: part[slab][co] = sum over the slab of dy[v][co]
-__global__ __launch_bounds__(UT) void k_bias_grad(uint16_t const *__restrict__ dy,
-                                                  float *__restrict__ part, int64_t nvox,
-                                                  int CO, int slab) {
-  int64_t const v0 = (int64_t)blockIdx.x * slab;
-  int64_t const v1 = v0 + slab < nvox ? v0 + slab : nvox;
-  for (int co = threadIdx.x; co < CO; co += blockDim.x) {
-    float s = 0.f;
-    for (int64_t v = v0; v < v1; ++v) s += bf2f(dy[v * CO + co]);
-    part[(int64_t)blockIdx.x * CO + co] = s;
-  }
-}
-
-// out[i] = sum_s part[s][i], s in order
+// out[i] = sum_s part[s][i] for s < nslab, in order; grid.y splits the
+// slabs into groups of `per` whose sums go to out[group][i] (a second pass
+// with grid.y = 1 then sums the groups in order)
 __global__ void k_reduce_parts(float const *__restrict__ part, float *__restrict__ out,
-                               int nslab, int len) {
+                               int nslab, int len, int per) {
   int const i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
+  int const s0 = blockIdx.y * per;
+  int const s1 = s0 + per < nslab ? s0 + per : nslab;
   float s = 0.f;
-  for (int k = 0; k < nslab; ++k) s += part[(int64_t)k * len + i];
-  out[i] = s;
+  for (int k = s0; k < s1; ++k) s += part[(int64_t)k * len + i];
+  out[(int64_t)blockIdx.y * len + i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -234,130 +341,137 @@ __global__ void k_reduce_parts(float const *__restrict__ part, float *__restrict
 // the winning offset (first maximum in (a, b, e) order)
 __global__ void k_maxpool(uint16_t const *__restrict__ x, uint16_t *__restrict__ y,
                           uint8_t *__restrict__ arg, int N, int D, int H, int W, int C) {
-  int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int const Do = D / 2, Ho = H / 2, Wo = W / 2;
   int64_t const total = (int64_t)N * Do * Ho * Wo * C;
-  if (idx >= total) return;
-  int const c = (int)(idx % C);
-  int64_t r = idx / C;
-  int const w = (int)(r % Wo);
-  r /= Wo;
-  int const h = (int)(r % Ho);
-  r /= Ho;
-  int const d = (int)(r % Do);
-  int const n = (int)(r / Do);
-  float best = -INFINITY;
-  int barg = 0;
-  for (int o = 0; o < 8; ++o) {
-    int const a = o >> 2, b = (o >> 1) & 1, e = o & 1;
-    float const v =
-        bf2f(x[((((int64_t)n * D + 2 * d + a) * H + 2 * h + b) * W + 2 * w + e) * C + c]);
-    if (v > best) {
-      best = v;
-      barg = o;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int const c = (int)(idx % C);
+    int64_t r = idx / C;
+    int const w = (int)(r % Wo);
+    r /= Wo;
+    int const h = (int)(r % Ho);
+    r /= Ho;
+    int const d = (int)(r % Do);
+    int const n = (int)(r / Do);
+    float best = -INFINITY;
+    int barg = 0;
+    for (int o = 0; o < 8; ++o) {
+      int const a = o >> 2, b = (o >> 1) & 1, e = o & 1;
+      float const v =
+          bf2f(x[((((int64_t)n * D + 2 * d + a) * H + 2 * h + b) * W + 2 * w + e) * C + c]);
+      if (v > best) {
+        best = v;
+        barg = o;
+      }
     }
+    y[idx] = f2bf(best);
+    arg[idx] = (uint8_t)barg;
   }
-  y[idx] = f2bf(best);
-  arg[idx] = (uint8_t)barg;
 }
 
 __global__ void k_maxpool_bwd(uint16_t const *__restrict__ dy, uint8_t const *__restrict__ arg,
                               uint16_t *__restrict__ dx, int N, int D, int H, int W, int C) {
   // one thread per input element
-  int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t const total = (int64_t)N * D * H * W * C;
-  if (idx >= total) return;
-  int const c = (int)(idx % C);
-  int64_t r = idx / C;
-  int const w = (int)(r % W);
-  r /= W;
-  int const h = (int)(r % H);
-  r /= H;
-  int const d = (int)(r % D);
-  int const n = (int)(r / D);
-  int const Do = D / 2, Ho = H / 2, Wo = W / 2;
-  int64_t const o =
-      ((((int64_t)n * Do + d / 2) * Ho + h / 2) * Wo + w / 2) * C + c;
-  int const mine = ((d & 1) << 2) | ((h & 1) << 1) | (w & 1);
-  dx[idx] = (arg[o] == mine) ? dy[o] : (uint16_t)0;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int const c = (int)(idx % C);
+    int64_t r = idx / C;
+    int const w = (int)(r % W);
+    r /= W;
+    int const h = (int)(r % H);
+    r /= H;
+    int const d = (int)(r % D);
+    int const n = (int)(r / D);
+    int const Do = D / 2, Ho = H / 2, Wo = W / 2;
+    int64_t const o =
+        ((((int64_t)n * Do + d / 2) * Ho + h / 2) * Wo + w / 2) * C + c;
+    int const mine = ((d & 1) << 2) | ((h & 1) << 1) | (w & 1);
+    dx[idx] = (arg[o] == mine) ? dy[o] : (uint16_t)0;
+  }
 }
 
 // nearest x2 upsampling of x [n][D][H][W][C] into the channels [c0, c0 + C)
 // of y [n][2D][2H][2W][Cy]
 __global__ void k_upsample(uint16_t const *__restrict__ x, uint16_t *__restrict__ y, int N,
                            int D, int H, int W, int C, int Cy, int c0) {
-  int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t const total = (int64_t)N * 8 * D * H * W * C;
-  if (idx >= total) return;
-  int const c = (int)(idx % C);
-  int64_t r = idx / C;
-  int const w = (int)(r % (2 * W));
-  r /= 2 * W;
-  int const h = (int)(r % (2 * H));
-  r /= 2 * H;
-  int const d = (int)(r % (2 * D));
-  int const n = (int)(r / (2 * D));
-  y[((((int64_t)n * 2 * D + d) * 2 * H + h) * 2 * W + w) * Cy + c0 + c] =
-      x[((((int64_t)n * D + d / 2) * H + h / 2) * W + w / 2) * C + c];
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int const c = (int)(idx % C);
+    int64_t r = idx / C;
+    int const w = (int)(r % (2 * W));
+    r /= 2 * W;
+    int const h = (int)(r % (2 * H));
+    r /= 2 * H;
+    int const d = (int)(r % (2 * D));
+    int const n = (int)(r / (2 * D));
+    y[((((int64_t)n * 2 * D + d) * 2 * H + h) * 2 * W + w) * Cy + c0 + c] =
+        x[((((int64_t)n * D + d / 2) * H + h / 2) * W + w / 2) * C + c];
+  }
 }
 
 // gradient of the upsampling: dx[v][c] = sum of the 8 children's
 // dy[..][c0 + c], in a fixed order; accumulated in fp32
 __global__ void k_upsample_bwd(uint16_t const *__restrict__ dy, uint16_t *__restrict__ dx,
                                int N, int D, int H, int W, int C, int Cy, int c0) {
-  int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t const total = (int64_t)N * D * H * W * C;
-  if (idx >= total) return;
-  int const c = (int)(idx % C);
-  int64_t r = idx / C;
-  int const w = (int)(r % W);
-  r /= W;
-  int const h = (int)(r % H);
-  r /= H;
-  int const d = (int)(r % D);
-  int const n = (int)(r / D);
-  float s = 0.f;
-  for (int o = 0; o < 8; ++o) {
-    int const a = o >> 2, b = (o >> 1) & 1, e = o & 1;
-    s += bf2f(dy[((((int64_t)n * 2 * D + 2 * d + a) * 2 * H + 2 * h + b) * 2 * W + 2 * w + e) *
-                     Cy + c0 + c]);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int const c = (int)(idx % C);
+    int64_t r = idx / C;
+    int const w = (int)(r % W);
+    r /= W;
+    int const h = (int)(r % H);
+    r /= H;
+    int const d = (int)(r % D);
+    int const n = (int)(r / D);
+    float s = 0.f;
+    for (int o = 0; o < 8; ++o) {
+      int const a = o >> 2, b = (o >> 1) & 1, e = o & 1;
+      s += bf2f(dy[((((int64_t)n * 2 * D + 2 * d + a) * 2 * H + 2 * h + b) * 2 * W + 2 * w + e) *
+                       Cy + c0 + c]);
+    }
+    dx[idx] = f2bf(s);
   }
-  dx[idx] = f2bf(s);
 }
 
 // channel slice copy: y[v][cy0 + c] = x[v][cx0 + c], c < C, v < nvox
 __global__ void k_copy_channels(uint16_t const *__restrict__ x, uint16_t *__restrict__ y,
                                 int64_t nvox, int Cx, int cx0, int Cy, int cy0, int C,
                                 int mode, uint16_t const *__restrict__ gate) {
-  int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nvox * C) return;
-  int64_t const v = idx / C;
-  int const c = (int)(idx - v * C);
-  uint16_t val = x[v * Cx + cx0 + c];
-  // mode 1: add into y (fp32 sum, rounded once); mode 2: ReLU mask by gate
-  if (mode == 1) val = f2bf(bf2f(val) + bf2f(y[v * Cy + cy0 + c]));
-  if (mode == 2 && !(bf2f(gate[v * Cy + cy0 + c]) > 0.f)) val = 0;
-  y[v * Cy + cy0 + c] = val;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nvox * C;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t const v = idx / C;
+    int const c = (int)(idx - v * C);
+    uint16_t val = x[v * Cx + cx0 + c];
+    // mode 1: add into y (fp32 sum, rounded once); mode 2: ReLU mask by gate
+    if (mode == 1) val = f2bf(bf2f(val) + bf2f(y[v * Cy + cy0 + c]));
+    if (mode == 2 && !(bf2f(gate[v * Cy + cy0 + c]) > 0.f)) val = 0;
+    y[v * Cy + cy0 + c] = val;
+  }
 }
 
 // fp32 field -> channel 0 of a bf16 tensor with C channels (others zero)
 __global__ void k_pack_input(float const *__restrict__ src, uint16_t *__restrict__ x,
                              int64_t nvox, int C, float scale) {
-  int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nvox * C) return;
-  int64_t const v = idx / C;
-  int const c = (int)(idx - v * C);
-  x[idx] = c == 0 ? f2bf(src[v] * scale) : (uint16_t)0;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nvox * C;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t const v = idx / C;
+    int const c = (int)(idx - v * C);
+    x[idx] = c == 0 ? f2bf(src[v] * scale) : (uint16_t)0;
+  }
 }
 
 // channel 0 of a bf16 tensor -> fp32 field times scale (plus base if given)
 __global__ void k_unpack_output(uint16_t const *__restrict__ y, float *__restrict__ dst,
                                 float const *__restrict__ base, int64_t nvox, int C,
                                 float scale) {
-  int64_t const v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvox) return;
-  float const o = bf2f(y[v * C]) * scale;
-  dst[v] = base ? base[v] + o : o;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    float const o = bf2f(y[v * C]) * scale;
+    dst[v] = base ? base[v] + o : o;
+  }
 }
 
 // L2 loss on channel 0: partial sums of (y - t)^2 per block, and
@@ -402,13 +516,15 @@ __global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__re
 }
 
 __global__ void k_to_bf16(float const *__restrict__ a, uint16_t *__restrict__ b, int64_t n) {
-  int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = f2bf(a[i]);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = f2bf(a[i]);
 }
 
 __global__ void k_from_bf16(uint16_t const *__restrict__ b, float *__restrict__ a, int64_t n) {
-  int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = bf2f(b[i]);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = bf2f(b[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -419,6 +535,11 @@ __global__ void k_from_bf16(uint16_t const *__restrict__ b, float *__restrict__ 
   X(64, 32) X(32, 96) X(16, 48) X(64, 96) X(16, 64)
 
 static inline int64_t blocks_for(int64_t n, int t) { return (n + t - 1) / t; }
+// grid of the grid-stride elementwise kernels
+static inline int64_t blocks_capped(int64_t n, int t) {
+  int64_t const b = blocks_for(n, t);
+  return b < (1 << 20) ? b : (1 << 20);
+}
 
 extern "C" {
 
@@ -446,34 +567,80 @@ int agdu_flip_weights(void const *wt, void *wtt, int CI, int CO, void *stream) {
                  dim3(UT), stream, (uint16_t const *)wt, (uint16_t *)wtt, CI, CO);
 }
 
-int64_t agdu_wgrad_slabs(int N, int D, int H, int W) {
-  int64_t const nvox = (int64_t)N * D * H * W;
-  return blocks_for(nvox, 4096);
+#define WGRAD_CASES(X) \
+  X(16, 16, 3) X(16, 32, 3) X(32, 32, 3) X(48, 16, 3) X(32, 16, 3) X(32, 64, 1) X(64, 64, 1) \
+  X(96, 32, 1) X(64, 32, 1) X(16, 64, 3)
+
+// launch shape of the weight gradient: tap groups (grid.y), rows per block
+// and blocks (grid.x); 0 when the channel pair has no kernel
+static int wgrad_plan(int N, int D, int H, int W, int CI, int CO, int *groups, int *rb,
+                      int64_t *nblk) {
+  int nkh = 0;
+#define WC(a, b, k) \
+  if (CI == a && CO == b) nkh = k;
+  WGRAD_CASES(WC)
+#undef WC
+  if (nkh == 0) return 0;
+  *groups = nkh == 3 ? 3 : 9;
+  int64_t const rows = (int64_t)N * D * H;
+  int64_t const nwc = (W + 63) / 64;
+  int64_t const target = 2048 / *groups;
+  int64_t r = (rows * nwc + target - 1) / target;
+  if (r < 1) r = 1;
+  *rb = (int)r;
+  *nblk = nwc * ((rows + r - 1) / r);
+  return 1;
 }
 
-int agdu_conv3_wgrad(void const *x, void const *dy, float *part, float *dw, int N, int D,
-                     int H, int W, int CI, int CO, void *stream) {
-  int64_t const nslab = agdu_wgrad_slabs(N, D, H, W);
-  if (CI > 96 || CO > 64 || CI * CO > 24 * UT) {
-    fprintf(stderr, "ERROR: weight gradient for %d -> %d channels unsupported\n", CI, CO);
+#define RED_PER 32
+
+int64_t agdu_wgrad_part_len(int N, int D, int H, int W, int CI, int CO) {
+  int groups, rb;
+  int64_t nblk;
+  if (!wgrad_plan(N, D, H, W, CI, CO, &groups, &rb, &nblk)) return -1;
+  int64_t const ng = (nblk + RED_PER - 1) / RED_PER;
+  return nblk * (27 * CI * CO + CO) + ng * (27 * CI * CO + CO);
+}
+
+int agdu_conv3_wgrad(void const *x, void const *dy, float *part, float *dw, float *db, int N,
+                     int D, int H, int W, int CI, int CO, void *stream) {
+  int groups, rb;
+  int64_t nblk;
+  if (N < 1 || D < 1 || H < 1 || W < 1 ||
+      !wgrad_plan(N, D, H, W, CI, CO, &groups, &rb, &nblk) || nblk > 0x7fffffff) {
+    fprintf(stderr, "ERROR: weight gradient for %d -> %d channels, %d x %d x %d x %d "
+                    "unsupported\n", CI, CO, N, D, H, W);
     return 1;
   }
-  if (ulaunch("conv3_wgrad", k_conv3_wgrad, dim3(27, (unsigned)nslab), dim3(UT), stream,
-              (uint16_t const *)x, (uint16_t const *)dy, part, N, D, H, W, CI, CO, 4096))
-    return 1;
   int const len = 27 * CI * CO;
-  return ulaunch("reduce_parts", k_reduce_parts, dim3((unsigned)blocks_for(len, UT)),
-                 dim3(UT), stream, (float const *)part, dw, (int)nslab, len);
-}
-
-int agdu_bias_grad(void const *dy, float *part, float *db, int64_t nvox, int CO,
-                   void *stream) {
-  int64_t const nslab = blocks_for(nvox, 4096);
-  if (ulaunch("bias_grad", k_bias_grad, dim3((unsigned)nslab), dim3(64), stream,
-              (uint16_t const *)dy, part, nvox, CO, 4096))
+  float *const bpart = part + nblk * len;
+  float *const gw = bpart + nblk * CO;
+  int64_t const ng = (nblk + RED_PER - 1) / RED_PER;
+  float *const gb = gw + ng * len;
+  int64_t const rows = (int64_t)N * D * H;
+  dim3 const grid((unsigned)nblk, (unsigned)groups);
+  int rc = 1;
+#define WC(a, b, k)                                                                        \
+  if (CI == a && CO == b)                                                                  \
+    rc = ulaunch("conv3_wgrad", k_conv3_wgrad<a, b, k>, grid, dim3(UT), stream,            \
+                 (uint16_t const *)x, (uint16_t const *)dy, part, db ? bpart : (float *)0, \
+                 D, H, W, rows, rb);
+  WGRAD_CASES(WC)
+#undef WC
+  if (rc) return 1;
+  // two-level fixed-order sums over the blocks
+  if (ulaunch("reduce_parts", k_reduce_parts, dim3((unsigned)blocks_for(len, UT), (unsigned)ng),
+              dim3(UT), stream, (float const *)part, gw, (int)nblk, len, RED_PER) ||
+      ulaunch("reduce_parts", k_reduce_parts, dim3((unsigned)blocks_for(len, UT), 1), dim3(UT),
+              stream, (float const *)gw, dw, (int)ng, len, (int)ng))
     return 1;
-  return ulaunch("reduce_parts", k_reduce_parts, dim3((unsigned)blocks_for(CO, UT)),
-                 dim3(UT), stream, (float const *)part, db, (int)nslab, CO);
+  if (db == NULL) return 0;
+  if (ulaunch("reduce_parts", k_reduce_parts, dim3((unsigned)blocks_for(CO, UT), (unsigned)ng),
+              dim3(UT), stream, (float const *)bpart, gb, (int)nblk, CO, RED_PER) ||
+      ulaunch("reduce_parts", k_reduce_parts, dim3((unsigned)blocks_for(CO, UT), 1), dim3(UT),
+              stream, (float const *)gb, db, (int)ng, CO, (int)ng))
+    return 1;
+  return 0;
 }
 
 int agdu_maxpool(void const *x, void *y, void *arg, int N, int D, int H, int W, int C,
@@ -483,7 +650,7 @@ int agdu_maxpool(void const *x, void *y, void *arg, int N, int D, int H, int W, 
     return 1;
   }
   int64_t const total = (int64_t)N * (D / 2) * (H / 2) * (W / 2) * C;
-  return ulaunch("maxpool", k_maxpool, dim3((unsigned)blocks_for(total, UT)), dim3(UT),
+  return ulaunch("maxpool", k_maxpool, dim3((unsigned)blocks_capped(total, UT)), dim3(UT),
                  stream, (uint16_t const *)x, (uint16_t *)y, (uint8_t *)arg, N, D, H, W, C);
 }
 
@@ -494,7 +661,7 @@ int agdu_maxpool_bwd(void const *dy, void const *arg, void *dx, int N, int D, in
     return 1;
   }
   int64_t const total = (int64_t)N * D * H * W * C;
-  return ulaunch("maxpool_bwd", k_maxpool_bwd, dim3((unsigned)blocks_for(total, UT)),
+  return ulaunch("maxpool_bwd", k_maxpool_bwd, dim3((unsigned)blocks_capped(total, UT)),
                  dim3(UT), stream, (uint16_t const *)dy, (uint8_t const *)arg,
                  (uint16_t *)dx, N, D, H, W, C);
 }
@@ -502,14 +669,14 @@ int agdu_maxpool_bwd(void const *dy, void const *arg, void *dx, int N, int D, in
 int agdu_upsample(void const *x, void *y, int N, int D, int H, int W, int C, int Cy, int c0,
                   void *stream) {
   int64_t const total = (int64_t)N * 8 * D * H * W * C;
-  return ulaunch("upsample", k_upsample, dim3((unsigned)blocks_for(total, UT)), dim3(UT),
+  return ulaunch("upsample", k_upsample, dim3((unsigned)blocks_capped(total, UT)), dim3(UT),
                  stream, (uint16_t const *)x, (uint16_t *)y, N, D, H, W, C, Cy, c0);
 }
 
 int agdu_upsample_bwd(void const *dy, void *dx, int N, int D, int H, int W, int C, int Cy,
                       int c0, void *stream) {
   int64_t const total = (int64_t)N * D * H * W * C;
-  return ulaunch("upsample_bwd", k_upsample_bwd, dim3((unsigned)blocks_for(total, UT)),
+  return ulaunch("upsample_bwd", k_upsample_bwd, dim3((unsigned)blocks_capped(total, UT)),
                  dim3(UT), stream, (uint16_t const *)dy, (uint16_t *)dx, N, D, H, W, C, Cy,
                  c0);
 }
@@ -517,20 +684,20 @@ int agdu_upsample_bwd(void const *dy, void *dx, int N, int D, int H, int W, int 
 int agdu_copy_channels(void const *x, void *y, int64_t nvox, int Cx, int cx0, int Cy,
                        int cy0, int C, int mode, void const *gate, void *stream) {
   return ulaunch("copy_channels", k_copy_channels,
-                 dim3((unsigned)blocks_for(nvox * C, UT)), dim3(UT), stream,
+                 dim3((unsigned)blocks_capped(nvox * C, UT)), dim3(UT), stream,
                  (uint16_t const *)x, (uint16_t *)y, nvox, Cx, cx0, Cy, cy0, C, mode,
                  (uint16_t const *)gate);
 }
 
 int agdu_pack_input(float const *src, void *x, int64_t nvox, int C, float scale,
                     void *stream) {
-  return ulaunch("pack_input", k_pack_input, dim3((unsigned)blocks_for(nvox * C, UT)),
+  return ulaunch("pack_input", k_pack_input, dim3((unsigned)blocks_capped(nvox * C, UT)),
                  dim3(UT), stream, src, (uint16_t *)x, nvox, C, scale);
 }
 
 int agdu_unpack_output(void const *y, float *dst, float const *base, int64_t nvox, int C,
                        float scale, void *stream) {
-  return ulaunch("unpack_output", k_unpack_output, dim3((unsigned)blocks_for(nvox, UT)),
+  return ulaunch("unpack_output", k_unpack_output, dim3((unsigned)blocks_capped(nvox, UT)),
                  dim3(UT), stream, (uint16_t const *)y, dst, base, nvox, C, scale);
 }
 
@@ -551,12 +718,12 @@ int agdu_adam(float *w, float *m, float *v, float const *g, void *wb, int len, f
 }
 
 int agdu_to_bf16(float const *a, void *b, int64_t n, void *stream) {
-  return ulaunch("to_bf16", k_to_bf16, dim3((unsigned)blocks_for(n, UT)), dim3(UT), stream,
+  return ulaunch("to_bf16", k_to_bf16, dim3((unsigned)blocks_capped(n, UT)), dim3(UT), stream,
                  a, (uint16_t *)b, n);
 }
 
 int agdu_from_bf16(void const *b, float *a, int64_t n, void *stream) {
-  return ulaunch("from_bf16", k_from_bf16, dim3((unsigned)blocks_for(n, UT)), dim3(UT),
+  return ulaunch("from_bf16", k_from_bf16, dim3((unsigned)blocks_capped(n, UT)), dim3(UT),
                  stream, (uint16_t const *)b, a, n);
 }
 

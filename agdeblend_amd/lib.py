@@ -6,6 +6,7 @@ fallback for the product path.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -25,6 +26,12 @@ def load():
     if not os.path.exists(LIBPATH):
         raise RuntimeError(
             "libagdeblend.so is not built (%s); run agdeblend_amd.build.build()" % LIBPATH)
+    # PyTorch-ROCm ships its own HIP runtime (libamdhip64.so) beside the one
+    # this library links (libamdhip64.so.7): both work in one process only
+    # when torch's initialises the device first
+    torch = sys.modules.get("torch")
+    if torch is not None:
+        torch.cuda.is_available()
     L = ctypes.CDLL(LIBPATH)
     vp, ci, cf = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
 
@@ -72,9 +79,8 @@ def load():
     i64 = ctypes.c_int64
     fn("agdu_conv3", ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, ci, vp)
     fn("agdu_flip_weights", ci, vp, vp, ci, ci, vp)
-    fn("agdu_wgrad_slabs", i64, ci, ci, ci, ci)
-    fn("agdu_conv3_wgrad", ci, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, vp)
-    fn("agdu_bias_grad", ci, vp, vp, vp, i64, ci, vp)
+    fn("agdu_wgrad_part_len", i64, ci, ci, ci, ci, ci, ci)
+    fn("agdu_conv3_wgrad", ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, vp)
     fn("agdu_maxpool", ci, vp, vp, vp, ci, ci, ci, ci, ci, vp)
     fn("agdu_maxpool_bwd", ci, vp, vp, vp, ci, ci, ci, ci, ci, vp)
     fn("agdu_upsample", ci, vp, vp, ci, ci, ci, ci, ci, ci, ci, vp)
@@ -101,8 +107,8 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_finish",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
            "agdx_timing_get", "agdx_bytes",
-           "agdu_conv3", "agdu_flip_weights", "agdu_wgrad_slabs", "agdu_conv3_wgrad",
-           "agdu_bias_grad", "agdu_maxpool", "agdu_maxpool_bwd", "agdu_upsample",
+           "agdu_conv3", "agdu_flip_weights", "agdu_wgrad_part_len", "agdu_conv3_wgrad",
+           "agdu_maxpool", "agdu_maxpool_bwd", "agdu_upsample",
            "agdu_upsample_bwd", "agdu_copy_channels", "agdu_pack_input",
            "agdu_unpack_output", "agdu_l2_loss", "agdu_loss_parts", "agdu_adam",
            "agdu_to_bf16", "agdu_from_bf16"]

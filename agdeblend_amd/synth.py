@@ -90,26 +90,73 @@ def config0_small(seed=1001):
     return p
 
 
-def _volume3d(rng, n_ch, n_src, n_t, dt, f, dx=25.0, n_lin=6, n_hyp=4):
-    xr = np.arange(n_ch)[:, None] * dx
-    xs = np.arange(n_src)[None, :] * dx
-    tmax = (n_t - 1) * dt
-    taus, amps = [], []
+def _events3d(rng, n_ch, n_src, tmax, dx=25.0, n_lin=6, n_hyp=4):
+    """Draw the events of a (ChannelX, SourceX) grid of n_ch x n_src points
+    dx apart: n_lin planes and n_hyp hyperbolas. Returns (tau(xr, xs), amp)
+    pairs, tau evaluated at receiver / source coordinates in metres."""
+    xs_mean = (n_src - 1) * dx / 2.0
+    xr_mean = (n_ch - 1) * dx / 2.0
+    xs_max = (n_src - 1) * dx
+    ev = []
     for _ in range(n_lin):
         t0 = rng.uniform(0.1, 0.9) * tmax
         ps = rng.uniform(-1.0, 1.0) / rng.uniform(1500.0, 4000.0)
         pr = rng.uniform(-1.0, 1.0) / rng.uniform(1500.0, 4000.0)
-        taus.append(t0 + ps * (xs - xs.mean()) + pr * (xr - xr.mean()))
-        amps.append(rng.uniform(0.2, 1.0))
+        ev.append((lambda xr, xs, t0=t0, ps=ps, pr=pr:
+                   t0 + ps * (xs - xs_mean) + pr * (xr - xr_mean), rng.uniform(0.2, 1.0)))
     for _ in range(n_hyp):
         t0 = rng.uniform(0.1, 0.8) * tmax
         v = rng.uniform(1500.0, 4000.0)
-        shift = rng.uniform(-0.5, 0.5) * xs.max()
-        taus.append(np.sqrt(t0 ** 2 + ((xs - xr - shift) / v) ** 2))
-        amps.append(rng.uniform(0.2, 1.0))
-    data = np.zeros((n_ch, n_src, n_t), np.float32)
-    _add_events(data, [np.broadcast_to(t, (n_ch, n_src)) for t in taus], amps, dt, f)
+        shift = rng.uniform(-0.5, 0.5) * xs_max
+        ev.append((lambda xr, xs, t0=t0, v=v, shift=shift:
+                   np.sqrt(t0 ** 2 + ((xs - xr - shift) / v) ** 2), rng.uniform(0.2, 1.0)))
+    return ev
+
+
+def _eval_events(ev, xr, xs, n_t, dt, f):
+    data = np.zeros((xr.shape[0], xs.shape[1], n_t), np.float32)
+    shape = (xr.shape[0], xs.shape[1])
+    _add_events(data, [np.broadcast_to(tau(xr, xs), shape) for tau, _ in ev],
+                [a for _, a in ev], dt, f)
     return data
+
+
+def _volume3d(rng, n_ch, n_src, n_t, dt, f, dx=25.0, n_lin=6, n_hyp=4):
+    ev = _events3d(rng, n_ch, n_src, (n_t - 1) * dt, dx, n_lin, n_hyp)
+    xr = np.arange(n_ch)[:, None] * dx
+    xs = np.arange(n_src)[None, :] * dx
+    return _eval_events(ev, xr, xs, n_t, dt, f)
+
+
+def training_truth(seed, n_ch=32, n_src=64, n_t=2048, full=(64, 1024)):
+    """Unblended half of a training pair of the learned warm start: an event
+    set drawn as for the full (ChannelX, SourceX) grid of the 64 x 1024 x
+    2048 workload (25 Hz Ricker, 4 ms), evaluated on a random n_ch x n_src
+    sub-block, and the shot times of its sources (every 2 s with +-0.5 s
+    dither). Returns (truth float32 [n_ch][n_src][n_t], shot times int64
+    [n_ch][n_src])."""
+    rng = np.random.default_rng(seed)
+    dt, f, dx = 0.004, 25.0, 25.0
+    ev = _events3d(rng, full[0], full[1], (n_t - 1) * dt, dx)
+    c0 = int(rng.integers(0, full[0] - n_ch + 1))
+    s0 = int(rng.integers(0, full[1] - n_src + 1))
+    xr = (c0 + np.arange(n_ch))[:, None] * dx
+    xs = (s0 + np.arange(n_src))[None, :] * dx
+    truth = _eval_events(ev, xr, xs, n_t, dt, f)
+    shot1 = _shot_times(rng, n_src, 2.0, 0.5, dt)
+    shot = np.broadcast_to(shot1[None, :], (n_ch, n_src)).astype(np.int64)
+    return truth, shot
+
+
+def training_blend(truth, shot, blend_fn):
+    """Pseudo-deblended input of a training pair: the truth blended into one
+    continuous record per channel and cut back into traces at the shot
+    times by blend_fn (agd_blend Sum)."""
+    n_ch, n_src = truth.shape[:2]
+    channels = np.broadcast_to(np.arange(n_ch, dtype=np.intc)[:, None], (n_ch, n_src)).copy()
+    tt = np.full((n_ch, n_src), LIVE, np.intc)
+    out = blend_fn([shot], [channels], [tt], [truth], 0, trace_types_out=[tt])
+    return np.asarray(out[0], np.float32).reshape(truth.shape)
 
 
 def config1(seed=1001, n_ch=64, n_src=512, n_t=2048, windows=(16, 32, 256), n_its=200,

@@ -21,11 +21,14 @@ The hook into the inversion is the reference's initial model
 starts from the FK coefficients of the estimate.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
 
 from . import lib as L
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_c4.npz")
 
 LAYERS = (("e1a", 16, 16), ("e1b", 16, 16), ("e2a", 16, 32), ("e2b", 32, 32),
           ("b1", 32, 64), ("b2", 64, 64), ("d2a", 96, 32), ("d2b", 32, 32),
@@ -47,7 +50,10 @@ class UNet:
 
     def __init__(self, seed=0, device="cuda", head_scale=0.1):
         if not torch.cuda.is_available():
-            raise RuntimeError("the warm-start network needs a HIP device")
+            raise RuntimeError(
+                "the warm-start network needs a HIP device visible to torch; in a process "
+                "that also calls libagdeblend, import torch before the first library call "
+                "(torch's HIP runtime must initialise the device first)")
         self.L = L.load()
         self.dev = torch.device(device)
         self.offs = {}
@@ -144,16 +150,13 @@ class UNet:
                      gh32b=t(1, 32), gh96=t(1, 96), gh16=t(1, 16), gq64a=t(2, 64),
                      gq64b=t(2, 64), gq32=t(2, 32))
             nvox = N * D * H * W
-            # weight-gradient partials: the largest slabs x 27 x ci x co of
-            # the three levels (48x16 at full size, 96x32 and 64x64 below)
+            # weight-gradient scratch: the largest need over the layers
             nw = 0
-            for lv, cico in ((0, 48 * 16), (1, 96 * 32), (2, 64 * 64)):
+            for lv, ci, co in ((0, 16, 16), (0, 48, 16), (1, 16, 32), (1, 32, 32), (1, 96, 32),
+                               (2, 32, 64), (2, 64, 64)):
                 f = 2 ** lv
-                nsl = int(self.L.agdu_wgrad_slabs(N, D // f, H // f, W // f))
-                nw = max(nw, nsl * 27 * cico)
+                nw = max(nw, int(self.L.agdu_wgrad_part_len(N, D // f, H // f, W // f, ci, co)))
             B["wpart"] = torch.empty(nw, dtype=torch.float32, device=self.dev)
-            B["bpart"] = torch.empty(((nvox + 4095) // 4096) * 64, dtype=torch.float32,
-                                     device=self.dev)
             B["lpart"] = torch.empty(int(self.L.agdu_loss_parts(nvox)), dtype=torch.float32,
                                      device=self.dev)
         self._bufs[key] = B
@@ -171,11 +174,15 @@ class UNet:
     def forward(self, field, B):
         """field: float32 device tensor [N, D, H, W] (unscaled input data)."""
         N, D, H, W = field.shape
+        _check(self.L.agdu_pack_input(_vp(field), _vp(B["x"]), field.numel(), CH, self.scale,
+                                      self._s()), "agdu_pack_input")
+        return self._forward_packed(B, (N, D, H, W))
+
+    def _forward_packed(self, B, shape):
+        N, D, H, W = shape
         d0, d1, d2 = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
         s = self._s()
         nvox = N * D * H * W
-        _check(self.L.agdu_pack_input(_vp(field), _vp(B["x"]), nvox, CH, self.scale, s),
-               "agdu_pack_input")
         self._conv("e1a", B["x"], B["e1a"], d0, 3)
         self._conv("e1b", B["e1a"], B["e1b"], d0, 3)
         _check(self.L.agdu_maxpool(_vp(B["e1b"]), _vp(B["p1"]), _vp(B["a1"]), *d0, 16, s),
@@ -202,6 +209,23 @@ class UNet:
         self._conv("head", B["d1b"], B["head"], d0, 1)
         return B["head"]
 
+    def predict_ptr(self, src_ptr, shape, out):
+        """predict() on a float32 device array given by address (e.g. the
+        patch data of a deblending problem, agdx_data_dev), written into the
+        device tensor out (same shape)."""
+        N, D, H, W = shape
+        if tuple(out.shape) != tuple(shape) or out.dtype != torch.float32:
+            raise ValueError("out must be float32 of shape %s" % (shape,))
+        B = self._buffers(tuple(shape), False)
+        nvox = N * D * H * W
+        s = self._s()
+        _check(self.L.agdu_pack_input(ctypes.c_void_p(src_ptr), _vp(B["x"]), nvox, CH,
+                                      self.scale, s), "agdu_pack_input")
+        head = self._forward_packed(B, shape)
+        _check(self.L.agdu_unpack_output(_vp(head), _vp(out), ctypes.c_void_p(src_ptr), nvox, CH,
+                                         1.0 / self.scale, s), "agdu_unpack_output")
+        return out
+
     def predict(self, field, out=None):
         """Estimate of the unblended data from the pseudo-deblended field
         (float32 device tensor [N, D, H, W]): field + head[0] / scale."""
@@ -223,12 +247,9 @@ class UNet:
         s = self._s()
         B = self._cur
         _check(self.L.agdu_conv3_wgrad(_vp(x), _vp(dy), _vp(B["wpart"]),
-                                       ctypes.c_void_p(self.g.data_ptr() + 4 * a), *dims, ci,
+                                       ctypes.c_void_p(self.g.data_ptr() + 4 * a),
+                                       ctypes.c_void_p(self.g.data_ptr() + 4 * b), *dims, ci,
                                        co, s), "agdu_conv3_wgrad " + name)
-        nvox = dims[0] * dims[1] * dims[2] * dims[3]
-        _check(self.L.agdu_bias_grad(_vp(dy), _vp(B["bpart"]),
-                                     ctypes.c_void_p(self.g.data_ptr() + 4 * b), nvox, co, s),
-               "agdu_bias_grad " + name)
         if dx is not None:
             _check(self.L.agdu_conv3(_vp(dy), self._wptr(name, self.wt), None, _vp(gate),
                                      _vp(dx), *dims, co, ci, 4, s), "agdu_conv3 bwd " + name)
@@ -286,3 +307,43 @@ class UNet:
         _check(self.L.agdu_adam(_vp(self.w), _vp(self.m), _vp(self.v), _vp(self.g), _vp(self.wb),
                                 self.n_params, lr, b1, b2, eps, self.step_count, s), "agdu_adam")
         return B["lpart"]
+
+
+def flops_per_voxel():
+    """Multiply-adds x 2 of one forward pass per full-resolution voxel."""
+    f = 0.0
+    for name, ci, co in LAYERS:
+        lv = 2 if name in ("b1", "b2") else (1 if name in ("e2a", "e2b", "d2a", "d2b") else 0)
+        f += 2.0 * 27 * ci * co / 8 ** lv
+    return f
+
+
+def estimate(pm, net, out=None):
+    """Network estimate of every patch of a deblending problem whose data is
+    set (lib.Problem), read from and written to device memory: returns a
+    float32 device tensor in the flat layout of pm.data_dev(). Patches must
+    be 3-D with every size a multiple of 4."""
+    ptr, n = pm.data_dev()
+    if not ptr:
+        raise RuntimeError("the problem has no device data")
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device=net.dev)
+    off = 0
+    for shape in pm.shapes:
+        shape = tuple(int(v) for v in shape)
+        size = int(np.prod(shape))
+        if len(shape) != 3:
+            raise ValueError("the warm-start network takes 3-D patches, got %s" % (shape,))
+        net.predict_ptr(ptr + 4 * off, (1,) + shape, out[off:off + size].view((1,) + shape))
+        off += size
+    return out
+
+
+def deblend_warm(pm, net, initial_factor, n_its, out=None):
+    """The inversion of pm started from the network's estimate: estimate on
+    the device, then agdx_run_warm_dev (the reference's loop from that
+    initial model). Returns the estimate tensor."""
+    est = estimate(pm, net, out)
+    torch.cuda.synchronize(net.dev)
+    pm.run_warm_dev(initial_factor, n_its, est.data_ptr())
+    return est

@@ -26,15 +26,14 @@ int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gat
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream);
 /* weights of the input gradient: wtt[tap][ci][co] = wt[26 - tap][co][ci] */
 int agdu_flip_weights(void const *wt, void *wtt, int CI, int CO, void *stream);
-/* number of voxel slabs of the weight gradient (part holds
- * slabs * 27 * CI * CO floats) */
-int64_t agdu_wgrad_slabs(int N, int D, int H, int W);
-/* dw[tap][co][ci] = sum_v dy[v][co] x[v + tap][ci] */
-int agdu_conv3_wgrad(void const *x, void const *dy, float *part, float *dw, int N, int D,
-                     int H, int W, int CI, int CO, void *stream);
-/* db[co] = sum_v dy[v][co] (part: ceil(nvox / 4096) * CO floats) */
-int agdu_bias_grad(void const *dy, float *part, float *db, int64_t nvox, int CO,
-                   void *stream);
+/* floats of scratch the weight gradient needs (-1: no kernel for the
+ * channel pair) */
+int64_t agdu_wgrad_part_len(int N, int D, int H, int W, int CI, int CO);
+/* dw[tap][co][ci] = sum_v dy[v][co] x[v + tap][ci] and, if db is not NULL,
+ * db[co] = sum_v dy[v][co]; fp32, summed in a fixed order (deterministic).
+ * part: agdu_wgrad_part_len floats */
+int agdu_conv3_wgrad(void const *x, void const *dy, float *part, float *dw, float *db, int N,
+                     int D, int H, int W, int CI, int CO, void *stream);
 /* 2x2x2 max pooling, arg: winning offset per output element */
 int agdu_maxpool(void const *x, void *y, void *arg, int N, int D, int H, int W, int C,
                  void *stream);
