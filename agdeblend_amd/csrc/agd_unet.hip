@@ -58,9 +58,40 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // ---------------------------------------------------------------------------
 // Convolution (forward, or input gradient with flipped weights)
 // ---------------------------------------------------------------------------
-// grid: (ceil(W / 64), H, N * D); block: 256 = 4 waves, each 16 voxels along w.
+// Block tile: DB consecutive d planes x CT_H rows (h) x CT_W voxels (w) of
+// one n, all output channels; one wave per row, 4 matrix-core tiles of 16
+// voxels each. Per 16-channel input chunk the block stages that chunk of the
+// weights ([27][CO][16]) once and walks the DB planes with a ring of three
+// input planes in LDS ((CT_H + 2) rows x (CT_W + 2) voxels x 16 channels
+// each; one new plane per step), so the input is read (DB + 2) / DB times
+// instead of three. Both MFMA operands are contiguous 512-byte LDS reads per
+// wave. Staging loads are branch-free (clamped address, zero selected
+// afterwards) so a thread's loads stay in flight together.
+// grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / DB)); block 256.
 // mode bits: 1 = add bias, 2 = ReLU on the output, 4 = multiply the output
 // by (gate > 0) (the ReLU mask of the layer below, for input gradients).
+#define CT_W 64
+#define CT_H 4
+
+// LDS rows of 16 bf16 channels (32 bytes) with their four 8-byte quarters
+// rotated by (row >> 2) & 3: the 16 lanes of a matrix-core operand read
+// (16 consecutive rows, one quarter) then fall in 16 distinct bank pairs
+// instead of 4-way conflicts
+__device__ __forceinline__ int swz_q(int row, int q) { return row * 16 + ((q ^ ((row >> 2) & 3)) << 2); }
+__device__ __forceinline__ bf16x4 ld_swz(uint16_t const *base, int row, int q) {
+  return *(bf16x4 const *)(base + swz_q(row, q));
+}
+// store 8 channels (quarters 2 half, 2 half + 1) of row
+__device__ __forceinline__ void st_swz(uint16_t *base, int row, int half, uint4 v) {
+  *(uint2 *)(base + swz_q(row, 2 * half)) = make_uint2(v.x, v.y);
+  *(uint2 *)(base + swz_q(row, 2 * half + 1)) = make_uint2(v.z, v.w);
+}
+
+template <int CO>
+struct ConvDepth {
+  static constexpr int DB = CO <= 16 ? 4 : (CO <= 32 ? 2 : 1);
+};
+
 template <int CI, int CO>
 __global__ __launch_bounds__(UT) void k_conv3(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ wt,
@@ -68,61 +99,202 @@ __global__ __launch_bounds__(UT) void k_conv3(
     uint16_t *__restrict__ y, int D, int H, int W, int mode) {
   static_assert(CI % 16 == 0 && CO % 16 == 0, "channels in multiples of 16");
   constexpr int NT = CO / 16;
-  int const lane = threadIdx.x & 63;
-  int const wave = threadIdx.x >> 6;
-  int const n = blockIdx.z / D;
-  int const d = blockIdx.z - n * D;
-  int const h = blockIdx.y;
-  int const w0 = blockIdx.x * 64 + wave * 16;
+  constexpr int DB = ConvDepth<CO>::DB;
+  constexpr int PW = CT_W + 2, PH = CT_H + 2;
+  constexpr int PLANE = PH * PW * 16;  // bf16 elements of one staged plane
+  constexpr int NPI = PH * PW * 2;     // staging items (8 channels) per plane
+  constexpr int NWI = 27 * CO * 2;
+  constexpr int OBS = CO + 4;  // output staging row: CO channels + 4 (bank spread)
+  constexpr int SMEM = 3 * PLANE + 27 * CO * 16 > 4 * 64 * OBS ? 3 * PLANE + 27 * CO * 16
+                                                                : 4 * 64 * OBS;
+  __shared__ __align__(16) uint16_t smem[SMEM];
+  uint16_t *const sx = smem;               // ring of 3 input planes
+  uint16_t *const sw = smem + 3 * PLANE;   // weights of the chunk
+  int const tid = threadIdx.x;
+  int const lane = tid & 63;
+  int const wave = tid >> 6;
+  int const ndb = (D + DB - 1) / DB;
+  int const n = blockIdx.z / ndb;
+  int const d0 = (blockIdx.z - n * ndb) * DB;
+  int const h0 = blockIdx.y * CT_H;
+  int const w0 = blockIdx.x * CT_W;
   int const i = lane & 15;   // A row: voxel; B / D column: channel in the tile
   int const kq = lane >> 4;  // k quarter: channels 4 kq .. 4 kq + 3
-  f32x4 acc[NT];
+  f32x4 acc[DB][4][NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  int64_t const plane = (int64_t)H * W;
-  uint16_t const *xn = x + (int64_t)n * D * plane * CI;
-  for (int kd = 0; kd < 3; ++kd) {
-    int const dd = d + kd - 1;
+  for (int j = 0; j < DB; ++j)
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[j][m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint16_t const *const xn = x + (int64_t)n * D * H * W * CI;
+  int64_t const plane = (int64_t)H * W * CI;
+  constexpr int NPK = (NPI + UT - 1) / UT;  // staging items per thread and plane
+  // per staging item: 32-bit offset within a plane (the same for every
+  // plane and chunk) and whether its (h, w) lies inside the volume; items
+  // past the last repeat it (same value, same place: no branch, so the
+  // compiler keeps the loads ahead of the stores)
+  uint32_t ioff[NPK];
+  bool iok[NPK];
+#pragma unroll
+  for (int k = 0; k < NPK; ++k) {
+    int const e0 = tid + k * UT;
+    int const e = e0 < NPI ? e0 : NPI - 1;
+    int const half = e & 1;
+    int const r = e >> 1;
+    int const pw = r % PW, ph = r / PW;
+    int const hh = h0 + ph - 1, ww = w0 + pw - 1;
+    iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
+    ioff[k] = iok[k] ? (uint32_t)((hh * W + ww) * CI + half * 8) : (uint32_t)(half * 8);
+  }
+  // global loads of plane dd of chunk c (zeros outside the volume) ...
+  auto load_plane = [&](int dd, int c, uint4 (&v)[NPK]) {
     bool const okd = dd >= 0 && dd < D;
-    for (int kh = 0; kh < 3; ++kh) {
-      int const hh = h + kh - 1;
-      bool const okh = okd && hh >= 0 && hh < H;
+    uint16_t const *const src = xn + (okd ? dd : 0) * plane + c * 16;
 #pragma unroll
-      for (int kw = 0; kw < 3; ++kw) {
-        int const ww = w0 + i + kw - 1;
-        bool const ok = okh && ww >= 0 && ww < W;
-        int const tap = (kd * 3 + kh) * 3 + kw;
-        int64_t const vox = ((int64_t)(ok ? dd : 0) * H + (ok ? hh : 0)) * W + (ok ? ww : 0);
-        uint16_t const *xa = xn + vox * CI + 4 * kq;
-        uint16_t const *wb = wt + (int64_t)tap * CO * CI + (int64_t)i * CI + 4 * kq;
+    for (int k = 0; k < NPK; ++k) {
+      uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;  // a mask, not a branch
+      uint4 const t = *(uint4 const *)(src + ioff[k]);
+      v[k] = make_uint4(t.x & msk, t.y & msk, t.z & msk, t.w & msk);
+    }
+  };
+  // ... and their store into ring slot (dd + 3) % 3
+  auto store_plane = [&](int dd, uint4 const (&v)[NPK]) {
+    uint16_t *const dst = sx + ((dd + 3) % 3) * PLANE;
 #pragma unroll
-        for (int c = 0; c < CI / 16; ++c) {
-          bf16x4 a = *(bf16x4 const *)(xa + 16 * c);
-          if (!ok) a = bf16x4{0, 0, 0, 0};
+    for (int k = 0; k < NPK; ++k) {
+      int const e0 = tid + k * UT;
+      int const e = e0 < NPI ? e0 : NPI - 1;
+      st_swz(dst, e >> 1, e & 1, v[k]);
+    }
+  };
+#pragma unroll 1
+  for (int c = 0; c < CI / 16; ++c) {
+    __syncthreads();
+    {
+      uint4 v0[NPK], v1[NPK], v2[NPK];
+      load_plane(d0 - 1, c, v0);
+      load_plane(d0, c, v1);
+      load_plane(d0 + 1, c, v2);
 #pragma unroll
-          for (int t = 0; t < NT; ++t) {
-            bf16x4 const b = *(bf16x4 const *)(wb + (int64_t)t * 16 * CI + 16 * c);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-          }
+      for (int k = 0; k < (NWI + UT - 1) / UT; ++k) {
+        int const e0 = tid + k * UT;
+        int const e = e0 < NWI ? e0 : NWI - 1;
+        int const half = e & 1;
+        int const tc = e >> 1;  // tap * CO + co
+        uint4 const v = *(uint4 const *)(wt + (int64_t)tc * CI + c * 16 + half * 8);
+        st_swz(sw, tc, half, v);  // past the end: the last item again
+      }
+      store_plane(d0 - 1, v0);
+      store_plane(d0, v1);
+      store_plane(d0 + 1, v2);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < DB; ++j) {
+      int const d = d0 + j;
+      // plane d + 2 is loaded while plane d is computed, then stored into
+      // the slot of plane d - 1
+      uint4 nxt[NPK];
+      if (j + 1 < DB) load_plane(d + 2, c, nxt);
+      // operands of one (kd, kh) row of taps: A for the 4 voxel tiles and
+      // 3 kw shifts, B for the 3 taps and NT channel tiles; double-buffered
+      // so the LDS reads of the next row overlap this row's MFMAs
+      struct Ops {
+        bf16x4 a[3][4];
+        bf16x4 b[3][NT];
+      };
+      auto load_ops = [&](int kdh, Ops &o) {
+        int const kd = kdh / 3, kh = kdh - kd * 3;
+        uint16_t const *const pl = sx + ((d + kd - 1 + 3) % 3) * PLANE;
+        int const v0 = (wave + kh) * PW + i;
+#pragma unroll
+        for (int kw = 0; kw < 3; ++kw) {
+#pragma unroll
+          for (int m = 0; m < 4; ++m) o.a[kw][m] = ld_swz(pl, v0 + 16 * m + kw, kq);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) o.b[kw][t] = ld_swz(sw, (kdh * 3 + kw) * CO + t * 16 + i, kq);
         }
+      };
+      auto mma_ops = [&](Ops const &o) {
+#pragma unroll
+        for (int kw = 0; kw < 3; ++kw)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+              acc[j][m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(o.a[kw][m], o.b[kw][t],
+                                                                     acc[j][m][t], 0, 0, 0);
+      };
+      Ops opa, opb;
+      load_ops(0, opa);
+#pragma unroll 1
+      for (int p = 0; p < 4; ++p) {
+        load_ops(2 * p + 1, opb);
+        mma_ops(opa);
+        load_ops(2 * p + 2, opa);
+        mma_ops(opb);
+      }
+      mma_ops(opa);
+      if (j + 1 < DB) {
+        __syncthreads();  // every wave is done with plane d - 1
+        store_plane(d + 2, nxt);
+        __syncthreads();
       }
     }
   }
-  // D: lane holds voxels 4 kq + r (r < 4) of channel t * 16 + i
-  int64_t const orow = (((int64_t)n * D + d) * H + h) * W;
+  // Epilogue per plane: each wave writes its row (bias, ReLU applied) into
+  // its own LDS rows [voxel][OBS] (lane (i, kq) holds voxels 4 kq + r of
+  // channel t * 16 + i; the row padding puts the four kq groups in distinct
+  // banks), then stores 16-byte pieces of 8 channels, coalesced.
+  int const h = h0 + wave;
+  uint16_t *const ob = smem + wave * 64 * OBS;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    int const co = t * 16 + i;
-    float const b = (mode & 1) ? bias[co] : 0.f;
+  for (int j = 0; j < DB; ++j) {
+    int const d = d0 + j;
+    __syncthreads();  // the LDS is free (j = 0: done with the operands)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int const wv = w0 + 4 * kq + r;
-      if (wv >= W) continue;
-      int64_t const o = (orow + wv) * CO + co;
-      float v = acc[t][r] + b;
-      if (mode & 2) v = fmaxf(v, 0.f);
-      if ((mode & 4) && !(bf2f(gate[o]) > 0.f)) v = 0.f;
-      y[o] = f2bf(v);
+    for (int t = 0; t < NT; ++t) {
+      int const co = t * 16 + i;
+      float const b = (mode & 1) ? bias[co] : 0.f;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float v = acc[j][m][t][r] + b;
+          if (mode & 2) v = fmaxf(v, 0.f);
+          ob[(16 * m + 4 * kq + r) * OBS + co] = f2bf(v);
+        }
+    }
+    __syncthreads();
+    if (h < H && d < D) {
+      int64_t const orow = (((int64_t)n * D + d) * H + h) * W;
+      constexpr int C8 = CO / 8;
+#pragma unroll
+      for (int k = 0; k < (64 * C8 + 63) / 64; ++k) {
+        int const e = lane + 64 * k;
+        int const v = e / C8, c8 = e - v * C8;
+        int const wv = w0 + v;
+        if (e < 64 * C8 && wv < W) {
+          uint2 const lo = *(uint2 const *)(ob + v * OBS + c8 * 8);
+          uint2 const hi = *(uint2 const *)(ob + v * OBS + c8 * 8 + 4);
+          uint4 val = make_uint4(lo.x, lo.y, hi.x, hi.y);
+          int64_t const o = (orow + wv) * CO + c8 * 8;
+          if (mode & 4) {
+            uint4 const g = *(uint4 const *)(gate + o);
+            uint32_t const gw[4] = {g.x, g.y, g.z, g.w};
+            uint32_t vw[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              uint32_t const m0 = bf2f((uint16_t)(gw[q] & 0xffffu)) > 0.f ? 0xffffu : 0u;
+              uint32_t const m1 = bf2f((uint16_t)(gw[q] >> 16)) > 0.f ? 0xffff0000u : 0u;
+              vw[q] &= m0 | m1;
+            }
+            val = make_uint4(vw[0], vw[1], vw[2], vw[3]);
+          }
+          *(uint4 *)(y + o) = val;
+        }
+      }
     }
   }
 }
@@ -337,46 +509,72 @@ __global__ void k_reduce_parts(float const *__restrict__ part, float *__restrict
 // ---------------------------------------------------------------------------
 // Pooling, upsampling, concatenation, loss, optimizer
 // ---------------------------------------------------------------------------
+// Elementwise kernels move 8 channels (16 bytes) per thread; channel counts
+// and offsets are multiples of 8 (checked on the host).
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+typedef uint8_t u8x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ u16x8 ld8(uint16_t const *p) { return *(u16x8 const *)p; }
+__device__ __forceinline__ void st8(uint16_t *p, u16x8 v) { *(u16x8 *)p = v; }
+
 // 2x2x2 max pool: y[n][d][h][w][c] over x[n][2d..][2h..][2w..][c]; arg holds
 // the winning offset (first maximum in (a, b, e) order)
 __global__ void k_maxpool(uint16_t const *__restrict__ x, uint16_t *__restrict__ y,
                           uint8_t *__restrict__ arg, int N, int D, int H, int W, int C) {
-  int const Do = D / 2, Ho = H / 2, Wo = W / 2;
-  int64_t const total = (int64_t)N * Do * Ho * Wo * C;
+  int const Do = D / 2, Ho = H / 2, Wo = W / 2, C8 = C / 8;
+  int64_t const total = (int64_t)N * Do * Ho * Wo * C8;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    int const c = (int)(idx % C);
-    int64_t r = idx / C;
+    int const c = (int)(idx % C8) * 8;
+    int64_t r = idx / C8;
     int const w = (int)(r % Wo);
     r /= Wo;
     int const h = (int)(r % Ho);
     r /= Ho;
     int const d = (int)(r % Do);
     int const n = (int)(r / Do);
-    float best = -INFINITY;
-    int barg = 0;
+    float best[8];
+    int barg[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      best[k] = -INFINITY;
+      barg[k] = 0;
+    }
+#pragma unroll
     for (int o = 0; o < 8; ++o) {
       int const a = o >> 2, b = (o >> 1) & 1, e = o & 1;
-      float const v =
-          bf2f(x[((((int64_t)n * D + 2 * d + a) * H + 2 * h + b) * W + 2 * w + e) * C + c]);
-      if (v > best) {
-        best = v;
-        barg = o;
+      u16x8 const v =
+          ld8(x + ((((int64_t)n * D + 2 * d + a) * H + 2 * h + b) * W + 2 * w + e) * C + c);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        float const f = bf2f(v[k]);
+        if (f > best[k]) {
+          best[k] = f;
+          barg[k] = o;
+        }
       }
     }
-    y[idx] = f2bf(best);
-    arg[idx] = (uint8_t)barg;
+    u16x8 out;
+    u8x8 ar;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      out[k] = f2bf(best[k]);
+      ar[k] = (uint8_t)barg[k];
+    }
+    st8(y + idx * 8, out);
+    *(u8x8 *)(arg + idx * 8) = ar;
   }
 }
 
+// one thread per 8 channels of an input voxel
 __global__ void k_maxpool_bwd(uint16_t const *__restrict__ dy, uint8_t const *__restrict__ arg,
                               uint16_t *__restrict__ dx, int N, int D, int H, int W, int C) {
-  // one thread per input element
-  int64_t const total = (int64_t)N * D * H * W * C;
+  int const C8 = C / 8;
+  int64_t const total = (int64_t)N * D * H * W * C8;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    int const c = (int)(idx % C);
-    int64_t r = idx / C;
+    int const c = (int)(idx % C8) * 8;
+    int64_t r = idx / C8;
     int const w = (int)(r % W);
     r /= W;
     int const h = (int)(r % H);
@@ -384,10 +582,14 @@ __global__ void k_maxpool_bwd(uint16_t const *__restrict__ dy, uint8_t const *__
     int const d = (int)(r % D);
     int const n = (int)(r / D);
     int const Do = D / 2, Ho = H / 2, Wo = W / 2;
-    int64_t const o =
-        ((((int64_t)n * Do + d / 2) * Ho + h / 2) * Wo + w / 2) * C + c;
+    int64_t const o = ((((int64_t)n * Do + d / 2) * Ho + h / 2) * Wo + w / 2) * C + c;
     int const mine = ((d & 1) << 2) | ((h & 1) << 1) | (w & 1);
-    dx[idx] = (arg[o] == mine) ? dy[o] : (uint16_t)0;
+    u16x8 const g = ld8(dy + o);
+    u8x8 const a = *(u8x8 const *)(arg + o);
+    u16x8 out;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = a[k] == mine ? g[k] : (uint16_t)0;
+    st8(dx + idx * 8, out);
   }
 }
 
@@ -395,19 +597,20 @@ __global__ void k_maxpool_bwd(uint16_t const *__restrict__ dy, uint8_t const *__
 // of y [n][2D][2H][2W][Cy]
 __global__ void k_upsample(uint16_t const *__restrict__ x, uint16_t *__restrict__ y, int N,
                            int D, int H, int W, int C, int Cy, int c0) {
-  int64_t const total = (int64_t)N * 8 * D * H * W * C;
+  int const C8 = C / 8;
+  int64_t const total = (int64_t)N * 8 * D * H * W * C8;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    int const c = (int)(idx % C);
-    int64_t r = idx / C;
+    int const c = (int)(idx % C8) * 8;
+    int64_t r = idx / C8;
     int const w = (int)(r % (2 * W));
     r /= 2 * W;
     int const h = (int)(r % (2 * H));
     r /= 2 * H;
     int const d = (int)(r % (2 * D));
     int const n = (int)(r / (2 * D));
-    y[((((int64_t)n * 2 * D + d) * 2 * H + h) * 2 * W + w) * Cy + c0 + c] =
-        x[((((int64_t)n * D + d / 2) * H + h / 2) * W + w / 2) * C + c];
+    st8(y + ((((int64_t)n * 2 * D + d) * 2 * H + h) * 2 * W + w) * Cy + c0 + c,
+        ld8(x + ((((int64_t)n * D + d / 2) * H + h / 2) * W + w / 2) * C + c));
   }
 }
 
@@ -415,51 +618,73 @@ __global__ void k_upsample(uint16_t const *__restrict__ x, uint16_t *__restrict_
 // dy[..][c0 + c], in a fixed order; accumulated in fp32
 __global__ void k_upsample_bwd(uint16_t const *__restrict__ dy, uint16_t *__restrict__ dx,
                                int N, int D, int H, int W, int C, int Cy, int c0) {
-  int64_t const total = (int64_t)N * D * H * W * C;
+  int const C8 = C / 8;
+  int64_t const total = (int64_t)N * D * H * W * C8;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    int const c = (int)(idx % C);
-    int64_t r = idx / C;
+    int const c = (int)(idx % C8) * 8;
+    int64_t r = idx / C8;
     int const w = (int)(r % W);
     r /= W;
     int const h = (int)(r % H);
     r /= H;
     int const d = (int)(r % D);
     int const n = (int)(r / D);
-    float s = 0.f;
+    float s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = 0.f;
+#pragma unroll
     for (int o = 0; o < 8; ++o) {
       int const a = o >> 2, b = (o >> 1) & 1, e = o & 1;
-      s += bf2f(dy[((((int64_t)n * 2 * D + 2 * d + a) * 2 * H + 2 * h + b) * 2 * W + 2 * w + e) *
-                       Cy + c0 + c]);
+      u16x8 const g = ld8(dy + ((((int64_t)n * 2 * D + 2 * d + a) * 2 * H + 2 * h + b) * 2 * W +
+                                2 * w + e) * Cy + c0 + c);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] += bf2f(g[k]);
     }
-    dx[idx] = f2bf(s);
+    u16x8 out;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = f2bf(s[k]);
+    st8(dx + idx * 8, out);
   }
 }
 
 // channel slice copy: y[v][cy0 + c] = x[v][cx0 + c], c < C, v < nvox
+// (mode 1: added to y, fp32 sum rounded once; mode 2: zero where
+// gate[v][cy0 + c] <= 0)
 __global__ void k_copy_channels(uint16_t const *__restrict__ x, uint16_t *__restrict__ y,
                                 int64_t nvox, int Cx, int cx0, int Cy, int cy0, int C,
                                 int mode, uint16_t const *__restrict__ gate) {
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nvox * C;
+  int const C8 = C / 8;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nvox * C8;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    int64_t const v = idx / C;
-    int const c = (int)(idx - v * C);
-    uint16_t val = x[v * Cx + cx0 + c];
-    // mode 1: add into y (fp32 sum, rounded once); mode 2: ReLU mask by gate
-    if (mode == 1) val = f2bf(bf2f(val) + bf2f(y[v * Cy + cy0 + c]));
-    if (mode == 2 && !(bf2f(gate[v * Cy + cy0 + c]) > 0.f)) val = 0;
-    y[v * Cy + cy0 + c] = val;
+    int64_t const v = idx / C8;
+    int const c = (int)(idx - v * C8) * 8;
+    u16x8 val = ld8(x + v * Cx + cx0 + c);
+    if (mode == 1) {
+      u16x8 const o = ld8(y + v * Cy + cy0 + c);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) val[k] = f2bf(bf2f(val[k]) + bf2f(o[k]));
+    } else if (mode == 2) {
+      u16x8 const g = ld8(gate + v * Cy + cy0 + c);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (!(bf2f(g[k]) > 0.f)) val[k] = 0;
+    }
+    st8(y + v * Cy + cy0 + c, val);
   }
 }
 
-// fp32 field -> channel 0 of a bf16 tensor with C channels (others zero)
+// fp32 field -> channel 0 of a bf16 tensor with C channels (others zero);
+// one thread per voxel
 __global__ void k_pack_input(float const *__restrict__ src, uint16_t *__restrict__ x,
                              int64_t nvox, int C, float scale) {
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nvox * C;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int64_t const v = idx / C;
-    int const c = (int)(idx - v * C);
-    x[idx] = c == 0 ? f2bf(src[v] * scale) : (uint16_t)0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    u16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+    u16x8 first = z;
+    first[0] = f2bf(src[v] * scale);
+    st8(x + v * C, first);
+    for (int c = 8; c < C; c += 8) st8(x + v * C + c, z);
   }
 }
 
@@ -545,16 +770,20 @@ extern "C" {
 
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
-  if (N < 1 || D < 1 || H < 1 || W < 1 || H > 65535 || (int64_t)N * D > 65535) {
+  if (N < 1 || D < 1 || H < 1 || W < 1 || H > 65535 || (int64_t)N * D > 65535 ||
+      (int64_t)H * W * (CI > CO ? CI : CO) >= ((int64_t)1 << 32)) {
     fprintf(stderr, "ERROR: convolution shape %d x %d x %d x %d unsupported\n", N, D, H, W);
     return 1;
   }
-  dim3 const grid((unsigned)((W + 63) / 64), (unsigned)H, (unsigned)(N * D));
-#define CC(a, b)                                                                      \
-  if (CI == a && CO == b)                                                             \
-    return ulaunch("conv3", k_conv3<a, b>, grid, dim3(UT), stream, (uint16_t const *)x, \
-                   (uint16_t const *)wt, bias, (uint16_t const *)gate, (uint16_t *)y, D, \
-                   H, W, mode);
+#define CC(a, b)                                                                          \
+  if (CI == a && CO == b) {                                                               \
+    int const db = ConvDepth<b>::DB;                                                      \
+    dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H), \
+                    (unsigned)(N * ((D + db - 1) / db)));                                 \
+    return ulaunch("conv3", k_conv3<a, b>, grid, dim3(UT), stream, (uint16_t const *)x,     \
+                   (uint16_t const *)wt, bias, (uint16_t const *)gate, (uint16_t *)y, D,     \
+                   H, W, mode);                                                           \
+  }
   CONV_CASES(CC)
 #undef CC
   fprintf(stderr, "ERROR: no convolution kernel for %d -> %d channels\n", CI, CO);
@@ -649,7 +878,11 @@ int agdu_maxpool(void const *x, void *y, void *arg, int N, int D, int H, int W, 
     fprintf(stderr, "ERROR: max pooling needs even sizes, got %d x %d x %d\n", D, H, W);
     return 1;
   }
-  int64_t const total = (int64_t)N * (D / 2) * (H / 2) * (W / 2) * C;
+  if (C % 8) {
+    fprintf(stderr, "ERROR: channel count %d is not a multiple of 8\n", C);
+    return 1;
+  }
+  int64_t const total = (int64_t)N * (D / 2) * (H / 2) * (W / 2) * (C / 8);
   return ulaunch("maxpool", k_maxpool, dim3((unsigned)blocks_capped(total, UT)), dim3(UT),
                  stream, (uint16_t const *)x, (uint16_t *)y, (uint8_t *)arg, N, D, H, W, C);
 }
@@ -660,7 +893,11 @@ int agdu_maxpool_bwd(void const *dy, void const *arg, void *dx, int N, int D, in
     fprintf(stderr, "ERROR: max pooling needs even sizes, got %d x %d x %d\n", D, H, W);
     return 1;
   }
-  int64_t const total = (int64_t)N * D * H * W * C;
+  if (C % 8) {
+    fprintf(stderr, "ERROR: channel count %d is not a multiple of 8\n", C);
+    return 1;
+  }
+  int64_t const total = (int64_t)N * D * H * W * (C / 8);
   return ulaunch("maxpool_bwd", k_maxpool_bwd, dim3((unsigned)blocks_capped(total, UT)),
                  dim3(UT), stream, (uint16_t const *)dy, (uint8_t const *)arg,
                  (uint16_t *)dx, N, D, H, W, C);
@@ -668,14 +905,22 @@ int agdu_maxpool_bwd(void const *dy, void const *arg, void *dx, int N, int D, in
 
 int agdu_upsample(void const *x, void *y, int N, int D, int H, int W, int C, int Cy, int c0,
                   void *stream) {
-  int64_t const total = (int64_t)N * 8 * D * H * W * C;
+  if ((C | Cy | c0) % 8 || c0 + C > Cy) {
+    fprintf(stderr, "ERROR: upsampling channels %d at %d of %d unsupported\n", C, c0, Cy);
+    return 1;
+  }
+  int64_t const total = (int64_t)N * 8 * D * H * W * (C / 8);
   return ulaunch("upsample", k_upsample, dim3((unsigned)blocks_capped(total, UT)), dim3(UT),
                  stream, (uint16_t const *)x, (uint16_t *)y, N, D, H, W, C, Cy, c0);
 }
 
 int agdu_upsample_bwd(void const *dy, void *dx, int N, int D, int H, int W, int C, int Cy,
                       int c0, void *stream) {
-  int64_t const total = (int64_t)N * D * H * W * C;
+  if ((C | Cy | c0) % 8 || c0 + C > Cy) {
+    fprintf(stderr, "ERROR: upsampling channels %d at %d of %d unsupported\n", C, c0, Cy);
+    return 1;
+  }
+  int64_t const total = (int64_t)N * D * H * W * (C / 8);
   return ulaunch("upsample_bwd", k_upsample_bwd, dim3((unsigned)blocks_capped(total, UT)),
                  dim3(UT), stream, (uint16_t const *)dy, (uint16_t *)dx, N, D, H, W, C, Cy,
                  c0);
@@ -683,15 +928,24 @@ int agdu_upsample_bwd(void const *dy, void *dx, int N, int D, int H, int W, int 
 
 int agdu_copy_channels(void const *x, void *y, int64_t nvox, int Cx, int cx0, int Cy,
                        int cy0, int C, int mode, void const *gate, void *stream) {
+  if ((Cx | cx0 | Cy | cy0 | C) % 8 || cx0 + C > Cx || cy0 + C > Cy) {
+    fprintf(stderr, "ERROR: channel copy %d at %d of %d -> %d of %d unsupported\n", C, cx0, Cx,
+            cy0, Cy);
+    return 1;
+  }
   return ulaunch("copy_channels", k_copy_channels,
-                 dim3((unsigned)blocks_capped(nvox * C, UT)), dim3(UT), stream,
+                 dim3((unsigned)blocks_capped(nvox * (C / 8), UT)), dim3(UT), stream,
                  (uint16_t const *)x, (uint16_t *)y, nvox, Cx, cx0, Cy, cy0, C, mode,
                  (uint16_t const *)gate);
 }
 
 int agdu_pack_input(float const *src, void *x, int64_t nvox, int C, float scale,
                     void *stream) {
-  return ulaunch("pack_input", k_pack_input, dim3((unsigned)blocks_capped(nvox * C, UT)),
+  if (C % 8) {
+    fprintf(stderr, "ERROR: channel count %d is not a multiple of 8\n", C);
+    return 1;
+  }
+  return ulaunch("pack_input", k_pack_input, dim3((unsigned)blocks_capped(nvox, UT)),
                  dim3(UT), stream, src, (uint16_t *)x, nvox, C, scale);
 }
 
