@@ -87,19 +87,32 @@ __device__ __forceinline__ void st_swz(uint16_t *base, int row, int half, uint4 
   *(uint2 *)(base + swz_q(row, 2 * half + 1)) = make_uint2(v.z, v.w);
 }
 
-template <int CO>
+template <int CO, int CU>
 struct ConvDepth {
-  static constexpr int DB = CO <= 16 ? 4 : (CO <= 32 ? 2 : 1);
+  // depth planes per block: the ring saves input reads, the accumulators
+  // (DB x 4 x CO / 16 tiles) cost registers
+  static constexpr int DB = CO <= 16 ? (CU ? 2 : 4) : (CO <= 32 ? 2 : 1);
 };
 
-template <int CI, int CO>
+// CU > 0: the input is the concatenation [upsample(xu), x] without being
+// formed: input channels [0, CU) are xu's ([n][D/2][H/2][W/2][CU], nearest
+// x2 upsampling, D, H, W even) and [CU, CI) are x's ([n][D][H][W][CI - CU]).
+// IO = 1: the input is a one-channel fp32 field (x points to floats): input
+// channel 0 is bf16(field * fscale), channels 1..15 are zero (CI = 16).
+// IO = 2: the output is output channel 0 in fp32, dst[v] = base[v] +
+// fscale * (conv + bias[0]) (base may be NULL; W a multiple of 4).
+template <int CI, int CO, int CU, int IO>
 __global__ __launch_bounds__(UT) void k_conv3(
-    uint16_t const *__restrict__ x, uint16_t const *__restrict__ wt,
-    float const *__restrict__ bias, uint16_t const *__restrict__ gate,
-    uint16_t *__restrict__ y, int D, int H, int W, int mode) {
-  static_assert(CI % 16 == 0 && CO % 16 == 0, "channels in multiples of 16");
+    uint16_t const *__restrict__ x, uint16_t const *__restrict__ xu,
+    uint16_t const *__restrict__ wt, float const *__restrict__ bias,
+    uint16_t const *__restrict__ gate, uint16_t *__restrict__ y, int D, int H, int W,
+    int mode, float fscale, float const *__restrict__ base, float *__restrict__ dst) {
+  static_assert(CI % 16 == 0 && CO % 16 == 0 && CU % 16 == 0 && CU < CI,
+                "channels in multiples of 16");
+  static_assert(IO != 1 || (CI == 16 && CU == 0), "field input has one chunk");
+  constexpr int CS = CI - CU;  // channels of x
   constexpr int NT = CO / 16;
-  constexpr int DB = ConvDepth<CO>::DB;
+  constexpr int DB = ConvDepth<CO, CU>::DB;
   constexpr int PW = CT_W + 2, PH = CT_H + 2;
   constexpr int PLANE = PH * PW * 16;  // bf16 elements of one staged plane
   constexpr int NPI = PH * PW * 2;     // staging items (8 channels) per plane
@@ -127,14 +140,19 @@ __global__ __launch_bounds__(UT) void k_conv3(
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[j][m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  uint16_t const *const xn = x + (int64_t)n * D * H * W * CI;
-  int64_t const plane = (int64_t)H * W * CI;
+  constexpr int XS = IO == 1 ? 1 : CS;  // x elements per voxel
+  uint16_t const *const xn = x + (int64_t)n * D * H * W * (IO == 1 ? 2 : CS);
+  float const *const fn = (float const *)xn;
+  int64_t const plane = (int64_t)H * W * XS;
+  int const Hu = H / 2, Wu = W / 2;
+  uint16_t const *const xun = CU ? xu + (int64_t)n * (D / 2) * Hu * Wu * CU : nullptr;
+  int64_t const planeu = (int64_t)Hu * Wu * CU;
   constexpr int NPK = (NPI + UT - 1) / UT;  // staging items per thread and plane
   // per staging item: 32-bit offset within a plane (the same for every
   // plane and chunk) and whether its (h, w) lies inside the volume; items
   // past the last repeat it (same value, same place: no branch, so the
   // compiler keeps the loads ahead of the stores)
-  uint32_t ioff[NPK];
+  uint32_t ioff[NPK], iofu[NPK];
   bool iok[NPK];
 #pragma unroll
   for (int k = 0; k < NPK; ++k) {
@@ -145,16 +163,34 @@ __global__ __launch_bounds__(UT) void k_conv3(
     int const pw = r % PW, ph = r / PW;
     int const hh = h0 + ph - 1, ww = w0 + pw - 1;
     iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
-    ioff[k] = iok[k] ? (uint32_t)((hh * W + ww) * CI + half * 8) : (uint32_t)(half * 8);
+    ioff[k] = IO == 1 ? (iok[k] ? (uint32_t)(hh * W + ww) : 0u)
+                      : (iok[k] ? (uint32_t)((hh * W + ww) * CS + half * 8) : (uint32_t)(half * 8));
+    iofu[k] = iok[k] ? (uint32_t)(((hh >> 1) * Wu + (ww >> 1)) * CU + half * 8)
+                     : (uint32_t)(half * 8);
   }
   // global loads of plane dd of chunk c (zeros outside the volume) ...
   auto load_plane = [&](int dd, int c, uint4 (&v)[NPK]) {
     bool const okd = dd >= 0 && dd < D;
-    uint16_t const *const src = xn + (okd ? dd : 0) * plane + c * 16;
+    if (IO == 1) {
+      // channel 0 from the field; the item's other channels are zero
+      float const *const fsrc = fn + (okd ? dd : 0) * plane;
+#pragma unroll
+      for (int k = 0; k < NPK; ++k) {
+        int const e0 = tid + k * UT;
+        bool const first = ((e0 < NPI ? e0 : NPI - 1) & 1) == 0;
+        uint32_t const msk = (okd && iok[k] && first) ? 0xffffu : 0u;
+        float const f = fsrc[ioff[k]];
+        v[k] = make_uint4((uint32_t)f2bf(f * fscale) & msk, 0u, 0u, 0u);
+      }
+      return;
+    }
+    bool const up = CU > 0 && c < CU / 16;  // the same for the whole block
+    uint16_t const *const src = up ? xun + (okd ? dd >> 1 : 0) * planeu + c * 16
+                                   : xn + (okd ? dd : 0) * plane + (c - CU / 16) * 16;
 #pragma unroll
     for (int k = 0; k < NPK; ++k) {
       uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;  // a mask, not a branch
-      uint4 const t = *(uint4 const *)(src + ioff[k]);
+      uint4 const t = *(uint4 const *)(src + (up ? iofu[k] : ioff[k]));
       v[k] = make_uint4(t.x & msk, t.y & msk, t.z & msk, t.w & msk);
     }
   };
@@ -248,6 +284,34 @@ __global__ __launch_bounds__(UT) void k_conv3(
   // channel t * 16 + i; the row padding puts the four kq groups in distinct
   // banks), then stores 16-byte pieces of 8 channels, coalesced.
   int const h = h0 + wave;
+  if (IO == 2) {
+    // lanes of channel 0 (i == 0) hold 4 consecutive voxels of each tile
+    if (i == 0 && h < H) {
+      float const b0 = bias[0];
+#pragma unroll
+      for (int j = 0; j < DB; ++j) {
+        int const d = d0 + j;
+        if (d >= D) break;
+        int64_t const orow = (((int64_t)n * D + d) * H + h) * W;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          int const wv = w0 + 16 * m + 4 * kq;
+          if (wv >= W) continue;
+          float4 o = make_float4((acc[j][m][0][0] + b0) * fscale, (acc[j][m][0][1] + b0) * fscale,
+                                 (acc[j][m][0][2] + b0) * fscale, (acc[j][m][0][3] + b0) * fscale);
+          if (base) {
+            float4 const bb = *(float4 const *)(base + orow + wv);
+            o.x += bb.x;
+            o.y += bb.y;
+            o.z += bb.z;
+            o.w += bb.w;
+          }
+          *(float4 *)(dst + orow + wv) = o;
+        }
+      }
+    }
+    return;
+  }
   uint16_t *const ob = smem + wave * 64 * OBS;
 #pragma unroll
   for (int j = 0; j < DB; ++j) {
@@ -768,26 +832,84 @@ static inline int64_t blocks_capped(int64_t n, int t) {
 
 extern "C" {
 
-int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
-               int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
+static int conv_shape_ok(int N, int D, int H, int W, int CI, int CO) {
   if (N < 1 || D < 1 || H < 1 || W < 1 || H > 65535 || (int64_t)N * D > 65535 ||
       (int64_t)H * W * (CI > CO ? CI : CO) >= ((int64_t)1 << 32)) {
     fprintf(stderr, "ERROR: convolution shape %d x %d x %d x %d unsupported\n", N, D, H, W);
-    return 1;
+    return 0;
   }
-#define CC(a, b)                                                                          \
-  if (CI == a && CO == b) {                                                               \
-    int const db = ConvDepth<b>::DB;                                                      \
-    dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H), \
-                    (unsigned)(N * ((D + db - 1) / db)));                                 \
-    return ulaunch("conv3", k_conv3<a, b>, grid, dim3(UT), stream, (uint16_t const *)x,     \
-                   (uint16_t const *)wt, bias, (uint16_t const *)gate, (uint16_t *)y, D,     \
-                   H, W, mode);                                                           \
+  return 1;
+}
+
+#define CONV_LAUNCH_IO(a, b, u, io, fs, bs, ds)                                              \
+  {                                                                                        \
+    int const db = ConvDepth<b, u>::DB;                                                    \
+    dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H),  \
+                    (unsigned)(N * ((D + db - 1) / db)));                                  \
+    return ulaunch("conv3", k_conv3<a, b, u, io>, grid, dim3(UT), stream,                    \
+                   (uint16_t const *)x, (uint16_t const *)xu, (uint16_t const *)wt, bias,    \
+                   (uint16_t const *)gate, (uint16_t *)y, D, H, W, mode, fs, bs, ds);       \
   }
+#define CONV_LAUNCH(a, b, u) CONV_LAUNCH_IO(a, b, u, 0, 1.0f, (float const *)0, (float *)0)
+
+int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
+               int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
+  void const *const xu = NULL;
+  if (!conv_shape_ok(N, D, H, W, CI, CO)) return 1;
+#define CC(a, b) \
+  if (CI == a && CO == b) CONV_LAUNCH(a, b, 0)
   CONV_CASES(CC)
 #undef CC
   fprintf(stderr, "ERROR: no convolution kernel for %d -> %d channels\n", CI, CO);
   return 1;
+}
+
+#define CAT_CASES(X) X(32, 16, 16) X(64, 32, 32)
+
+int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *bias,
+                   void const *gate, void *y, int N, int D, int H, int W, int CU, int CS,
+                   int CO, int mode, void *stream) {
+  if (!conv_shape_ok(N, D, H, W, CU + CS, CO)) return 1;
+  if ((D | H | W) & 1) {
+    fprintf(stderr, "ERROR: upsampled concatenation needs even sizes, got %d x %d x %d\n", D,
+            H, W);
+    return 1;
+  }
+#define CT(u, c, o) \
+  if (CU == u && CS == c && CO == o) CONV_LAUNCH(u + c, o, u)
+  CAT_CASES(CT)
+#undef CT
+  fprintf(stderr, "ERROR: no concatenating convolution for %d + %d -> %d channels\n", CU, CS,
+          CO);
+  return 1;
+}
+
+int agdu_conv_in(float const *src, void const *wt, int wci, float const *bias, void *y, int N,
+                 int D, int H, int W, float scale, void *stream) {
+  void const *const x = src;
+  void const *const xu = NULL;
+  void const *const gate = NULL;
+  int const mode = 3;
+  if (!conv_shape_ok(N, D, H, W, 16, 16)) return 1;
+  if (wci != 16) {
+    fprintf(stderr, "ERROR: first-layer weights must have 16 input channels, got %d\n", wci);
+    return 1;
+  }
+  CONV_LAUNCH_IO(16, 16, 0, 1, scale, (float const *)0, (float *)0)
+}
+
+int agdu_conv_out(void const *x, void const *wt, int wco, float const *bias, float const *base,
+                  float *dst, int N, int D, int H, int W, float unscale, void *stream) {
+  void const *const xu = NULL;
+  void const *const gate = NULL;
+  void *const y = NULL;
+  int const mode = 0;
+  if (!conv_shape_ok(N, D, H, W, 16, 16)) return 1;
+  if (wco != 16 || W % 4) {
+    fprintf(stderr, "ERROR: head with %d output channels and W = %d unsupported\n", wco, W);
+    return 1;
+  }
+  CONV_LAUNCH_IO(16, 16, 0, 2, unscale, base, dst)
 }
 
 int agdu_flip_weights(void const *wt, void *wtt, int CI, int CO, void *stream) {

@@ -141,11 +141,14 @@ class UNet:
             f = 2 ** lv
             return torch.empty((N, D // f, H // f, W // f, c), dtype=dtype, device=self.dev)
 
-        B = dict(x=t(0, 16), e1a=t(0, 16), e1b=t(0, 16), p1=t(1, 16), a1=t(1, 16, torch.uint8),
+        B = dict(e1a=t(0, 16), e1b=t(0, 16), p1=t(1, 16), a1=t(1, 16, torch.uint8),
                  e2a=t(1, 32), e2b=t(1, 32), p2=t(2, 32), a2=t(2, 32, torch.uint8),
-                 b1=t(2, 64), b2=t(2, 64), c2=t(1, 96), d2a=t(1, 32), d2b=t(1, 32),
-                 c1=t(0, 48), d1a=t(0, 16), d1b=t(0, 16), head=t(0, 16))
+                 b1=t(2, 64), b2=t(2, 64), d2a=t(1, 32), d2b=t(1, 32), d1a=t(0, 16),
+                 d1b=t(0, 16))
         if train:
+            # the training pass keeps the packed input, the concatenations and
+            # the head's 16 channels for the backward pass
+            B.update(x=t(0, 16), c2=t(1, 96), c1=t(0, 48), head=t(0, 16))
             B.update(g16a=t(0, 16), g16b=t(0, 16), g48=t(0, 48), gh32a=t(1, 32),
                      gh32b=t(1, 32), gh96=t(1, 96), gh16=t(1, 16), gq64a=t(2, 64),
                      gq64b=t(2, 64), gq32=t(2, 32))
@@ -210,31 +213,55 @@ class UNet:
         return B["head"]
 
     def predict_ptr(self, src_ptr, shape, out):
-        """predict() on a float32 device array given by address (e.g. the
-        patch data of a deblending problem, agdx_data_dev), written into the
-        device tensor out (same shape)."""
+        """Estimate of the unblended data from the pseudo-deblended field at
+        device address src_ptr (float32 [N, D, H, W], e.g. the patch data of
+        a deblending problem, agdx_data_dev): field + head[0] / scale, into
+        the float32 device tensor out. The inference pass: the first layer
+        reads the field itself, the decoder convolutions read the upsampled
+        and skip tensors in place of their concatenation, and the head
+        computes only its channel 0, in fp32, added to the field."""
         N, D, H, W = shape
         if tuple(out.shape) != tuple(shape) or out.dtype != torch.float32:
             raise ValueError("out must be float32 of shape %s" % (shape,))
         B = self._buffers(tuple(shape), False)
-        nvox = N * D * H * W
+        d0, d1, d2 = (N, D, H, W), (N, D // 2, H // 2, W // 2), (N, D // 4, H // 4, W // 4)
         s = self._s()
-        _check(self.L.agdu_pack_input(ctypes.c_void_p(src_ptr), _vp(B["x"]), nvox, CH,
-                                      self.scale, s), "agdu_pack_input")
-        head = self._forward_packed(B, shape)
-        _check(self.L.agdu_unpack_output(_vp(head), _vp(out), ctypes.c_void_p(src_ptr), nvox, CH,
-                                         1.0 / self.scale, s), "agdu_unpack_output")
+        L = self.L
+        src = ctypes.c_void_p(src_ptr)
+        _check(L.agdu_conv_in(src, self._wptr("e1a"), 16, self._bptr("e1a"), _vp(B["e1a"]), *d0,
+                              self.scale, s), "agdu_conv_in")
+        self._conv("e1b", B["e1a"], B["e1b"], d0, 3)
+        _check(L.agdu_maxpool(_vp(B["e1b"]), _vp(B["p1"]), _vp(B["a1"]), *d0, 16, s),
+               "agdu_maxpool")
+        self._conv("e2a", B["p1"], B["e2a"], d1, 3)
+        self._conv("e2b", B["e2a"], B["e2b"], d1, 3)
+        _check(L.agdu_maxpool(_vp(B["e2b"]), _vp(B["p2"]), _vp(B["a2"]), *d1, 32, s),
+               "agdu_maxpool")
+        self._conv("b1", B["p2"], B["b1"], d2, 3)
+        self._conv("b2", B["b1"], B["b2"], d2, 3)
+        _check(L.agdu_conv3_cat(_vp(B["b2"]), _vp(B["e2b"]), self._wptr("d2a"), self._bptr("d2a"),
+                                None, _vp(B["d2a"]), *d1, 64, 32, 32, 3, s), "agdu_conv3_cat d2a")
+        self._conv("d2b", B["d2a"], B["d2b"], d1, 3)
+        _check(L.agdu_conv3_cat(_vp(B["d2b"]), _vp(B["e1b"]), self._wptr("d1a"), self._bptr("d1a"),
+                                None, _vp(B["d1a"]), *d0, 32, 16, 16, 3, s), "agdu_conv3_cat d1a")
+        self._conv("d1b", B["d1a"], B["d1b"], d0, 3)
+        _check(L.agdu_conv_out(_vp(B["d1b"]), self._wptr("head"), 16, self._bptr("head"), src,
+                               _vp(out), *d0, 1.0 / self.scale, s), "agdu_conv_out")
         return out
 
     def predict(self, field, out=None):
-        """Estimate of the unblended data from the pseudo-deblended field
-        (float32 device tensor [N, D, H, W]): field + head[0] / scale."""
-        B = self._buffers(tuple(field.shape), False)
-        head = self.forward(field, B)
+        """predict_ptr on a float32 device tensor [N, D, H, W]."""
         if out is None:
             out = torch.empty_like(field)
-        nvox = field.numel()
-        _check(self.L.agdu_unpack_output(_vp(head), _vp(out), _vp(field), nvox, CH,
+        return self.predict_ptr(field.data_ptr(), tuple(field.shape), out)
+
+    def predict_unfused(self, field):
+        """The same estimate through the training pass's kernels (packed
+        input, formed concatenations, 16-channel head), for comparison."""
+        B = self._buffers(tuple(field.shape), True)
+        head = self.forward(field, B)
+        out = torch.empty_like(field)
+        _check(self.L.agdu_unpack_output(_vp(head), _vp(out), _vp(field), field.numel(), CH,
                                          1.0 / self.scale, self._s()), "agdu_unpack_output")
         return out
 

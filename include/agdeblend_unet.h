@@ -24,6 +24,23 @@ extern "C" {
  * (gate > 0) if mode & 4 (gate has y's layout) */
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream);
+/* agdu_conv3 on the concatenation [upsample(xu), x] without forming it:
+ * xu is [N][D/2][H/2][W/2][CU] (nearest x2 upsampling; D, H, W even), x is
+ * [N][D][H][W][CS]; wt is [27][CO][CU + CS] */
+int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *bias,
+                   void const *gate, void *y, int N, int D, int H, int W, int CU, int CS,
+                   int CO, int mode, void *stream);
+/* First layer from a one-channel fp32 field:
+ * y[v][co] = ReLU(bias[co] + sum_tap wt[tap][co][0] bf16(src[v + tap] scale)),
+ * 16 output channels, wt [27][16][wci] with wci = 16 (the packed input's
+ * channels 1..15 are zero) */
+int agdu_conv_in(float const *src, void const *wt, int wci, float const *bias, void *y, int N,
+                 int D, int H, int W, float scale, void *stream);
+/* Output channel 0 of a 16 -> 16 convolution, in fp32:
+ * dst[v] = base[v] + unscale (bias[0] + sum_tap sum_ci wt[tap][0][ci] x[v + tap][ci])
+ * (base may be NULL), wt [27][wco][16] with wco = 16, W a multiple of 4 */
+int agdu_conv_out(void const *x, void const *wt, int wco, float const *bias, float const *base,
+                  float *dst, int N, int D, int H, int W, float unscale, void *stream);
 /* weights of the input gradient: wtt[tap][ci][co] = wt[26 - tap][co][ci] */
 int agdu_flip_weights(void const *wt, void *wtt, int CI, int CO, void *stream);
 /* floats of scratch the weight gradient needs (-1: no kernel for the

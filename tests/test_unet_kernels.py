@@ -198,3 +198,56 @@ def test_loss_and_adam():
         wr = wr - 1e-2 * (mr / (1 - 0.9 ** step)) / (np.sqrt(vr / (1 - 0.999 ** step)) + 1e-8)
     assert np.abs(w.cpu().numpy() - wr).max() <= 1e-5
     assert np.array_equal(wb.float().cpu().numpy(), R.bf16_round(w.cpu().numpy()).astype(np.float32))
+
+
+@pytest.mark.parametrize("cu,cs,co", [(32, 16, 16), (64, 32, 32)])
+def test_conv3_cat_equals_formed_concatenation(cu, cs, co):
+    """The convolution reading [upsample(xu), x] in place gives the same bits
+    as upsampling, concatenating and convolving."""
+    rng = np.random.default_rng(cu + cs)
+    N, D, H, W = 1, 4, 6, 70
+    xu = _bf(rng.standard_normal((N, D // 2, H // 2, W // 2, cu)))
+    xs = _bf(rng.standard_normal((N, D, H, W, cs)))
+    w = _bf(rng.standard_normal((27, co, cu + cs)) / np.sqrt(27 * (cu + cs)))
+    b = torch.tensor(rng.standard_normal(co).astype(np.float32), device="cuda")
+    L = _lib()
+    cat = torch.zeros((N, D, H, W, cu + cs), dtype=torch.bfloat16, device="cuda")
+    assert L.agdu_upsample(_p(xu), _p(cat), N, D // 2, H // 2, W // 2, cu, cu + cs, 0, None) == 0
+    assert L.agdu_copy_channels(_p(xs), _p(cat), N * D * H * W, cs, 0, cu + cs, cu, cs, 0, None,
+                                None) == 0
+    y1 = torch.empty((N, D, H, W, co), dtype=torch.bfloat16, device="cuda")
+    y2 = torch.empty_like(y1)
+    assert L.agdu_conv3(_p(cat), _p(w), _p(b), None, _p(y1), N, D, H, W, cu + cs, co, 3,
+                        None) == 0
+    assert L.agdu_conv3_cat(_p(xu), _p(xs), _p(w), _p(b), None, _p(y2), N, D, H, W, cu, cs, co,
+                            3, None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+
+
+def test_first_layer_and_head_kernels():
+    rng = np.random.default_rng(21)
+    N, D, H, W = 1, 3, 5, 72  # the head needs W % 4 == 0
+    field = rng.standard_normal((N, D, H, W)).astype(np.float32)
+    w = R.bf16_round(rng.standard_normal((27, 16, 16)) / 6.0)
+    b = rng.standard_normal(16).astype(np.float32) * 0.1
+    L = _lib()
+    f_d = torch.tensor(field, device="cuda")
+    w_d = _bf(w)
+    b_d = torch.tensor(b, device="cuda")
+    y = torch.empty((N, D, H, W, 16), dtype=torch.bfloat16, device="cuda")
+    assert L.agdu_conv_in(_p(f_d), _p(w_d), 16, _p(b_d), _p(y), N, D, H, W, 1.5, None) == 0
+    x = np.zeros((N, D, H, W, 16))
+    x[..., 0] = R.bf16_round(field * 1.5)
+    ref = R.conv3(x, w, b.astype(np.float64), relu=True)
+    torch.cuda.synchronize()
+    _close(_np(y), ref, 1e-2)
+    # head: channel 0 of a 16 -> 16 convolution, fp32, plus base
+    xin = R.bf16_round(rng.standard_normal((N, D, H, W, 16)))
+    xin_d = _bf(xin)
+    out = torch.empty((N, D, H, W), dtype=torch.float32, device="cuda")
+    assert L.agdu_conv_out(_p(xin_d), _p(w_d), 16, _p(b_d), _p(f_d), _p(out), N, D, H, W, 0.25,
+                           None) == 0
+    torch.cuda.synchronize()
+    ref = field + 0.25 * R.conv3(xin, w, b.astype(np.float64))[..., 0]
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-4 * np.abs(ref).max()

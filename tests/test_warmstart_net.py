@@ -61,9 +61,25 @@ def test_prediction_is_input_plus_scaled_head():
     rng = np.random.default_rng(8)
     field = rng.standard_normal((1, 4, 4, 128)).astype(np.float32)
     f_d = torch.tensor(field, device="cuda")
-    y = net.predict(f_d).cpu().numpy()
-    head = net._buffers(field.shape, False)["head"].float().cpu().numpy()
+    y = net.predict_unfused(f_d).cpu().numpy()
+    head = net._buffers(field.shape, True)["head"].float().cpu().numpy()
     assert np.allclose(y, field + head[..., 0] / 0.5, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 12, 200), (2, 4, 8, 64)])
+def test_inference_pass_matches_training_pass(shape):
+    """The inference kernels (first layer from the field, concatenations read
+    in place, fp32 head channel) against the training pass's kernels: the
+    same network, so the estimates differ only by rounding (bf16 of the
+    head's output, fp32 summation order)."""
+    net = _net(6)
+    net.scale = 1.7
+    rng = np.random.default_rng(10)
+    field = rng.standard_normal(shape).astype(np.float32)
+    f_d = torch.tensor(field, device="cuda")
+    a = net.predict(f_d).cpu().numpy().astype(np.float64) - field
+    b = net.predict_unfused(f_d).cpu().numpy().astype(np.float64) - field
+    assert _rel(a, b) < 2e-2
 
 
 def test_training_reduces_the_loss():
