@@ -40,7 +40,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FK_KERNELS = ["fwd_spatial", "fwd_time", "adj_time", "adj_spatial", "shrink"]
 OTHER_KERNELS = ["blend", "wavelet_fwd", "wavelet_adj", "norm2"]
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "fk_operator_c1_hbm_traffic.md")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(*a):
@@ -67,17 +67,22 @@ def make_workload(cfg):
 
 def traffic_from_summary(workload):
     """HBM bytes per iteration of the FK kernels from the committed counter
-    summary (a line 'traffic_bytes_per_iteration <workload, spaces as _>
-    <bytes>'), or None."""
+    summaries profiles/fk_operator_*_hbm_traffic.md (a line
+    'traffic_bytes_per_iteration <workload, spaces as _> <bytes>'), or None."""
+    key = workload.replace(" ", "_")
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        names = sorted(os.listdir(PROFILES))
+    except OSError:
+        return None
+    for name in names:
+        if not (name.startswith("fk_operator_") and name.endswith("_hbm_traffic.md")):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
             for line in f:
                 parts = line.split()
                 if (len(parts) == 3 and parts[0] == "traffic_bytes_per_iteration"
-                        and parts[1] == workload.replace(" ", "_")):
+                        and parts[1] == key):
                     return float(parts[2])
-    except OSError:
-        pass
     return None
 
 
