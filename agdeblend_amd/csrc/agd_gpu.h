@@ -79,6 +79,7 @@ typedef struct {
   int *lamb_idx;
   double *partials;           /* reduction partials */
   int n_partials;
+  int precision;              /* spatial DFTs: 0 FP32 register FFTs, 1 bf16 x 3 MFMA */
 } AgdDev;
 
 /* A gather of source traces into a flat record, or of record slices back
@@ -127,6 +128,8 @@ int agdg_window_traces(AgdDev const *g, int64_t const *src_off, int const *src_n
 int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fuse);
 /* 1 when every spatial class has a specialised kernel (fusion available) */
 int agdg_can_fuse(AgdDev const *g);
+/* number of spatial classes the split-precision transforms can serve */
+int agdg_x3_classes(AgdDev const *g);
 int agdg_shrink(AgdDev const *g, int64_t n_complex, float lamb);
 /* fold of spatial tapers for the stepsize; max written to scalar[1] */
 int agdg_stepsize_fold(AgdDev const *g);

@@ -2172,6 +2172,19 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
   return agdg_sync();
 }
 
+/* Spatial DFT precision of the problem's operators: 0 the FP32 register
+ * FFTs (default), 1 split-precision bf16 x 3 matrix-core DFTs for the
+ * windows they serve (square P in {20, 40}); the others keep the FP32
+ * path. Returns the number of window classes the bf16 x 3 path serves, or
+ * -1 on error. The split-precision path has no fused adjoint. */
+EXPORT int agdx_set_spatial_precision(AgdProblem *p, int mode) {
+  if (p->host_only || (mode != 0 && mode != 1)) return -1;
+  p->dev.precision = mode;
+  p->fuse = agdg_can_fuse(&p->dev) && getenv("AGD_NO_FUSE") == NULL;
+  p->u_valid = 0;
+  return mode ? agdg_x3_classes(&p->dev) : 0;
+}
+
 EXPORT int agdx_iterate(AgdProblem *p, int n, float lamb) {
   float *sched;
   int it, rc;

@@ -1421,6 +1421,270 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
 }
 
 // ---------------------------------------------------------------------------
+// Split-precision (bf16 x 3) spatial DFTs on the matrix cores, beside the
+// FP32 register FFTs above (selected per problem, agdx_set_spatial_precision).
+// A window's spatial transform of one plane is two complex matrix products
+// with constant DFT matrices: forward U = A0 X A1^T (A[n][c] = e^{+2 pi i n c / P},
+// n < W, c < P: the inverse DFT cropped to the window), adjoint
+// G = B0 V B1^T (B[c][n] = e^{-2 pi i c n / P}: the DFT of the zero-padded
+// traces). Each complex product is one real GEMM on stacked real and
+// imaginary parts ([Yr; Yi] = [[Cr, -Ci]; [Ci, Cr]] [Zr; Zi]); each FP32
+// operand v is split into bf16 parts h = bf16(v), l = bf16(v - h), and the
+// product accumulates h.h + h.l + l.h in FP32 on v_mfma_f32_16x16x16_bf16
+// (the dropped l.l term is ~2^-16 relative). Square windows, P in {20, 40},
+// tiled U / V; one plane at a time per block, its operands in LDS.
+// ---------------------------------------------------------------------------
+namespace x3 {
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned int const u = __float_as_uint(f);
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // nearest even
+}
+__device__ __forceinline__ float bf2f(unsigned short h) {
+  return __uint_as_float((unsigned int)h << 16);
+}
+__device__ __forceinline__ void split4(float const (&v)[4], bf16x4 &hi, bf16x4 &lo) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    unsigned short const h = f2bf(v[r]);
+    hi[r] = (short)h;
+    lo[r] = (short)f2bf(v[r] - bf2f(h));
+  }
+}
+// coefficient of output part po (0 re, 1 im) from input part pi of the
+// complex factor c: [[Cr, -Ci], [Ci, Cr]]
+__device__ __forceinline__ float part(float2 c, int po, int pi) {
+  return po == pi ? c.x : (pi == 1 ? -c.y : c.y);
+}
+// e^{SIGN 2 pi i m / P}
+template <int P, int SIGN>
+__device__ __forceinline__ float2 coef(int m) {
+  float s, c;
+  sincospif(2.f * (float)(m % P) / (float)P, &s, &c);
+  return make_float2(c, SIGN * s);
+}
+__host__ __device__ constexpr int r16(int n) { return (n + 15) / 16 * 16; }
+
+template <int P>
+struct Shape {
+  static constexpr int W = P * 4 / 5;
+  static constexpr int KP = r16(P);
+  static constexpr int KW = r16(W);
+  static constexpr int PITCH = KP + 4;     // floats per LDS row
+  static constexpr int PL = KP * PITCH;    // floats of one part of a buffer
+};
+
+// Y = C Z with C[p][q] = e^{SIGN 2 pi i p q / P} for p < MC, q < KC (else 0).
+// Z: LDS [2][rows r16(KC)][PITCH] (real, imaginary parts), columns
+// r16(NZ); Y likewise with rows r16(MC). Waves own 16-row tiles of the
+// stacked output, whose constant operands stay in registers.
+template <int P, int MC, int KC, int NZ, int SIGN>
+__device__ __forceinline__ void left(float const *__restrict__ z, float *__restrict__ y) {
+  constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
+  constexpr int KMC = r16(MC), KKC = r16(KC), KNZ = r16(NZ);
+  constexpr int NMT = 2 * KMC / 16, NKS = 2 * KKC / 16, NNT = KNZ / 16;
+  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int const i = lane & 15, kq = lane >> 4;
+  for (int mt = wave; mt < NMT; mt += NTHR / 64) {
+    int const mr = mt * 16 + i;
+    int const po = mr / KMC, pr = mr - po * KMC;
+    bf16x4 ah[NKS], al[NKS];
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int const kk = ks * 16 + 4 * kq + r;
+        int const pi = kk / KKC, q = kk - pi * KKC;
+        float2 const c = (pr < MC && q < KC) ? coef<P, SIGN>(pr * q) : make_float2(0.f, 0.f);
+        v[r] = part(c, po, pi);
+      }
+      split4(v, ah[ks], al[ks]);
+    }
+    for (int nt = 0; nt < NNT; ++nt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      int const col = nt * 16 + i;
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int const kk = ks * 16 + 4 * kq + r;
+          int const pi = kk / KKC, q = kk - pi * KKC;
+          v[r] = z[pi * PL + q * PITCH + col];
+        }
+        bf16x4 bh, bl;
+        split4(v, bh, bl);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[ks], bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[ks], bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[ks], bh, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int const yr = mt * 16 + 4 * kq + r;
+        int const yp = yr / KMC;
+        y[yp * PL + (yr - yp * KMC) * PITCH + col] = acc[r];
+      }
+    }
+  }
+}
+
+// Y = Z C^T with C[o][q] = e^{SIGN 2 pi i o q / P} for o < NC, q < KC (else
+// 0). Z: LDS rows r16(MZ), columns r16(KC); Y: rows r16(MZ), columns
+// r16(NC). Waves own 16-column tiles of the stacked output.
+template <int P, int MZ, int KC, int NC, int SIGN>
+__device__ __forceinline__ void right(float const *__restrict__ z, float *__restrict__ y) {
+  constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
+  constexpr int KMZ = r16(MZ), KKC = r16(KC), KNC = r16(NC);
+  constexpr int NNT = 2 * KNC / 16, NKS = 2 * KKC / 16, NMT = KMZ / 16;
+  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int const i = lane & 15, kq = lane >> 4;
+  for (int nt = wave; nt < NNT; nt += NTHR / 64) {
+    int const nc = nt * 16 + i;
+    int const po = nc / KNC, o = nc - po * KNC;
+    bf16x4 bh[NKS], bl[NKS];
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int const kk = ks * 16 + 4 * kq + r;
+        int const pi = kk / KKC, q = kk - pi * KKC;
+        float2 const c = (o < NC && q < KC) ? coef<P, SIGN>(o * q) : make_float2(0.f, 0.f);
+        v[r] = part(c, po, pi);
+      }
+      split4(v, bh[ks], bl[ks]);
+    }
+    for (int mt = 0; mt < NMT; ++mt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      int const row = mt * 16 + i;
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        int const kk = ks * 16 + 4 * kq;
+        int const pi = kk / KKC, q = kk - pi * KKC;
+        float4 const zz = *(float4 const *)(z + pi * PL + row * PITCH + q);
+        float const v[4] = {zz.x, zz.y, zz.z, zz.w};
+        bf16x4 ah, al;
+        split4(v, ah, al);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[ks], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[ks], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[ks], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) y[po * PL + (mt * 16 + 4 * kq + r) * PITCH + o] = acc[r];
+    }
+  }
+}
+}  // namespace x3
+
+// X -> U (the forward spatial stage), bf16 x 3 DFT matrices
+template <int P>
+__global__ __launch_bounds__(NTHR) void k_fwd_spatial_x3(
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
+    float2 const *__restrict__ X, float2 *__restrict__ U) {
+  using S = x3::Shape<P>;
+  constexpr int W = S::W, KP = S::KP, PITCH = S::PITCH, PL = S::PL;
+  __shared__ __align__(16) float za[2 * PL];
+  __shared__ __align__(16) float zb[2 * PL];
+  AgdTask const t = tasks[task0 + blockIdx.x];
+  AgdWin const &w = wins[t.win];
+  float2 const *const src = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * (P * P);
+  float2 *const Utw = U + w.u_off + (int64_t)t.tw * w.Ft * (W * W);
+  for (int k = 0; k < t.nk; ++k) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < KP * KP; e += NTHR) {
+      int const r = e / KP, c = e - r * KP;
+      float2 const v = (r < P && c < P) ? src[(int64_t)k * P * P + r * P + c]
+                                        : make_float2(0.f, 0.f);
+      za[r * PITCH + c] = v.x;
+      za[PL + r * PITCH + c] = v.y;
+    }
+    __syncthreads();
+    x3::left<P, W, P, P, +1>(za, zb);   // T = A0 X: rows n0, columns c1
+    __syncthreads();
+    x3::right<P, W, P, W, +1>(zb, za);  // U = T A1^T: rows n0, columns n1
+    __syncthreads();
+    for (int tr = threadIdx.x; tr < W * W; tr += NTHR) {
+      int const n0 = tr / W, n1 = tr - n0 * W;
+      Utw[((int64_t)(tr / TG) * w.Ft + t.k0 + k) * TG + (tr % TG)] =
+          make_float2(za[n0 * PITCH + n1], za[PL + n0 * PITCH + n1]);
+    }
+  }
+}
+
+// V -> G and the model update by MODE (as k_adj_spatial_t), bf16 x 3
+template <int P, int MODE>
+__global__ __launch_bounds__(NTHR) void k_adj_spatial_x3(
+    AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
+    float2 const *__restrict__ V, float2 *__restrict__ X, float alpha, float lamb,
+    float const *__restrict__ lamb_dev, unsigned int *__restrict__ maxbits) {
+  using S = x3::Shape<P>;
+  constexpr int W = S::W, KW = S::KW, PITCH = S::PITCH, PL = S::PL;
+  __shared__ __align__(16) float za[2 * PL];
+  __shared__ __align__(16) float zb[2 * PL];
+  if (lamb_dev != nullptr) lamb = lamb_dev[0];
+  AgdTask const t = tasks[task0 + blockIdx.x];
+  AgdWin const &w = wins[t.win];
+  float2 const *const Vtw = V + w.u_off + (int64_t)t.tw * w.Ft * (W * W);
+  float2 *const x = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * (P * P);
+  float mx = 0.f;
+  for (int k = 0; k < t.nk; ++k) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < KW * KW; e += NTHR) {
+      int const n0 = e / KW, n1 = e - n0 * KW;
+      float2 v = make_float2(0.f, 0.f);
+      if (n0 < W && n1 < W) {
+        int const tr = n0 * W + n1;
+        v = Vtw[((int64_t)(tr / TG) * w.Ft + t.k0 + k) * TG + (tr % TG)];
+      }
+      za[n0 * PITCH + n1] = v.x;
+      za[PL + n0 * PITCH + n1] = v.y;
+    }
+    __syncthreads();
+    x3::right<P, W, W, P, -1>(za, zb);  // R = V B1^T: rows n0, columns c1
+    __syncthreads();
+    x3::left<P, P, W, P, -1>(zb, za);   // G = B0 R: rows c0, columns c1
+    __syncthreads();
+    float2 *const xk = x + (int64_t)k * P * P;
+    for (int e = threadIdx.x; e < P * P; e += NTHR) {
+      int const c0 = e / P, c1 = e - c0 * P;
+      float2 const gv = make_float2(za[c0 * PITCH + c1], za[PL + c0 * PITCH + c1]);
+      float2 nv;
+      if constexpr (MODE == 3) {
+        nv = gv;
+      } else if constexpr (MODE == 1) {
+        nv = make_float2(0.f - alpha * gv.x, 0.f - alpha * gv.y);
+        mx = fmaxf(mx, nv.x * nv.x + nv.y * nv.y);
+      } else {
+        nv = xk[e];
+        nv.x -= alpha * gv.x;
+        nv.y -= alpha * gv.y;
+        if constexpr (MODE == 0) {
+          float const a = nv.x * nv.x + nv.y * nv.y;
+          float const sh = fmaxf(1.f - lamb * __builtin_amdgcn_rsqf(a), 0.f);
+          nv.x *= sh;
+          nv.y *= sh;
+        }
+      }
+      xk[e] = nv;
+    }
+  }
+  if constexpr (MODE == 1) {
+    __shared__ float red[NTHR / 64];
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = red[0];
+      for (int q = 1; q < NTHR / 64; ++q) m = fmaxf(m, red[q]);
+      atomicMax(maxbits, __float_as_uint(m));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
 // Sizes with specialised kernels; the host groups work by these keys
@@ -1579,15 +1843,44 @@ int agdg_sync(void) {
 void *agdg_stream(void) { return (void *)g_stream; }
 
 int agdg_can_fuse(AgdDev const *g) {
+  if (g->precision != 0) return 0;  // the split-precision transforms are not fused
   for (int c = 0; c < g->n_sclass; ++c)
     if (g->sclass[c].p0 == 0) return 0;
   return 1;
+}
+
+// classes served by the split-precision transforms: square P in {20, 40},
+// tiled U / V
+static inline int x3_class(AgdDev const *g, AgdClass const &c) {
+  return g->precision == 1 && c.tb && c.p0 == c.p1 && (c.p0 == 20 || c.p0 == 40);
+}
+
+int agdg_x3_classes(AgdDev const *g) {
+  int n = 0;
+  for (int ci = 0; ci < g->n_sclass; ++ci) {
+    AgdClass c = g->sclass[ci];
+    AgdDev h = *g;
+    h.precision = 1;
+    n += x3_class(&h, c);
+  }
+  return n;
 }
 
 int agdg_fwd_spatial(AgdDev const *g, float const *X) {
   for (int ci = 0; ci < g->n_sclass; ++ci) {
     AgdClass const &c = g->sclass[ci];
     bool done = false;
+    if (x3_class(g, c)) {
+      int first;
+      int const n = class_range(g, c, &first);
+      if (n > 0 &&
+          (c.p0 == 20 ? launch("fwd_spatial_x3", k_fwd_spatial_x3<20>, dim3(n), dim3(NTHR), 0,
+                               g->win, g->tasks, first, (float2 const *)X, (float2 *)g->U)
+                      : launch("fwd_spatial_x3", k_fwd_spatial_x3<40>, dim3(n), dim3(NTHR), 0,
+                               g->win, g->tasks, first, (float2 const *)X, (float2 *)g->U)))
+        return 1;
+      continue;
+    }
 #define FWD_CASE(a, b)                                                    \
     if (!done && c.p0 == a && c.p1 == b) {                                \
       if (fwd_spatial_sized<a, b>(g, c, X)) return 1;                     \
@@ -1684,10 +1977,41 @@ int agdg_adj_time(AgdDev const *g, float const *Tin, int plain) {
 
 extern "C" {
 
+}  // extern "C"
+
+template <int P>
+static int launch_adj_x3(AgdDev const *g, AgdClass const &c, float alpha, float lamb,
+                         int mode) {
+  int first;
+  int const n = class_range(g, c, &first);
+  if (n <= 0) return 0;
+#define X3_ARGS g->win, g->tasks, first, (float2 const *)g->V, (float2 *)g->X, alpha, lamb, \
+                (float const *)g->lamb_dev, g->scalar
+  switch (mode) {
+    case 0: return launch("adj_spatial_x3", k_adj_spatial_x3<P, 0>, dim3(n), dim3(NTHR), 0, X3_ARGS);
+    case 1: return launch("adj_spatial_x3", k_adj_spatial_x3<P, 1>, dim3(n), dim3(NTHR), 0, X3_ARGS);
+    case 2: return launch("adj_spatial_x3", k_adj_spatial_x3<P, 2>, dim3(n), dim3(NTHR), 0, X3_ARGS);
+    default: return launch("adj_spatial_x3", k_adj_spatial_x3<P, 3>, dim3(n), dim3(NTHR), 0, X3_ARGS);
+  }
+#undef X3_ARGS
+}
+
+extern "C" {
+
 int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fuse) {
   for (int ci = 0; ci < g->n_sclass; ++ci) {
     AgdClass const &c = g->sclass[ci];
     bool done = false;
+    if (x3_class(g, c)) {
+      if (fuse) {
+        fprintf(stderr, "ERROR: fused update requested with split-precision transforms\n");
+        return 1;
+      }
+      if (c.p0 == 20 ? launch_adj_x3<20>(g, c, alpha, lamb, mode)
+                     : launch_adj_x3<40>(g, c, alpha, lamb, mode))
+        return 1;
+      continue;
+    }
 #define AS_CASE(a, b)                                                       \
     if (!done && c.p0 == a && c.p1 == b) {                                  \
       if (adj_spatial_sized<a, b>(g, c, alpha, lamb, mode, fuse)) return 1; \

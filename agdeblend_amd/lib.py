@@ -63,6 +63,7 @@ def load():
     fn("agdx_run", ci, vp, cf, ci, ci, vp)
     fn("agdx_run_warm", ci, vp, cf, ci, ci, vp)
     fn("agdx_warm_model", ci, vp, vp)
+    fn("agdx_set_spatial_precision", ci, vp, ci)
     fn("agdx_data_dev", vp, vp)
     fn("agdx_data_len", ctypes.c_int64, vp)
     fn("agdx_warm_model_dev", ci, vp, vp)
@@ -105,7 +106,7 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
-           "agdx_warm_model", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
+           "agdx_warm_model", "agdx_set_spatial_precision", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
            "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
@@ -260,6 +261,15 @@ class Problem:
         if self.L.agdx_run_warm(self.h, float(initial_factor), int(n_its), int(print_freq),
                                 yt.ptr):
             raise RuntimeError("agdx_run_warm failed")
+
+    def set_spatial_precision(self, mode):
+        """0: FP32 register FFTs; 1: split-precision bf16 x 3 matrix-core DFTs
+        (square spatial windows, P in {20, 40}). Returns the number of window
+        classes the bf16 x 3 path serves."""
+        n = int(self.L.agdx_set_spatial_precision(self.h, int(mode)))
+        if n < 0:
+            raise RuntimeError("agdx_set_spatial_precision failed")
+        return n
 
     def data_dev(self):
         """Device address of the patch data set by set_data (flat, patch

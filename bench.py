@@ -38,7 +38,8 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-FK_KERNELS = ["fwd_spatial", "fwd_time", "adj_time", "adj_spatial", "shrink"]
+FK_KERNELS = ["fwd_spatial", "fwd_time", "adj_time", "adj_spatial", "shrink",
+              "fwd_spatial_x3", "adj_spatial_x3"]
 OTHER_KERNELS = ["blend", "wavelet_fwd", "wavelet_adj", "norm2"]
 PROFILES = os.path.join(ROOT, "profiles")
 
@@ -237,6 +238,8 @@ def main():
     ap.add_argument("--warm-sweep", type=str, default="1,2,3,5,8,12,20,30")
     ap.add_argument("--warm-factor", type=float, default=0.03)
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--precision", type=int, default=0,
+                    help="spatial DFTs: 0 FP32 register FFTs, 1 bf16 x 3 matrix-core DFTs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-snr", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
@@ -270,6 +273,7 @@ def main():
     t0 = time.perf_counter()
     pm = L.Problem(**kw)
     t_setup = time.perf_counter() - t0
+    n_x3 = pm.set_spatial_precision(args.precision) if args.precision else 0
     pm.set_data(prob["blended"])
     n_windows = pm.n_windows
     by = pm.bytes()
@@ -360,7 +364,9 @@ def main():
                     target_db=args.target_db, setup_s=round(t_setup, 3),
                     end_to_end_s=None if e2e is None else round(e2e, 3),
                     sweep=[(n, round(t, 4), round(s, 2)) for n, t, s in sweep_res],
-                    parallelism="1 GPU", warm_start=warm),
+                    parallelism="1 GPU", warm_start=warm,
+                    spatial_dft="bf16x3 MFMA (%d window classes)" % n_x3 if args.precision
+                    else "fp32 register FFT"),
         roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
 

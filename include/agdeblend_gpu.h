@@ -96,6 +96,11 @@ int agdx_warm_model(AgdProblem *p, float const *const *y);
  * first update; lambda0 and the schedule are those of the zero start. */
 int agdx_run_warm(AgdProblem *p, float initial_factor, int n_its, int print_freq,
                   float const *const *warm);
+/* Spatial DFT precision: 0 FP32 register FFTs (default), 1 split-precision
+ * bf16 x 3 matrix-core DFTs (square spatial windows with P in {20, 40};
+ * other windows keep FP32). Returns the number of window classes served by
+ * the bf16 x 3 path, -1 on error. */
+int agdx_set_spatial_precision(AgdProblem *p, int mode);
 /* Device copy of the patch data set by agdx_set_data: all patches
  * concatenated in patch order, agdx_data_len floats (valid until
  * agdx_warm_model, agdx_finish or agdx_get_output overwrite it). */
