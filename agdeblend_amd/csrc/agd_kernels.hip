@@ -1479,33 +1479,24 @@ struct Shape {
 // Y = C Z with C[p][q] = e^{SIGN 2 pi i p q / P} for p < MC, q < KC (else 0).
 // Z: LDS [2][rows r16(KC)][PITCH] (real, imaginary parts), columns
 // r16(NZ); Y likewise with rows r16(MC). Waves own 16-row tiles of the
-// stacked output, whose constant operands stay in registers.
+// stacked output (tile wave + 4 t); their constant operands are built once
+// per block (Left::init) and stay in registers.
 template <int P, int MC, int KC, int NZ, int SIGN>
-__device__ __forceinline__ void left(float const *__restrict__ z, float *__restrict__ y) {
-  constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
-  constexpr int KMC = r16(MC), KKC = r16(KC), KNZ = r16(NZ);
-  constexpr int NMT = 2 * KMC / 16, NKS = 2 * KKC / 16, NNT = KNZ / 16;
-  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int const i = lane & 15, kq = lane >> 4;
-  for (int mt = wave; mt < NMT; mt += NTHR / 64) {
-    int const mr = mt * 16 + i;
-    int const po = mr / KMC, pr = mr - po * KMC;
-    bf16x4 ah[NKS], al[NKS];
+struct Left {
+  static constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
+  static constexpr int KMC = r16(MC), KKC = r16(KC), KNZ = r16(NZ);
+  static constexpr int NMT = 2 * KMC / 16, NKS = 2 * KKC / 16, NNT = KNZ / 16;
+  static constexpr int TPW = (NMT + NTHR / 64 - 1) / (NTHR / 64);  // tiles per wave
+  bf16x4 h[TPW][NKS], l[TPW][NKS];
+
+  __device__ __forceinline__ void init() {
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const i = lane & 15, kq = lane >> 4;
 #pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
-      float v[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int const kk = ks * 16 + 4 * kq + r;
-        int const pi = kk / KKC, q = kk - pi * KKC;
-        float2 const c = (pr < MC && q < KC) ? coef<P, SIGN>(pr * q) : make_float2(0.f, 0.f);
-        v[r] = part(c, po, pi);
-      }
-      split4(v, ah[ks], al[ks]);
-    }
-    for (int nt = 0; nt < NNT; ++nt) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      int const col = nt * 16 + i;
+    for (int t = 0; t < TPW; ++t) {
+      int const mt = wave + t * (NTHR / 64);
+      int const mr = mt * 16 + i;
+      int const po = mr / KMC, pr = mr - po * KMC;
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks) {
         float v[4];
@@ -1513,70 +1504,119 @@ __device__ __forceinline__ void left(float const *__restrict__ z, float *__restr
         for (int r = 0; r < 4; ++r) {
           int const kk = ks * 16 + 4 * kq + r;
           int const pi = kk / KKC, q = kk - pi * KKC;
-          v[r] = z[pi * PL + q * PITCH + col];
+          float2 const c = (mt < NMT && pr < MC && q < KC) ? coef<P, SIGN>(pr * q)
+                                                           : make_float2(0.f, 0.f);
+          v[r] = part(c, po, pi);
         }
-        bf16x4 bh, bl;
-        split4(v, bh, bl);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[ks], bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[ks], bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[ks], bh, acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int const yr = mt * 16 + 4 * kq + r;
-        int const yp = yr / KMC;
-        y[yp * PL + (yr - yp * KMC) * PITCH + col] = acc[r];
+        split4(v, h[t][ks], l[t][ks]);
       }
     }
   }
-}
+
+  __device__ __forceinline__ void apply(float const *__restrict__ z,
+                                        float *__restrict__ y) const {
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const i = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      int const mt = wave + t * (NTHR / 64);
+      if (mt >= NMT) break;
+      for (int nt = 0; nt < NNT; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        int const col = nt * 16 + i;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+          float v[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            int const kk = ks * 16 + 4 * kq + r;
+            int const pi = kk / KKC, q = kk - pi * KKC;
+            v[r] = z[pi * PL + q * PITCH + col];
+          }
+          bf16x4 bh, bl;
+          split4(v, bh, bl);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bh, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bl, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l[t][ks], bh, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int const yr = mt * 16 + 4 * kq + r;
+          int const yp = yr / KMC;
+          y[yp * PL + (yr - yp * KMC) * PITCH + col] = acc[r];
+        }
+      }
+    }
+  }
+};
 
 // Y = Z C^T with C[o][q] = e^{SIGN 2 pi i o q / P} for o < NC, q < KC (else
 // 0). Z: LDS rows r16(MZ), columns r16(KC); Y: rows r16(MZ), columns
-// r16(NC). Waves own 16-column tiles of the stacked output.
+// r16(NC). Waves own 16-column tiles of the stacked output, whose constant
+// operands are built once per block.
 template <int P, int MZ, int KC, int NC, int SIGN>
-__device__ __forceinline__ void right(float const *__restrict__ z, float *__restrict__ y) {
-  constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
-  constexpr int KMZ = r16(MZ), KKC = r16(KC), KNC = r16(NC);
-  constexpr int NNT = 2 * KNC / 16, NKS = 2 * KKC / 16, NMT = KMZ / 16;
-  int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int const i = lane & 15, kq = lane >> 4;
-  for (int nt = wave; nt < NNT; nt += NTHR / 64) {
-    int const nc = nt * 16 + i;
-    int const po = nc / KNC, o = nc - po * KNC;
-    bf16x4 bh[NKS], bl[NKS];
+struct Right {
+  static constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
+  static constexpr int KMZ = r16(MZ), KKC = r16(KC), KNC = r16(NC);
+  static constexpr int NNT = 2 * KNC / 16, NKS = 2 * KKC / 16, NMT = KMZ / 16;
+  static constexpr int TPW = (NNT + NTHR / 64 - 1) / (NTHR / 64);
+  bf16x4 h[TPW][NKS], l[TPW][NKS];
+
+  __device__ __forceinline__ void init() {
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const i = lane & 15, kq = lane >> 4;
 #pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
-      float v[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int const kk = ks * 16 + 4 * kq + r;
-        int const pi = kk / KKC, q = kk - pi * KKC;
-        float2 const c = (o < NC && q < KC) ? coef<P, SIGN>(o * q) : make_float2(0.f, 0.f);
-        v[r] = part(c, po, pi);
-      }
-      split4(v, bh[ks], bl[ks]);
-    }
-    for (int mt = 0; mt < NMT; ++mt) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      int const row = mt * 16 + i;
+    for (int t = 0; t < TPW; ++t) {
+      int const nt = wave + t * (NTHR / 64);
+      int const nc = nt * 16 + i;
+      int const po = nc / KNC, o = nc - po * KNC;
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks) {
-        int const kk = ks * 16 + 4 * kq;
-        int const pi = kk / KKC, q = kk - pi * KKC;
-        float4 const zz = *(float4 const *)(z + pi * PL + row * PITCH + q);
-        float const v[4] = {zz.x, zz.y, zz.z, zz.w};
-        bf16x4 ah, al;
-        split4(v, ah, al);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[ks], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[ks], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[ks], acc, 0, 0, 0);
-      }
+        float v[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) y[po * PL + (mt * 16 + 4 * kq + r) * PITCH + o] = acc[r];
+        for (int r = 0; r < 4; ++r) {
+          int const kk = ks * 16 + 4 * kq + r;
+          int const pi = kk / KKC, q = kk - pi * KKC;
+          float2 const c = (nt < NNT && o < NC && q < KC) ? coef<P, SIGN>(o * q)
+                                                          : make_float2(0.f, 0.f);
+          v[r] = part(c, po, pi);
+        }
+        split4(v, h[t][ks], l[t][ks]);
+      }
     }
   }
-}
+
+  __device__ __forceinline__ void apply(float const *__restrict__ z,
+                                        float *__restrict__ y) const {
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const i = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      int const nt = wave + t * (NTHR / 64);
+      if (nt >= NNT) break;
+      int const nc = nt * 16 + i;
+      int const po = nc / KNC, o = nc - po * KNC;
+      for (int mt = 0; mt < NMT; ++mt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        int const row = mt * 16 + i;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+          int const kk = ks * 16 + 4 * kq;
+          int const pi = kk / KKC, q = kk - pi * KKC;
+          float4 const zz = *(float4 const *)(z + pi * PL + row * PITCH + q);
+          float const v[4] = {zz.x, zz.y, zz.z, zz.w};
+          bf16x4 ah, al;
+          split4(v, ah, al);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, h[t][ks], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, l[t][ks], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, h[t][ks], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y[po * PL + (mt * 16 + 4 * kq + r) * PITCH + o] = acc[r];
+      }
+    }
+  }
+};
 }  // namespace x3
 
 // X -> U (the forward spatial stage), bf16 x 3 DFT matrices
@@ -1592,6 +1632,10 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_x3(
   AgdWin const &w = wins[t.win];
   float2 const *const src = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * (P * P);
   float2 *const Utw = U + w.u_off + (int64_t)t.tw * w.Ft * (W * W);
+  x3::Left<P, W, P, P, +1> a0;   // T = A0 X: rows n0, columns c1
+  x3::Right<P, W, P, W, +1> a1;  // U = T A1^T: rows n0, columns n1
+  a0.init();
+  a1.init();
   for (int k = 0; k < t.nk; ++k) {
     __syncthreads();
     for (int e = threadIdx.x; e < KP * KP; e += NTHR) {
@@ -1602,9 +1646,9 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_x3(
       za[PL + r * PITCH + c] = v.y;
     }
     __syncthreads();
-    x3::left<P, W, P, P, +1>(za, zb);   // T = A0 X: rows n0, columns c1
+    a0.apply(za, zb);
     __syncthreads();
-    x3::right<P, W, P, W, +1>(zb, za);  // U = T A1^T: rows n0, columns n1
+    a1.apply(zb, za);
     __syncthreads();
     for (int tr = threadIdx.x; tr < W * W; tr += NTHR) {
       int const n0 = tr / W, n1 = tr - n0 * W;
@@ -1630,6 +1674,10 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_x3(
   float2 const *const Vtw = V + w.u_off + (int64_t)t.tw * w.Ft * (W * W);
   float2 *const x = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * (P * P);
   float mx = 0.f;
+  x3::Right<P, W, W, P, -1> b1;  // R = V B1^T: rows n0, columns c1
+  x3::Left<P, P, W, P, -1> b0;   // G = B0 R: rows c0, columns c1
+  b1.init();
+  b0.init();
   for (int k = 0; k < t.nk; ++k) {
     __syncthreads();
     for (int e = threadIdx.x; e < KW * KW; e += NTHR) {
@@ -1643,9 +1691,9 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_x3(
       za[PL + n0 * PITCH + n1] = v.y;
     }
     __syncthreads();
-    x3::right<P, W, W, P, -1>(za, zb);  // R = V B1^T: rows n0, columns c1
+    b1.apply(za, zb);
     __syncthreads();
-    x3::left<P, P, W, P, -1>(zb, za);   // G = B0 R: rows c0, columns c1
+    b0.apply(zb, za);
     __syncthreads();
     float2 *const xk = x + (int64_t)k * P * P;
     for (int e = threadIdx.x; e < P * P; e += NTHR) {
