@@ -23,6 +23,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <time.h>
 #include <string.h>
 
 #include "agd_gpu.h"
@@ -672,6 +673,8 @@ struct AgdProblem {
   float *lamb_cur;  /* device threshold read by captured iterations */
   int sched_cap;    /* capacity of dev.lamb_sched (floats) */
   int u_valid;   /* U holds the forward spatial transform of X */
+  int *nk_req;   /* per spatial class: planes per task (0: the default rule) */
+  size_t u_len;  /* complex elements of U and of V */
 };
 
 static void *dev_alloc(AgdProblem *p, size_t bytes) {
@@ -708,6 +711,7 @@ EXPORT void agdx_destroy(AgdProblem *p) {
   free(p->wpatch_trace);
   free(p->dcoord);
   for (i = 0; i < p->n_sclass; i++) free((void *)p->sclass[i].wpos);
+  free(p->nk_req);
   for (i = 0; i < p->n_tclass; i++) free((void *)p->tclass[i].wpos);
   free(p->sclass);
   free(p->tclass);
@@ -1063,6 +1067,72 @@ static void time_key(AgdWin const *W, int *m1, int *m2) {
 /* Spatial tasks (chunks of time-frequency planes) and time blocks (runs of
  * traces), each grouped into classes of windows with equal transform
  * sizes; a class is served by one kernel launch. */
+/* planes per spatial task of a specialised class of padded sizes (a, b):
+ * one column transform per thread of a 256-thread block (b per plane),
+ * within a 48 KB LDS tile of spec_len(a) rows of b + 1 per plane;
+ * AGD_SPATIAL_NK overrides */
+static int default_nk(int a, int b) {
+  size_t const plane = (size_t)spec_len(a) * (b + 1) * 8;
+  char const *e = getenv("AGD_SPATIAL_NK");
+  int nk = e ? atoi(e) : 256 / b;
+  if (nk < 1) nk = 1;
+  if ((size_t)nk * plane > 48 * 1024) nk = (int)(48 * 1024 / plane);
+  return nk;
+}
+
+/* Spatial tasks of the classes cls[0, ncls): per window of the class, per
+ * time window, runs of nk time frequencies (nk_req[c] when given and > 0,
+ * else the default rule); fills each class's begin, count, wpos and LDS
+ * size, and the generic kernels' LDS size in *lds_sp. */
+static int spatial_tasks(AgdProblem *p, AgdClass *cls, int ncls, int const *nk_req,
+                         vec *tasks, size_t *lds_sp) {
+  int c, w;
+  for (c = 0; c < ncls; c++) {
+    AgdClass *cl = cls + c;
+    int *wpos = (int *)malloc(((size_t)p->n_win + 1) * sizeof(int));
+    if (wpos == NULL) return 1;
+    cl->wpos = wpos;
+    cl->lds = 0;
+    cl->begin = (int)tasks->n;
+    for (w = 0; w < p->n_win; w++) {
+      AgdWin const *W = &p->win[w];
+      int a, b, nk, tw, k0;
+      wpos[w] = (int)tasks->n - cl->begin;
+      spatial_key(W, &a, &b);
+      if (a != cl->p0 || b != cl->p1 || (a && (W->ug != W->ntr) != cl->tb)) continue;
+      if (a) {
+        nk = (nk_req && nk_req[c] > 0) ? nk_req[c] : default_nk(a, b);
+      } else {
+        nk = (int)(LDS_BUDGET / (2 * (size_t)W->S * 8));
+      }
+      if (nk < 1) {
+        fprintf(stderr, "ERROR: window of %d padded traces exceeds the LDS budget\n", W->S);
+        return 1;
+      }
+      if (nk > W->Ft) nk = W->Ft;
+      for (tw = 0; tw < W->ntw; tw++) {
+        for (k0 = 0; k0 < W->Ft; k0 += nk) {
+          AgdTask *tk = (AgdTask *)vec_push(tasks);
+          if (tk == NULL) return 1;
+          tk->win = w;
+          tk->tw = tw;
+          tk->k0 = k0;
+          tk->nk = (k0 + nk <= W->Ft) ? nk : W->Ft - k0;
+        }
+      }
+      if (a) {
+        size_t const l = (size_t)nk * spec_len(a) * (b + 1) * 8;
+        if (l > cl->lds) cl->lds = l;
+      } else if ((size_t)2 * nk * W->S * 8 > *lds_sp) {
+        *lds_sp = (size_t)2 * nk * W->S * 8;
+      }
+    }
+    cl->count = (int)tasks->n - cl->begin;
+    wpos[p->n_win] = cl->count;
+  }
+  return 0;
+}
+
 static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp_out,
                          size_t *lds_t_out) {
   int w, c;
@@ -1099,55 +1169,7 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
       n->p1 = b;
     }
   }
-  for (c = 0; c < (int)sc.n; c++) {
-    AgdClass *cl = (AgdClass *)sc.p + c;
-    int *wpos = (int *)malloc(((size_t)p->n_win + 1) * sizeof(int));
-    if (wpos == NULL) return 1;
-    cl->wpos = wpos;
-    cl->begin = (int)tasks->n;
-    for (w = 0; w < p->n_win; w++) {
-      AgdWin const *W = &p->win[w];
-      int a, b, nk, tw, k0;
-      wpos[w] = (int)tasks->n - cl->begin;
-      spatial_key(W, &a, &b);
-      if (a != cl->p0 || b != cl->p1 || (a && (W->ug != W->ntr) != cl->tb)) continue;
-      if (a) {
-        /* one column transform per thread of a 256-thread block (P1 per
-         * plane), within a 48 KB LDS tile of W0 rows of P1 + 1 per plane;
-         * AGD_SPATIAL_NK overrides */
-        size_t const plane = (size_t)spec_len(a) * (b + 1) * 8;
-        char const *e = getenv("AGD_SPATIAL_NK");
-        nk = e ? atoi(e) : 256 / b;
-        if (nk < 1) nk = 1;
-        if ((size_t)nk * plane > 48 * 1024) nk = (int)(48 * 1024 / plane);
-      } else {
-        nk = (int)(LDS_BUDGET / (2 * (size_t)W->S * 8));
-      }
-      if (nk < 1) {
-        fprintf(stderr, "ERROR: window of %d padded traces exceeds the LDS budget\n", W->S);
-        return 1;
-      }
-      if (nk > W->Ft) nk = W->Ft;
-      for (tw = 0; tw < W->ntw; tw++) {
-        for (k0 = 0; k0 < W->Ft; k0 += nk) {
-          AgdTask *tk = (AgdTask *)vec_push(tasks);
-          if (tk == NULL) return 1;
-          tk->win = w;
-          tk->tw = tw;
-          tk->k0 = k0;
-          tk->nk = (k0 + nk <= W->Ft) ? nk : W->Ft - k0;
-        }
-      }
-      if (a) {
-        size_t const l = (size_t)nk * spec_len(a) * (b + 1) * 8;
-        if (l > cl->lds) cl->lds = l;
-      } else if ((size_t)2 * nk * W->S * 8 > lds_sp) {
-        lds_sp = (size_t)2 * nk * W->S * 8;
-      }
-    }
-    cl->count = (int)tasks->n - cl->begin;
-    wpos[p->n_win] = cl->count;
-  }
+  if (spatial_tasks(p, (AgdClass *)sc.p, (int)sc.n, NULL, tasks, &lds_sp)) return 1;
   for (c = 0; c < (int)tc.n; c++) {
     AgdClass *cl = (AgdClass *)tc.p + c;
     int tb;
@@ -1544,6 +1566,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->dev.X = (float *)dev_alloc(p, (size_t)xo * 8);
     p->dev.U = (float *)dev_alloc(p, (size_t)uo * 8);
     p->dev.V = (float *)dev_alloc(p, (size_t)uo * 8);
+    p->u_len = (size_t)uo;
     p->dev.T = (float *)dev_alloc(p, (size_t)to * 4);
     p->dev.T2 = p->use_wavelets ? (float *)dev_alloc(p, (size_t)to * 4) : p->dev.T;
     p->dev.d = (float *)dev_alloc(p, (size_t)p->cfg.rec_len * 4 + 16);
@@ -2184,6 +2207,119 @@ EXPORT int agdx_set_spatial_precision(AgdProblem *p, int mode) {
   p->u_valid = 0;
   return mode ? agdg_x3_classes(&p->dev) : 0;
 }
+
+/* Rebuild the spatial tasks with the planes per task in p->nk_req and
+ * replace the device task list. */
+static int rebuild_spatial(AgdProblem *p) {
+  vec tasks;
+  size_t lds_sp = 0;
+  int c, i;
+  void *q, *old = (void *)p->dev.tasks;
+  vec_init(&tasks, sizeof(AgdTask));
+  for (c = 0; c < p->n_sclass; c++) {
+    free((void *)p->sclass[c].wpos);
+    p->sclass[c].wpos = NULL;
+  }
+  if (spatial_tasks(p, p->sclass, p->n_sclass, p->nk_req, &tasks, &lds_sp) ||
+      lds_sp > LDS_MAX) {
+    vec_free(&tasks);
+    return 1;
+  }
+  q = agdg_alloc(tasks.n ? tasks.n * sizeof(AgdTask) : 16);
+  if (q == NULL || (tasks.n && agdg_h2d(q, tasks.p, tasks.n * sizeof(AgdTask)))) {
+    agdg_free(q);
+    vec_free(&tasks);
+    return 1;
+  }
+  for (i = 0; i < p->n_dalloc; i++)
+    if (p->dalloc[i] == old) p->dalloc[i] = q;
+  agdg_free(old);
+  p->dev.tasks = (AgdTask const *)q;
+  p->dev.n_tasks = (int)tasks.n;
+  p->dev.lds_spatial = lds_sp;
+  p->u_valid = 0;
+  vec_free(&tasks);
+  return 0;
+}
+
+/* wall time of reps launches of one iteration's spatial stage (the fused
+ * adjoint with alpha = 0 and threshold 0 on a zero V, which leaves X as it
+ * is, or the adjoint and forward spatial kernels) */
+static double time_spatial(AgdProblem *p, int reps) {
+  struct timespec t0, t1;
+  float *lamb_dev = p->dev.lamb_dev;
+  int i, rc = 0;
+  p->dev.lamb_dev = NULL;
+  for (i = -1; i < reps && rc == 0; i++) {
+    if (i == 0) {
+      if (agdg_sync()) rc = 1;
+      clock_gettime(CLOCK_MONOTONIC, &t0);
+    }
+    if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, p->fuse ? 0 : 2, p->fuse)) rc = 1;
+    if (!p->fuse && agdg_fwd_spatial(&p->dev, p->dev.X)) rc = 1;
+  }
+  if (agdg_sync()) rc = 1;
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  p->dev.lamb_dev = lamb_dev;
+  if (rc) return -1.0;
+  return ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) / reps;
+}
+
+/* Autotuner of the spatial kernels' tile: for each specialised spatial
+ * class in turn (the others at their current choice), time one
+ * iteration's spatial stage for every candidate planes-per-task count that
+ * fits the LDS, and keep the fastest. Clears V, leaves the tasks set to the
+ * choice; the model X is unchanged. nk_out (n_sclass ints, or NULL)
+ * receives the choice per class (0 for generic classes); ms_out (same) the
+ * time of the chosen configuration. Returns 0, or 1 on error. */
+EXPORT int agdx_autotune(AgdProblem *p, int reps, int *nk_out, double *ms_out) {
+  static int const cand[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
+  int c, k;
+  if (p->host_only) return 1;
+  if (reps < 1) reps = 3;
+  if (p->nk_req == NULL) {
+    p->nk_req = (int *)calloc((size_t)(p->n_sclass > 0 ? p->n_sclass : 1), sizeof(int));
+    if (p->nk_req == NULL) return 1;
+  }
+  if (agdg_zero(p->dev.V, p->u_len * 8)) return 1;
+  for (c = 0; c < p->n_sclass; c++) {
+    AgdClass const *cl = &p->sclass[c];
+    int best = 0, maxft = 1, w;
+    double best_t = 0.0;
+    if (cl->p0 == 0) {
+      if (nk_out) nk_out[c] = 0;
+      continue;
+    }
+    for (w = 0; w < p->n_win; w++) {
+      int a, b;
+      spatial_key(&p->win[w], &a, &b);
+      if (a == cl->p0 && b == cl->p1 && p->win[w].Ft > maxft) maxft = p->win[w].Ft;
+    }
+    for (k = 0; k < (int)(sizeof(cand) / sizeof(cand[0])); k++) {
+      size_t const plane = (size_t)spec_len(cl->p0) * (cl->p1 + 1) * 8;
+      double t;
+      if ((size_t)cand[k] * plane > 96 * 1024 || cand[k] > maxft) continue;
+      p->nk_req[c] = cand[k];
+      if (rebuild_spatial(p)) return 1;
+      t = time_spatial(p, reps);
+      if (t < 0.0) return 1;
+      if (getenv("AGD_TUNE_VERBOSE"))
+        fprintf(stderr, "autotune: spatial class %dx%d nk %2d: %.4f ms\n", cl->p0, cl->p1,
+                cand[k], t * 1e3);
+      if (best == 0 || t < best_t) {
+        best = cand[k];
+        best_t = t;
+      }
+    }
+    p->nk_req[c] = best;
+    if (rebuild_spatial(p)) return 1;
+    if (nk_out) nk_out[c] = best;
+    if (ms_out) ms_out[c] = best_t * 1e3;
+  }
+  return 0;
+}
+
+EXPORT int agdx_n_spatial_classes(AgdProblem const *p) { return p->n_sclass; }
 
 EXPORT int agdx_iterate(AgdProblem *p, int n, float lamb) {
   float *sched;

@@ -64,6 +64,8 @@ def load():
     fn("agdx_run_warm", ci, vp, cf, ci, ci, vp)
     fn("agdx_warm_model", ci, vp, vp)
     fn("agdx_set_spatial_precision", ci, vp, ci)
+    fn("agdx_autotune", ci, vp, ci, vp, vp)
+    fn("agdx_n_spatial_classes", ci, vp)
     fn("agdx_data_dev", vp, vp)
     fn("agdx_data_len", ctypes.c_int64, vp)
     fn("agdx_warm_model_dev", ci, vp, vp)
@@ -106,7 +108,8 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
-           "agdx_warm_model", "agdx_set_spatial_precision", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
+           "agdx_warm_model", "agdx_set_spatial_precision", "agdx_autotune",
+           "agdx_n_spatial_classes", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
            "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
@@ -270,6 +273,16 @@ class Problem:
         if n < 0:
             raise RuntimeError("agdx_set_spatial_precision failed")
         return n
+
+    def autotune(self, reps=3):
+        """Pick the spatial kernels' planes per task by timing; returns the
+        choice per spatial class (0: generic class) and the times (ms)."""
+        n = int(self.L.agdx_n_spatial_classes(self.h))
+        nk = np.zeros(max(n, 1), C_INT)
+        ms = np.zeros(max(n, 1), np.float64)
+        if self.L.agdx_autotune(self.h, int(reps), _p(nk), _p(ms)):
+            raise RuntimeError("agdx_autotune failed")
+        return nk[:n].tolist(), ms[:n].tolist()
 
     def data_dev(self):
         """Device address of the patch data set by set_data (flat, patch

@@ -238,6 +238,8 @@ def main():
     ap.add_argument("--warm-sweep", type=str, default="1,2,3,5,8,12,20,30")
     ap.add_argument("--warm-factor", type=float, default=0.03)
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default spatial tile instead of the autotuner's choice")
     ap.add_argument("--precision", type=int, default=0,
                     help="spatial DFTs: 0 FP32 register FFTs, 1 bf16 x 3 matrix-core DFTs")
     ap.add_argument("--no-cpu", action="store_true")
@@ -275,6 +277,14 @@ def main():
     t_setup = time.perf_counter() - t0
     n_x3 = pm.set_spatial_precision(args.precision) if args.precision else 0
     pm.set_data(prob["blended"])
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        nk, ms = pm.autotune(reps=3)
+        tune = dict(planes_per_task=nk, spatial_ms=[round(t, 4) for t in ms],
+                    tuning_s=round(time.perf_counter() - t0, 3))
+        log("autotune: planes per spatial task %s (%s ms), %.2f s" %
+            (nk, ", ".join("%.3f" % t for t in ms), tune["tuning_s"]))
     n_windows = pm.n_windows
     by = pm.bytes()
     b_fk = 3 * by["M"] + 2 * by["T"]
@@ -364,7 +374,7 @@ def main():
                     target_db=args.target_db, setup_s=round(t_setup, 3),
                     end_to_end_s=None if e2e is None else round(e2e, 3),
                     sweep=[(n, round(t, 4), round(s, 2)) for n, t, s in sweep_res],
-                    parallelism="1 GPU", warm_start=warm,
+                    parallelism="1 GPU", warm_start=warm, autotune=tune,
                     spatial_dft="bf16x3 MFMA (%d window classes)" % n_x3 if args.precision
                     else "fp32 register FFT"),
         roofline=roofline, cpu_baseline=cpu)

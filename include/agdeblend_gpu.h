@@ -101,6 +101,15 @@ int agdx_run_warm(AgdProblem *p, float initial_factor, int n_its, int print_freq
  * other windows keep FP32). Returns the number of window classes served by
  * the bf16 x 3 path, -1 on error. */
 int agdx_set_spatial_precision(AgdProblem *p, int mode);
+/* Autotuner of the spatial kernels' tile (planes per task): per
+ * specialised spatial class, every candidate that fits the LDS is timed on
+ * one iteration's spatial stage (reps launches) and the fastest kept.
+ * Clears the adjoint intermediate; the model is unchanged. nk_out and
+ * ms_out (one entry per spatial class, or NULL) receive the choice (0 for
+ * generic classes) and its time per iteration. */
+int agdx_autotune(AgdProblem *p, int reps, int *nk_out, double *ms_out);
+/* Number of spatial kernel classes of the problem. */
+int agdx_n_spatial_classes(AgdProblem const *p);
 /* Device copy of the patch data set by agdx_set_data: all patches
  * concatenated in patch order, agdx_data_len floats (valid until
  * agdx_warm_model, agdx_finish or agdx_get_output overwrite it). */
