@@ -2124,7 +2124,11 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
     fprintf(stderr, "ERROR: no data set\n");
     return 1;
   }
-  if (x0 == NULL) {
+  if (x0 == NULL && (warm != NULL || warm_dev != NULL) && n_its <= 1) {
+    /* a warm start with no iteration after it uses no threshold: skip the
+     * first update that only serves to compute lambda0 */
+    lamb = 0.0f;
+  } else if (x0 == NULL) {
     unsigned int bits = 0;
     float l2;
     /* x = 0 - alpha * A^H d; lambda0 = max |x| */

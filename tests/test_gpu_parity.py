@@ -308,14 +308,16 @@ def test_warm_model_output_is_the_data(name):
 
 
 @pytest.mark.parametrize("cfg", [0, 1])
-def test_warm_deblend_matches_fp64_oracle(cfg):
+@pytest.mark.parametrize("n_its", [1, 8])
+def test_warm_deblend_matches_fp64_oracle(cfg, n_its):
+    """n_its = 1: the warm-start model's output, with no threshold needed
+    (the library skips the first update that computes lambda0)."""
     prob = synth.SMALL[cfg]()
     g = prob["geometry"]
     kw = _kw(g)
     blended = synth.blend_input(prob, O.blend_same_geometry)
     # a data estimate between the blended input and the truth
     warm = [0.5 * (np.asarray(a, np.float64) + b) for a, b in zip(blended, prob["truth"])]
-    n_its = 8
     ref, _ = O.deblend(data=blended, initial_factor=1.0, n_its=n_its, double=True, warm=warm,
                        **kw)
     pm = L.Problem(**kw)
