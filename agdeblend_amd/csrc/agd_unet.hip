@@ -363,6 +363,110 @@ __global__ __launch_bounds__(UT) void k_conv3(
   }
 }
 
+// ---------------------------------------------------------------------------
+// First layer: one live input channel (the field), so the taps are the GEMM's
+// K: D[voxel][co] = sum_tap bf16(field[voxel + tap] * scale) w[tap][co][0],
+// K = 27 taps padded to 32 (two K = 16 steps) instead of 27 steps of 16
+// channels. Block: CT_H rows x CT_W voxels of FDB consecutive d planes, a
+// ring of three bf16 planes of the field in LDS; output as in k_conv3
+// (bias, ReLU, 16-byte stores through per-wave LDS rows).
+// grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / FDB)); block 256.
+// ---------------------------------------------------------------------------
+#define FDB 4
+__global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src,
+                                                   uint16_t const *__restrict__ wt, int wci,
+                                                   float const *__restrict__ bias,
+                                                   uint16_t *__restrict__ y, int D, int H,
+                                                   int W, float scale) {
+  constexpr int PW = CT_W + 2, PH = CT_H + 2, PLANE = PH * PW;
+  constexpr int OBS = 20;
+  __shared__ __align__(16) uint16_t sx[3 * PLANE];
+  __shared__ __align__(16) uint16_t ob[4 * 64 * OBS];
+  int const tid = threadIdx.x;
+  int const lane = tid & 63, wave = tid >> 6;
+  int const i = lane & 15, kq = lane >> 4;
+  int const ndb = (D + FDB - 1) / FDB;
+  int const n = blockIdx.z / ndb;
+  int const d0 = (blockIdx.z - n * ndb) * FDB;
+  int const h0 = blockIdx.y * CT_H, w0 = blockIdx.x * CT_W;
+  float const *const fn = src + (int64_t)n * D * H * W;
+  // B: taps 16 ks + 4 kq + r of output channel i (zero past tap 26)
+  bf16x4 bw[2];
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int const t = 16 * ks + 4 * kq + r;
+      bw[ks][r] = t < 27 ? (short)wt[((int64_t)t * 16 + i) * wci] : (short)0;
+    }
+  // LDS offsets of this lane's A taps relative to its voxel (slot, row, col)
+  int toff[2][4], tkd[2][4];
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int const t = 16 * ks + 4 * kq + r;
+      int const tt = t < 27 ? t : 13;  // padded taps read the centre (weight 0)
+      tkd[ks][r] = tt / 9;
+      toff[ks][r] = ((tt / 3) % 3) * PW + tt % 3;
+    }
+  auto stage = [&](int dd) {
+    uint16_t *const dst = sx + ((dd + 3) % 3) * PLANE;
+    bool const okd = dd >= 0 && dd < D;
+    for (int e = tid; e < PLANE; e += UT) {
+      int const ph = e / PW, pw = e - ph * PW;
+      int const hh = h0 + ph - 1, ww = w0 + pw - 1;
+      bool const ok = okd && hh >= 0 && hh < H && ww >= 0 && ww < W;
+      float const v = fn[ok ? ((int64_t)dd * H + hh) * W + ww : 0];
+      dst[e] = ok ? f2bf(v * scale) : (uint16_t)0;
+    }
+  };
+  stage(d0 - 1);
+  stage(d0);
+  float const bsel = bias[i];
+  int const h = h0 + wave;
+  uint16_t *const obw = ob + wave * 64 * OBS;
+  for (int j = 0; j < FDB; ++j) {
+    int const d = d0 + j;
+    stage(d + 1);
+    __syncthreads();
+    f32x4 acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+      int const vb = wave * PW + 16 * m + i;  // the voxel's (row, col) in a plane
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        bf16x4 a;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          a[r] = (short)sx[((d + tkd[ks][r] - 1 + 3) % 3) * PLANE + vb + toff[ks][r]];
+        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[ks], acc[m], 0, 0, 0);
+      }
+    }
+    // D: lane holds voxels 16 m + 4 kq + r of channel i
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        obw[(16 * m + 4 * kq + r) * OBS + i] = f2bf(fmaxf(acc[m][r] + bsel, 0.f));
+    __syncthreads();  // the output rows are complete; the ring slot of d - 1 is free
+    if (h < H && d < D) {
+      int64_t const orow = (((int64_t)n * D + d) * H + h) * W;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        int const e = lane + 64 * k;  // (voxel, 8-channel half)
+        int const v = e >> 1, c8 = e & 1;
+        if (w0 + v < W) {
+          uint2 const lo = *(uint2 const *)(obw + v * OBS + c8 * 8);
+          uint2 const hi = *(uint2 const *)(obw + v * OBS + c8 * 8 + 4);
+          *(uint4 *)(y + (orow + w0 + v) * 16 + c8 * 8) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
+      }
+    }
+  }
+}
+
 // wt_t[tap][ci][co] = wt[26 - tap][co][ci]: the weights of the input
 // gradient (a convolution of dy with the flipped kernel, CO -> CI channels)
 __global__ void k_flip_weights(uint16_t const *__restrict__ wt, uint16_t *__restrict__ wtt,
@@ -891,11 +995,18 @@ int agdu_conv_in(float const *src, void const *wt, int wci, float const *bias, v
   void const *const gate = NULL;
   int const mode = 3;
   if (!conv_shape_ok(N, D, H, W, 16, 16)) return 1;
-  if (wci != 16) {
-    fprintf(stderr, "ERROR: first-layer weights must have 16 input channels, got %d\n", wci);
+  (void)x;
+  (void)xu;
+  (void)gate;
+  (void)mode;
+  if (wci < 1) {
+    fprintf(stderr, "ERROR: first-layer weights must have input channels, got %d\n", wci);
     return 1;
   }
-  CONV_LAUNCH_IO(16, 16, 0, 1, scale, (float const *)0, (float *)0)
+  dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H),
+                  (unsigned)(N * ((D + FDB - 1) / FDB)));
+  return ulaunch("conv_first", k_conv_first, grid, dim3(UT), stream, src,
+                 (uint16_t const *)wt, wci, bias, (uint16_t *)y, D, H, W, scale);
 }
 
 int agdu_conv_out(void const *x, void const *wt, int wco, float const *bias, float const *base,

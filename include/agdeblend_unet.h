@@ -32,8 +32,7 @@ int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *b
                    int CO, int mode, void *stream);
 /* First layer from a one-channel fp32 field:
  * y[v][co] = ReLU(bias[co] + sum_tap wt[tap][co][0] bf16(src[v + tap] scale)),
- * 16 output channels, wt [27][16][wci] with wci = 16 (the packed input's
- * channels 1..15 are zero) */
+ * 16 output channels, wt [27][16][wci] (input channel 0 is used) */
 int agdu_conv_in(float const *src, void const *wt, int wci, float const *bias, void *y, int N,
                  int D, int H, int W, float scale, void *stream);
 /* Output channel 0 of a 16 -> 16 convolution, in fp32:
