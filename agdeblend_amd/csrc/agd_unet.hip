@@ -97,19 +97,14 @@ struct ConvDepth {
 // CU > 0: the input is the concatenation [upsample(xu), x] without being
 // formed: input channels [0, CU) are xu's ([n][D/2][H/2][W/2][CU], nearest
 // x2 upsampling, D, H, W even) and [CU, CI) are x's ([n][D][H][W][CI - CU]).
-// IO = 1: the input is a one-channel fp32 field (x points to floats): input
-// channel 0 is bf16(field * fscale), channels 1..15 are zero (CI = 16).
-// IO = 2: the output is output channel 0 in fp32, dst[v] = base[v] +
-// fscale * (conv + bias[0]) (base may be NULL; W a multiple of 4).
-template <int CI, int CO, int CU, int IO>
+template <int CI, int CO, int CU>
 __global__ __launch_bounds__(UT) void k_conv3(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ xu,
     uint16_t const *__restrict__ wt, float const *__restrict__ bias,
     uint16_t const *__restrict__ gate, uint16_t *__restrict__ y, int D, int H, int W,
-    int mode, float fscale, float const *__restrict__ base, float *__restrict__ dst) {
+    int mode) {
   static_assert(CI % 16 == 0 && CO % 16 == 0 && CU % 16 == 0 && CU < CI,
                 "channels in multiples of 16");
-  static_assert(IO != 1 || (CI == 16 && CU == 0), "field input has one chunk");
   constexpr int CS = CI - CU;  // channels of x
   constexpr int NT = CO / 16;
   constexpr int DB = ConvDepth<CO, CU>::DB;
@@ -140,10 +135,8 @@ __global__ __launch_bounds__(UT) void k_conv3(
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[j][m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  constexpr int XS = IO == 1 ? 1 : CS;  // x elements per voxel
-  uint16_t const *const xn = x + (int64_t)n * D * H * W * (IO == 1 ? 2 : CS);
-  float const *const fn = (float const *)xn;
-  int64_t const plane = (int64_t)H * W * XS;
+  uint16_t const *const xn = x + (int64_t)n * D * H * W * CS;
+  int64_t const plane = (int64_t)H * W * CS;
   int const Hu = H / 2, Wu = W / 2;
   uint16_t const *const xun = CU ? xu + (int64_t)n * (D / 2) * Hu * Wu * CU : nullptr;
   int64_t const planeu = (int64_t)Hu * Wu * CU;
@@ -163,27 +156,13 @@ __global__ __launch_bounds__(UT) void k_conv3(
     int const pw = r % PW, ph = r / PW;
     int const hh = h0 + ph - 1, ww = w0 + pw - 1;
     iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
-    ioff[k] = IO == 1 ? (iok[k] ? (uint32_t)(hh * W + ww) : 0u)
-                      : (iok[k] ? (uint32_t)((hh * W + ww) * CS + half * 8) : (uint32_t)(half * 8));
+    ioff[k] = iok[k] ? (uint32_t)((hh * W + ww) * CS + half * 8) : (uint32_t)(half * 8);
     iofu[k] = iok[k] ? (uint32_t)(((hh >> 1) * Wu + (ww >> 1)) * CU + half * 8)
                      : (uint32_t)(half * 8);
   }
   // global loads of plane dd of chunk c (zeros outside the volume) ...
   auto load_plane = [&](int dd, int c, uint4 (&v)[NPK]) {
     bool const okd = dd >= 0 && dd < D;
-    if (IO == 1) {
-      // channel 0 from the field; the item's other channels are zero
-      float const *const fsrc = fn + (okd ? dd : 0) * plane;
-#pragma unroll
-      for (int k = 0; k < NPK; ++k) {
-        int const e0 = tid + k * UT;
-        bool const first = ((e0 < NPI ? e0 : NPI - 1) & 1) == 0;
-        uint32_t const msk = (okd && iok[k] && first) ? 0xffffu : 0u;
-        float const f = fsrc[ioff[k]];
-        v[k] = make_uint4((uint32_t)f2bf(f * fscale) & msk, 0u, 0u, 0u);
-      }
-      return;
-    }
     bool const up = CU > 0 && c < CU / 16;  // the same for the whole block
     uint16_t const *const src = up ? xun + (okd ? dd >> 1 : 0) * planeu + c * 16
                                    : xn + (okd ? dd : 0) * plane + (c - CU / 16) * 16;
@@ -284,34 +263,6 @@ __global__ __launch_bounds__(UT) void k_conv3(
   // channel t * 16 + i; the row padding puts the four kq groups in distinct
   // banks), then stores 16-byte pieces of 8 channels, coalesced.
   int const h = h0 + wave;
-  if (IO == 2) {
-    // lanes of channel 0 (i == 0) hold 4 consecutive voxels of each tile
-    if (i == 0 && h < H) {
-      float const b0 = bias[0];
-#pragma unroll
-      for (int j = 0; j < DB; ++j) {
-        int const d = d0 + j;
-        if (d >= D) break;
-        int64_t const orow = (((int64_t)n * D + d) * H + h) * W;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          int const wv = w0 + 16 * m + 4 * kq;
-          if (wv >= W) continue;
-          float4 o = make_float4((acc[j][m][0][0] + b0) * fscale, (acc[j][m][0][1] + b0) * fscale,
-                                 (acc[j][m][0][2] + b0) * fscale, (acc[j][m][0][3] + b0) * fscale);
-          if (base) {
-            float4 const bb = *(float4 const *)(base + orow + wv);
-            o.x += bb.x;
-            o.y += bb.y;
-            o.z += bb.z;
-            o.w += bb.w;
-          }
-          *(float4 *)(dst + orow + wv) = o;
-        }
-      }
-    }
-    return;
-  }
   uint16_t *const ob = smem + wave * 64 * OBS;
 #pragma unroll
   for (int j = 0; j < DB; ++j) {
@@ -463,6 +414,132 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
           *(uint4 *)(y + (orow + w0 + v) * 16 + c8 * 8) = make_uint4(lo.x, lo.y, hi.x, hi.y);
         }
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Head: output channel 0 of a 16 -> 16 convolution, in fp32, plus the field:
+// out[v] = base[v] + s (b0 + sum_tap sum_ci x[v + tap][ci] w[tap][0][ci]).
+// For an input plane p and depth offset kd, E[u][(kh, kw)] =
+// sum_ci x_p[u][ci] w[(kd, kh, kw)][0][ci] over every staged voxel u (halo
+// included) is a GEMM with M = voxels, K = 16 channels, N = 9 taps (padded
+// to 16): one MFMA per 16 voxels. Then out[v] gathers E[v + (kh - 1,
+// kw - 1)][(kh, kw)] over the three planes, each with its own kd. Block:
+// CT_H rows x CT_W voxels of HDB consecutive d planes, the input planes in
+// a ring of three (swizzled rows, as in k_conv3), one E buffer reused per kd.
+// grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / HDB)); block 256.
+// ---------------------------------------------------------------------------
+#define HDB 4
+__global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x,
+                                                  uint16_t const *__restrict__ wt, int wco,
+                                                  float const *__restrict__ bias,
+                                                  float const *__restrict__ base,
+                                                  float *__restrict__ dst, int D, int H, int W,
+                                                  float unscale) {
+  constexpr int PW = CT_W + 2, PH = CT_H + 2;
+  constexpr int NV = PH * PW;           // staged voxels per plane (396)
+  constexpr int NVT = (NV + 15) / 16;   // 16-voxel tiles (25)
+  constexpr int PLANE = NV * 16;        // bf16 elements per staged plane
+  constexpr int NPI = NV * 2;           // staging items of 8 channels
+  constexpr int NPK = (NPI + UT - 1) / UT;
+  constexpr int EP = 9;                 // floats per E row (9 taps)
+  __shared__ __align__(16) uint16_t sx[3 * PLANE];
+  __shared__ __align__(16) float se[NVT * 16 * EP];
+  int const tid = threadIdx.x;
+  int const lane = tid & 63, wave = tid >> 6;
+  int const i = lane & 15, kq = lane >> 4;
+  int const ndb = (D + HDB - 1) / HDB;
+  int const n = blockIdx.z / ndb;
+  int const d0 = (blockIdx.z - n * ndb) * HDB;
+  int const h0 = blockIdx.y * CT_H, w0 = blockIdx.x * CT_W;
+  uint16_t const *const xn = x + (int64_t)n * D * H * W * 16;
+  int64_t const plane = (int64_t)H * W * 16;
+  // B per kd: taps (kd, t9 = i) of output channel 0, input channels 4 kq..
+  bf16x4 bw[3];
+#pragma unroll
+  for (int kd = 0; kd < 3; ++kd)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      bw[kd][r] = i < 9 ? (short)wt[((int64_t)(kd * 9 + i) * wco) * 16 + 4 * kq + r] : (short)0;
+  uint32_t ioff[NPK];
+  bool iok[NPK];
+#pragma unroll
+  for (int k = 0; k < NPK; ++k) {
+    int const e0 = tid + k * UT;
+    int const e = e0 < NPI ? e0 : NPI - 1;
+    int const half = e & 1, r = e >> 1;
+    int const pw = r % PW, ph = r / PW;
+    int const hh = h0 + ph - 1, ww = w0 + pw - 1;
+    iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
+    ioff[k] = iok[k] ? (uint32_t)((hh * W + ww) * 16 + half * 8) : (uint32_t)(half * 8);
+  }
+  auto load_plane = [&](int dd, uint4 (&v)[NPK]) {
+    bool const okd = dd >= 0 && dd < D;
+    uint16_t const *const src = xn + (okd ? dd : 0) * plane;
+#pragma unroll
+    for (int k = 0; k < NPK; ++k) {
+      uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;
+      uint4 const t = *(uint4 const *)(src + ioff[k]);
+      v[k] = make_uint4(t.x & msk, t.y & msk, t.z & msk, t.w & msk);
+    }
+  };
+  auto store_plane = [&](int dd, uint4 const (&v)[NPK]) {
+    uint16_t *const dstp = sx + ((dd + 3) % 3) * PLANE;
+#pragma unroll
+    for (int k = 0; k < NPK; ++k) {
+      int const e0 = tid + k * UT;
+      int const e = e0 < NPI ? e0 : NPI - 1;
+      st_swz(dstp, e >> 1, e & 1, v[k]);
+    }
+  };
+  {
+    uint4 v0[NPK], v1[NPK], v2[NPK];
+    load_plane(d0 - 1, v0);
+    load_plane(d0, v1);
+    load_plane(d0 + 1, v2);
+    store_plane(d0 - 1, v0);
+    store_plane(d0, v1);
+    store_plane(d0 + 1, v2);
+  }
+  int const row = tid >> 6, col = tid & 63;
+  float const b0 = bias[0];
+  for (int j = 0; j < HDB; ++j) {
+    int const d = d0 + j;
+    uint4 nxt[NPK];
+    if (j + 1 < HDB) load_plane(d + 2, nxt);
+    float acc = 0.f;
+#pragma unroll
+    for (int kd = 0; kd < 3; ++kd) {
+      __syncthreads();  // the ring is staged; the E buffer is free
+      uint16_t const *const pl = sx + ((d + kd - 1 + 3) % 3) * PLANE;
+      for (int t = wave; t < NVT; t += UT / 64) {
+        int const u = t * 16 + i;
+        bf16x4 const a = ld_swz(pl, u < NV ? u : NV - 1, kq);
+        f32x4 e = {0.f, 0.f, 0.f, 0.f};
+        e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[kd], e, 0, 0, 0);
+        // D: lane holds voxels 16 t + 4 kq + r of tap i
+        if (i < 9) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) se[(t * 16 + 4 * kq + r) * EP + i] = e[r];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kh = 0; kh < 3; ++kh)
+#pragma unroll
+        for (int kw = 0; kw < 3; ++kw)
+          acc += se[((row + kh) * PW + col + kw) * EP + kh * 3 + kw];
+    }
+    int const h = h0 + row, w = w0 + col;
+    if (h < H && w < W && d < D) {
+      int64_t const v = (((int64_t)n * D + d) * H + h) * W + w;
+      float const o = (acc + b0) * unscale;
+      dst[v] = base ? base[v] + o : o;
+    }
+    if (j + 1 < HDB) {
+      __syncthreads();  // every wave is done with plane d - 1
+      store_plane(d + 2, nxt);
     }
   }
 }
@@ -945,16 +1022,15 @@ static int conv_shape_ok(int N, int D, int H, int W, int CI, int CO) {
   return 1;
 }
 
-#define CONV_LAUNCH_IO(a, b, u, io, fs, bs, ds)                                              \
+#define CONV_LAUNCH(a, b, u)                                                                \
   {                                                                                        \
     int const db = ConvDepth<b, u>::DB;                                                    \
     dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H),  \
                     (unsigned)(N * ((D + db - 1) / db)));                                  \
-    return ulaunch("conv3", k_conv3<a, b, u, io>, grid, dim3(UT), stream,                    \
-                   (uint16_t const *)x, (uint16_t const *)xu, (uint16_t const *)wt, bias,    \
-                   (uint16_t const *)gate, (uint16_t *)y, D, H, W, mode, fs, bs, ds);       \
+    return ulaunch("conv3", k_conv3<a, b, u>, grid, dim3(UT), stream, (uint16_t const *)x,   \
+                   (uint16_t const *)xu, (uint16_t const *)wt, bias, (uint16_t const *)gate, \
+                   (uint16_t *)y, D, H, W, mode);                                          \
   }
-#define CONV_LAUNCH(a, b, u) CONV_LAUNCH_IO(a, b, u, 0, 1.0f, (float const *)0, (float *)0)
 
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
@@ -990,15 +1066,7 @@ int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *b
 
 int agdu_conv_in(float const *src, void const *wt, int wci, float const *bias, void *y, int N,
                  int D, int H, int W, float scale, void *stream) {
-  void const *const x = src;
-  void const *const xu = NULL;
-  void const *const gate = NULL;
-  int const mode = 3;
   if (!conv_shape_ok(N, D, H, W, 16, 16)) return 1;
-  (void)x;
-  (void)xu;
-  (void)gate;
-  (void)mode;
   if (wci < 1) {
     fprintf(stderr, "ERROR: first-layer weights must have input channels, got %d\n", wci);
     return 1;
@@ -1011,16 +1079,15 @@ int agdu_conv_in(float const *src, void const *wt, int wci, float const *bias, v
 
 int agdu_conv_out(void const *x, void const *wt, int wco, float const *bias, float const *base,
                   float *dst, int N, int D, int H, int W, float unscale, void *stream) {
-  void const *const xu = NULL;
-  void const *const gate = NULL;
-  void *const y = NULL;
-  int const mode = 0;
   if (!conv_shape_ok(N, D, H, W, 16, 16)) return 1;
-  if (wco != 16 || W % 4) {
-    fprintf(stderr, "ERROR: head with %d output channels and W = %d unsupported\n", wco, W);
+  if (wco < 1) {
+    fprintf(stderr, "ERROR: head with %d output channels unsupported\n", wco);
     return 1;
   }
-  CONV_LAUNCH_IO(16, 16, 0, 2, unscale, base, dst)
+  dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H),
+                  (unsigned)(N * ((D + HDB - 1) / HDB)));
+  return ulaunch("conv_head", k_conv_head, grid, dim3(UT), stream, (uint16_t const *)x,
+                 (uint16_t const *)wt, wco, bias, base, dst, D, H, W, unscale);
 }
 
 int agdu_flip_weights(void const *wt, void *wtt, int CI, int CO, void *stream) {

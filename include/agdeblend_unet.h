@@ -37,7 +37,7 @@ int agdu_conv_in(float const *src, void const *wt, int wci, float const *bias, v
                  int D, int H, int W, float scale, void *stream);
 /* Output channel 0 of a 16 -> 16 convolution, in fp32:
  * dst[v] = base[v] + unscale (bias[0] + sum_tap sum_ci wt[tap][0][ci] x[v + tap][ci])
- * (base may be NULL), wt [27][wco][16] with wco = 16, W a multiple of 4 */
+ * (base may be NULL), wt [27][wco][16] */
 int agdu_conv_out(void const *x, void const *wt, int wco, float const *bias, float const *base,
                   float *dst, int N, int D, int H, int W, float unscale, void *stream);
 /* weights of the input gradient: wtt[tap][ci][co] = wt[26 - tap][co][ci] */
