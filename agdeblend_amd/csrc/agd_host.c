@@ -1948,8 +1948,9 @@ EXPORT float agdx_last_lambda0(void) { return g_last_lambda0; }
 static int warm_model_flat(AgdProblem *p, float const *y_dev) {
   if (agdg_window_traces(&p->dev, p->d_wsrc, p->d_wnt, y_dev, p->dev.T)) return 1;
   if (agdg_adj_time(&p->dev, p->dev.T, 1)) return 1;
-  if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, 0)) return 1;
-  p->u_valid = 0;
+  /* with the fused kernels the model's forward spatial stage comes along */
+  if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, p->fuse)) return 1;
+  p->u_valid = p->fuse;
   return 0;
 }
 
@@ -2169,13 +2170,13 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
   slope *= lamb;
   lamb *= f;
   if (warm != NULL) {
-    if (warm_model(p, warm)) return 1;
+    if (warm_model(p, warm)) return 1; /* sets u_valid */
   } else if (warm_dev != NULL) {
     if (warm_model_flat(p, warm_dev)) return 1;
-  } else if (x0 == NULL) {
-    if (agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
+  } else {
+    if (x0 == NULL && agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
+    p->u_valid = 0;
   }
-  p->u_valid = 0;
   if (print_freq > 0) {
     double r2 = 0.0;
     if (agdg_norm2(&p->dev, p->dev.d, p->cfg.rec_len, &r2)) return 1;

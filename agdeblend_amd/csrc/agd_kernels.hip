@@ -1021,8 +1021,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
 //   1  x <- -alpha g, max |x|^2 into maxbits (first iteration)
 //   2  x <- x - alpha g
 //   3  x <- g
-// With FUSE (mode 0) the updated model's forward spatial transform, the
-// next iteration's first stage, is written to U.
+// With FUSE (modes 0 and 3) the new model's forward spatial transform, the
+// next forward's first stage, is written to U.
 template <int P0, int P1, bool TILED, int MODE, bool FUSE>
 __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
@@ -1779,14 +1779,16 @@ static int adj_spatial_layout(AgdDev const *g, AgdClass const &c, float alpha, f
                   : launch_adj_spatial<P0, P1, TILED, 0, false>(g, c, alpha, lamb);
     case 1: return launch_adj_spatial<P0, P1, TILED, 1, false>(g, c, alpha, lamb);
     case 2: return launch_adj_spatial<P0, P1, TILED, 2, false>(g, c, alpha, lamb);
-    default: return launch_adj_spatial<P0, P1, TILED, 3, false>(g, c, alpha, lamb);
+    default:
+      return fuse ? launch_adj_spatial<P0, P1, TILED, 3, true>(g, c, alpha, lamb)
+                  : launch_adj_spatial<P0, P1, TILED, 3, false>(g, c, alpha, lamb);
   }
 }
 
 template <int P0, int P1>
 static int adj_spatial_sized(AgdDev const *g, AgdClass const &c, float alpha, float lamb,
                              int mode, int fuse) {
-  if (fuse && mode != 0) {
+  if (fuse && mode != 0 && mode != 3) {
     fprintf(stderr, "ERROR: fused update requested for mode %d\n", mode);
     return 1;
   }
