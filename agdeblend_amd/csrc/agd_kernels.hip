@@ -1472,91 +1472,24 @@ struct Shape {
   static constexpr int W = P * 4 / 5;
   static constexpr int KP = r16(P);
   static constexpr int KW = r16(W);
-  static constexpr int PITCH = KP + 4;     // floats per LDS row
-  static constexpr int PL = KP * PITCH;    // floats of one part of a buffer
+  static constexpr int BP = KP + 8;   // bf16 per LDS row of the split operands
+  static constexpr int BL = KP * BP;  // bf16 of one (part, hi / lo) array
+  static constexpr int FP = KP + 4;   // floats per LDS row of the fp32 output
+  static constexpr int FL = KP * FP;  // floats of one part of the fp32 output
 };
 
-// Y = C Z with C[p][q] = e^{SIGN 2 pi i p q / P} for p < MC, q < KC (else 0).
-// Z: LDS [2][rows r16(KC)][PITCH] (real, imaginary parts), columns
-// r16(NZ); Y likewise with rows r16(MC). Waves own 16-row tiles of the
-// stacked output (tile wave + 4 t); their constant operands are built once
-// per block (Left::init) and stay in registers.
-template <int P, int MC, int KC, int NZ, int SIGN>
-struct Left {
-  static constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
-  static constexpr int KMC = r16(MC), KKC = r16(KC), KNZ = r16(NZ);
-  static constexpr int NMT = 2 * KMC / 16, NKS = 2 * KKC / 16, NNT = KNZ / 16;
-  static constexpr int TPW = (NMT + NTHR / 64 - 1) / (NTHR / 64);  // tiles per wave
-  bf16x4 h[TPW][NKS], l[TPW][NKS];
+// The split data live in LDS as bf16 hi / lo arrays per part (real,
+// imaginary): s[(part * 2 + lo) * BL + row * BP + col]. A split array is
+// read 4 consecutive elements (8 bytes) at a time along its rows.
+__device__ __forceinline__ bf16x4 ld4(uint16_t const *p) { return *(bf16x4 const *)p; }
 
-  __device__ __forceinline__ void init() {
-    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int const i = lane & 15, kq = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      int const mt = wave + t * (NTHR / 64);
-      int const mr = mt * 16 + i;
-      int const po = mr / KMC, pr = mr - po * KMC;
-#pragma unroll
-      for (int ks = 0; ks < NKS; ++ks) {
-        float v[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int const kk = ks * 16 + 4 * kq + r;
-          int const pi = kk / KKC, q = kk - pi * KKC;
-          float2 const c = (mt < NMT && pr < MC && q < KC) ? coef<P, SIGN>(pr * q)
-                                                           : make_float2(0.f, 0.f);
-          v[r] = part(c, po, pi);
-        }
-        split4(v, h[t][ks], l[t][ks]);
-      }
-    }
-  }
-
-  __device__ __forceinline__ void apply(float const *__restrict__ z,
-                                        float *__restrict__ y) const {
-    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int const i = lane & 15, kq = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      int const mt = wave + t * (NTHR / 64);
-      if (mt >= NMT) break;
-      for (int nt = 0; nt < NNT; ++nt) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        int const col = nt * 16 + i;
-#pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
-          float v[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            int const kk = ks * 16 + 4 * kq + r;
-            int const pi = kk / KKC, q = kk - pi * KKC;
-            v[r] = z[pi * PL + q * PITCH + col];
-          }
-          bf16x4 bh, bl;
-          split4(v, bh, bl);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bh, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bl, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l[t][ks], bh, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int const yr = mt * 16 + 4 * kq + r;
-          int const yp = yr / KMC;
-          y[yp * PL + (yr - yp * KMC) * PITCH + col] = acc[r];
-        }
-      }
-    }
-  }
-};
-
-// Y = Z C^T with C[o][q] = e^{SIGN 2 pi i o q / P} for o < NC, q < KC (else
-// 0). Z: LDS rows r16(MZ), columns r16(KC); Y: rows r16(MZ), columns
-// r16(NC). Waves own 16-column tiles of the stacked output, whose constant
-// operands are built once per block.
+// Y = Z C^T, C[o][q] = e^{SIGN 2 pi i o q / P} for o < NC, q < KC (else 0):
+// Z split row-major ([row][q]), rows r16(MZ); Y written split and
+// column-major ([o][row]) for the next product. Waves own 16-column tiles
+// of the stacked output; the constant operands are built once (init).
 template <int P, int MZ, int KC, int NC, int SIGN>
 struct Right {
-  static constexpr int PITCH = Shape<P>::PITCH, PL = Shape<P>::PL;
+  static constexpr int BP = Shape<P>::BP, BL = Shape<P>::BL;
   static constexpr int KMZ = r16(MZ), KKC = r16(KC), KNC = r16(NC);
   static constexpr int NNT = 2 * KNC / 16, NKS = 2 * KKC / 16, NMT = KMZ / 16;
   static constexpr int TPW = (NNT + NTHR / 64 - 1) / (NTHR / 64);
@@ -1586,8 +1519,8 @@ struct Right {
     }
   }
 
-  __device__ __forceinline__ void apply(float const *__restrict__ z,
-                                        float *__restrict__ y) const {
+  __device__ __forceinline__ void apply(uint16_t const *__restrict__ z,
+                                        uint16_t *__restrict__ y) const {
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int const i = lane & 15, kq = lane >> 4;
 #pragma unroll
@@ -1603,102 +1536,235 @@ struct Right {
         for (int ks = 0; ks < NKS; ++ks) {
           int const kk = ks * 16 + 4 * kq;
           int const pi = kk / KKC, q = kk - pi * KKC;
-          float4 const zz = *(float4 const *)(z + pi * PL + row * PITCH + q);
-          float const v[4] = {zz.x, zz.y, zz.z, zz.w};
-          bf16x4 ah, al;
-          split4(v, ah, al);
+          bf16x4 const ah = ld4(z + (pi * 2) * BL + row * BP + q);
+          bf16x4 const al = ld4(z + (pi * 2 + 1) * BL + row * BP + q);
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, h[t][ks], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, l[t][ks], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, h[t][ks], acc, 0, 0, 0);
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) y[po * PL + (mt * 16 + 4 * kq + r) * PITCH + o] = acc[r];
+        // D: rows mt 16 + 4 kq + r of column nc: 4 consecutive rows of the
+        // column-major output
+        float const v[4] = {acc[0], acc[1], acc[2], acc[3]};
+        bf16x4 vh, vl;
+        split4(v, vh, vl);
+        int const off = o * BP + mt * 16 + 4 * kq;
+        *(bf16x4 *)(y + (po * 2) * BL + off) = vh;
+        *(bf16x4 *)(y + (po * 2 + 1) * BL + off) = vl;
       }
     }
   }
 };
+
+// Y = C Z, C[p][q] = e^{SIGN 2 pi i p q / P} for p < MC, q < KC (else 0):
+// Z split column-major ([col][q]), columns r16(NZ); Y in fp32 row-major
+// ([part][p][col], FP floats per row). Waves own 16-row tiles of the
+// stacked output; the constant operands are built once (init).
+template <int P, int MC, int KC, int NZ, int SIGN>
+struct Left {
+  static constexpr int BP = Shape<P>::BP, BL = Shape<P>::BL;
+  static constexpr int FP = Shape<P>::FP, FL = Shape<P>::FL;
+  static constexpr int KMC = r16(MC), KKC = r16(KC), KNZ = r16(NZ);
+  static constexpr int NMT = 2 * KMC / 16, NKS = 2 * KKC / 16, NNT = KNZ / 16;
+  static constexpr int TPW = (NMT + NTHR / 64 - 1) / (NTHR / 64);
+  bf16x4 h[TPW][NKS], l[TPW][NKS];
+
+  __device__ __forceinline__ void init() {
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const i = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      int const mt = wave + t * (NTHR / 64);
+      int const mr = mt * 16 + i;
+      int const po = mr / KMC, pr = mr - po * KMC;
+#pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int const kk = ks * 16 + 4 * kq + r;
+          int const pi = kk / KKC, q = kk - pi * KKC;
+          float2 const c = (mt < NMT && pr < MC && q < KC) ? coef<P, SIGN>(pr * q)
+                                                           : make_float2(0.f, 0.f);
+          v[r] = part(c, po, pi);
+        }
+        split4(v, h[t][ks], l[t][ks]);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void apply(uint16_t const *__restrict__ z,
+                                        float *__restrict__ y) const {
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int const i = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      int const mt = wave + t * (NTHR / 64);
+      if (mt >= NMT) break;
+      for (int nt = 0; nt < NNT; ++nt) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        int const col = nt * 16 + i;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+          int const kk = ks * 16 + 4 * kq;
+          int const pi = kk / KKC, q = kk - pi * KKC;
+          bf16x4 const bh = ld4(z + (pi * 2) * BL + col * BP + q);
+          bf16x4 const bl = ld4(z + (pi * 2 + 1) * BL + col * BP + q);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bh, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bl, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l[t][ks], bh, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int const yr = mt * 16 + 4 * kq + r;
+          int const yp = yr / KMC;
+          y[yp * FL + (yr - yp * KMC) * FP + col] = acc[r];
+        }
+      }
+    }
+  }
+};
+
+// one complex value split into the four arrays at (row, col)
+__device__ __forceinline__ void put_split(uint16_t *s, int BL, int off, float2 v) {
+  unsigned short const rh = f2bf(v.x), ih = f2bf(v.y);
+  s[off] = rh;
+  s[BL + off] = f2bf(v.x - bf2f(rh));
+  s[2 * BL + off] = ih;
+  s[3 * BL + off] = f2bf(v.y - bf2f(ih));
+}
 }  // namespace x3
 
-// X -> U (the forward spatial stage), bf16 x 3 DFT matrices
+// X -> U (the forward spatial stage), bf16 x 3 DFT matrices:
+// Y1 = X A1^T (columns), then U = A0 Y1 (rows)
 template <int P>
 __global__ __launch_bounds__(NTHR) void k_fwd_spatial_x3(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     float2 const *__restrict__ X, float2 *__restrict__ U) {
   using S = x3::Shape<P>;
-  constexpr int W = S::W, KP = S::KP, PITCH = S::PITCH, PL = S::PL;
-  __shared__ __align__(16) float za[2 * PL];
-  __shared__ __align__(16) float zb[2 * PL];
+  constexpr int W = S::W, KP = S::KP, BP = S::BP, BL = S::BL, FP = S::FP, FL = S::FL;
+  static_assert(2 * FL * 4 <= 4 * BL * 2, "the fp32 output fits the input's space");
+  __shared__ __align__(16) uint16_t za[4 * BL];  // X split, row-major; then U (fp32)
+  __shared__ __align__(16) uint16_t zb[4 * BL];  // Y1 split, column-major
+  float *const zo = (float *)za;
   AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const &w = wins[t.win];
   float2 const *const src = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * (P * P);
   float2 *const Utw = U + w.u_off + (int64_t)t.tw * w.Ft * (W * W);
-  x3::Left<P, W, P, P, +1> a0;   // T = A0 X: rows n0, columns c1
-  x3::Right<P, W, P, W, +1> a1;  // U = T A1^T: rows n0, columns n1
-  a0.init();
+  x3::Right<P, P, P, W, +1> a1;  // Y1 = X A1^T: rows c0, columns n1
+  x3::Left<P, W, P, W, +1> a0;   // U = A0 Y1: rows n0, columns n1
   a1.init();
+  a0.init();
+  // plane k + 1 is loaded into registers while plane k is transformed
+  constexpr int NE = (KP * KP + NTHR - 1) / NTHR;
+  auto load = [&](int k, float2 (&v)[NE]) {
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      int const e = threadIdx.x + q * NTHR;
+      int const r = e / KP, c = e - r * KP;
+      bool const ok = e < KP * KP && r < P && c < P && k < t.nk;
+      float2 const u = src[ok ? (int64_t)k * P * P + r * P + c : 0];
+      v[q] = ok ? u : make_float2(0.f, 0.f);
+    }
+  };
+  float2 cur[NE];
+  load(0, cur);
   for (int k = 0; k < t.nk; ++k) {
     __syncthreads();
-    for (int e = threadIdx.x; e < KP * KP; e += NTHR) {
-      int const r = e / KP, c = e - r * KP;
-      float2 const v = (r < P && c < P) ? src[(int64_t)k * P * P + r * P + c]
-                                        : make_float2(0.f, 0.f);
-      za[r * PITCH + c] = v.x;
-      za[PL + r * PITCH + c] = v.y;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      int const e = threadIdx.x + q * NTHR;
+      if (e < KP * KP) x3::put_split(za, BL, (e / KP) * BP + e % KP, cur[q]);
     }
     __syncthreads();
-    a0.apply(za, zb);
+    load(k + 1, cur);
+    a1.apply(za, zb);
     __syncthreads();
-    a1.apply(zb, za);
+    a0.apply(zb, zo);
     __syncthreads();
     for (int tr = threadIdx.x; tr < W * W; tr += NTHR) {
       int const n0 = tr / W, n1 = tr - n0 * W;
       Utw[((int64_t)(tr / TG) * w.Ft + t.k0 + k) * TG + (tr % TG)] =
-          make_float2(za[n0 * PITCH + n1], za[PL + n0 * PITCH + n1]);
+          make_float2(zo[n0 * FP + n1], zo[FL + n0 * FP + n1]);
     }
   }
 }
 
-// V -> G and the model update by MODE (as k_adj_spatial_t), bf16 x 3
+// V -> G and the model update by MODE (as k_adj_spatial_t), bf16 x 3:
+// R = V B1^T (columns), then G = B0 R (rows)
 template <int P, int MODE>
 __global__ __launch_bounds__(NTHR) void k_adj_spatial_x3(
     AgdWin const *__restrict__ wins, AgdTask const *__restrict__ tasks, int task0,
     float2 const *__restrict__ V, float2 *__restrict__ X, float alpha, float lamb,
     float const *__restrict__ lamb_dev, unsigned int *__restrict__ maxbits) {
   using S = x3::Shape<P>;
-  constexpr int W = S::W, KW = S::KW, PITCH = S::PITCH, PL = S::PL;
-  __shared__ __align__(16) float za[2 * PL];
-  __shared__ __align__(16) float zb[2 * PL];
+  constexpr int W = S::W, KW = S::KW, BP = S::BP, BL = S::BL, FP = S::FP, FL = S::FL;
+  static_assert(2 * FL * 4 <= 4 * BL * 2, "the fp32 output fits the input's space");
+  __shared__ __align__(16) uint16_t za[4 * BL];  // V split, row-major; then G (fp32)
+  __shared__ __align__(16) uint16_t zb[4 * BL];  // R split, column-major
+  float *const zo = (float *)za;
   if (lamb_dev != nullptr) lamb = lamb_dev[0];
   AgdTask const t = tasks[task0 + blockIdx.x];
   AgdWin const &w = wins[t.win];
   float2 const *const Vtw = V + w.u_off + (int64_t)t.tw * w.Ft * (W * W);
   float2 *const x = X + w.x_off + ((int64_t)t.tw * w.Ft + t.k0) * (P * P);
-  float mx = 0.f;
   x3::Right<P, W, W, P, -1> b1;  // R = V B1^T: rows n0, columns c1
   x3::Left<P, P, W, P, -1> b0;   // G = B0 R: rows c0, columns c1
   b1.init();
   b0.init();
+  float mx = 0.f;
+  // plane k + 1 is loaded into registers while plane k is transformed
+  constexpr int NE = (KW * KW + NTHR - 1) / NTHR;
+  auto load = [&](int k, float2 (&v)[NE]) {
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      int const e = threadIdx.x + q * NTHR;
+      int const n0 = e / KW, n1 = e - n0 * KW;
+      bool const ok = e < KW * KW && n0 < W && n1 < W && k < t.nk;
+      int const tr = ok ? n0 * W + n1 : 0;
+      float2 const u = Vtw[((int64_t)(tr / TG) * w.Ft + t.k0 + (ok ? k : 0)) * TG + (tr % TG)];
+      v[q] = ok ? u : make_float2(0.f, 0.f);
+    }
+  };
+  // the model values the update reads (modes 0 and 2), prefetched likewise
+  constexpr int NX = (MODE == 0 || MODE == 2) ? (P * P + NTHR - 1) / NTHR : 1;
+  auto load_x = [&](int k, float2 (&v)[NX]) {
+    if constexpr (MODE == 0 || MODE == 2) {
+#pragma unroll
+      for (int q = 0; q < NX; ++q) {
+        int const e = threadIdx.x + q * NTHR;
+        bool const ok = e < P * P && k < t.nk;
+        float2 const u = x[ok ? (int64_t)k * P * P + e : 0];
+        v[q] = ok ? u : make_float2(0.f, 0.f);
+      }
+    }
+  };
+  float2 cur[NE], xcur[NX];
+  load(0, cur);
+  load_x(0, xcur);
   for (int k = 0; k < t.nk; ++k) {
     __syncthreads();
-    for (int e = threadIdx.x; e < KW * KW; e += NTHR) {
-      int const n0 = e / KW, n1 = e - n0 * KW;
-      float2 v = make_float2(0.f, 0.f);
-      if (n0 < W && n1 < W) {
-        int const tr = n0 * W + n1;
-        v = Vtw[((int64_t)(tr / TG) * w.Ft + t.k0 + k) * TG + (tr % TG)];
-      }
-      za[n0 * PITCH + n1] = v.x;
-      za[PL + n0 * PITCH + n1] = v.y;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      int const e = threadIdx.x + q * NTHR;
+      if (e < KW * KW) x3::put_split(za, BL, (e / KW) * BP + e % KW, cur[q]);
     }
     __syncthreads();
+    load(k + 1, cur);
+    float2 xnow[NX];
+#pragma unroll
+    for (int q = 0; q < NX; ++q) xnow[q] = xcur[q];
+    load_x(k + 1, xcur);
     b1.apply(za, zb);
     __syncthreads();
-    b0.apply(zb, za);
+    b0.apply(zb, zo);
     __syncthreads();
     float2 *const xk = x + (int64_t)k * P * P;
-    for (int e = threadIdx.x; e < P * P; e += NTHR) {
+#pragma unroll
+    for (int q = 0; q < (P * P + NTHR - 1) / NTHR; ++q) {
+      int const e = threadIdx.x + q * NTHR;
+      if (e >= P * P) break;
       int const c0 = e / P, c1 = e - c0 * P;
-      float2 const gv = make_float2(za[c0 * PITCH + c1], za[PL + c0 * PITCH + c1]);
+      float2 const gv = make_float2(zo[c0 * FP + c1], zo[FL + c0 * FP + c1]);
       float2 nv;
       if constexpr (MODE == 3) {
         nv = gv;
@@ -1706,7 +1772,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_x3(
         nv = make_float2(0.f - alpha * gv.x, 0.f - alpha * gv.y);
         mx = fmaxf(mx, nv.x * nv.x + nv.y * nv.y);
       } else {
-        nv = xk[e];
+        nv = xnow[q];
         nv.x -= alpha * gv.x;
         nv.y -= alpha * gv.y;
         if constexpr (MODE == 0) {
