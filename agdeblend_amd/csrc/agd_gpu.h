@@ -80,6 +80,8 @@ typedef struct {
   double *partials;           /* reduction partials */
   int n_partials;
   int precision;              /* spatial DFTs: 0 FP32 register FFTs, 1 bf16 x 3 MFMA */
+  int wavelet_mh;             /* half length of every window's wavelet transform when
+                                 they share one (0 otherwise): a specialised kernel */
 } AgdDev;
 
 /* A gather of source traces into a flat record, or of record slices back

@@ -1563,6 +1563,18 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->dev.wspec = (float const *)dev_upload(p, wspec.p, wspec.n * 2 * sizeof(float));
     p->dev.wsrc_off = (int const *)dev_upload(p, wsrc_off, (size_t)p->n_wtr * sizeof(int));
     p->dev.lds_wavelet = lds_w;
+    /* one transform length for every wavelet window: the specialised kernel
+     * (AGD_GENERIC_WAVELET forces the generic one) */
+    p->dev.wavelet_mh = 0;
+    if (p->use_wavelets && getenv("AGD_GENERIC_WAVELET") == NULL) {
+      int mh = -1;
+      for (w = 0; w < p->n_win; w++) {
+        if (p->win[w].wplan < 0) continue;
+        if (mh < 0) mh = p->win[w].wM / 2;
+        else if (mh != p->win[w].wM / 2) mh = 0;
+      }
+      p->dev.wavelet_mh = mh > 0 ? mh : 0;
+    }
     p->dev.X = (float *)dev_alloc(p, (size_t)xo * 8);
     p->dev.U = (float *)dev_alloc(p, (size_t)uo * 8);
     p->dev.V = (float *)dev_alloc(p, (size_t)uo * 8);

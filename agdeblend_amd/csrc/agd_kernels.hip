@@ -851,6 +851,150 @@ __global__ __launch_bounds__(NTHR) void k_wavelet(
 }
 
 // ---------------------------------------------------------------------------
+// Wavelet convolution with a compile-time transform length: the real length
+// Mr = 2 MH of the window's convolution, MH = M1 M2. One wave per window
+// trace; the half-length complex FFT (z[n] = x[2n] + i x[2n+1]) is a
+// four-step transform in one padded LDS tile [M1][M2 + 1]: forward over the
+// columns (length M1, register codelets), twiddle, then over the rows
+// (length M2), leaving Z[k1 + M1 k2] at (k1, k2); the real-spectrum unpack,
+// the product with the wavelet spectrum and the pack for the inverse run in
+// place on the pairs (k, MH - k); the inverse runs rows, twiddle, columns,
+// which returns the natural order. The same arithmetic as k_wavelet.
+// ---------------------------------------------------------------------------
+template <int M1, int M2>
+__global__ __launch_bounds__(64) void k_wavelet_t(
+    AgdWin const *__restrict__ wins, int const *__restrict__ trace_win, int trace_first,
+    AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
+    float2 const *__restrict__ wspec, int const *__restrict__ wsrc_off, int adjoint,
+    AgdEntry const *__restrict__ went, float const *__restrict__ rec, float *__restrict__ T,
+    float *__restrict__ T2) {
+  constexpr int MH = M1 * M2, LD = M2 + 1, MR = 2 * MH;
+  __shared__ float2 sa[M1 * LD];
+  int const lane = threadIdx.x;
+  int const g = trace_first + blockIdx.x;
+  AgdWin const w = wins[trace_win[g]];
+  int const tr = g - w.trace0;
+  int const N = w.nconv;
+  int const nfk = w.nfk;
+  float *Tio = T + w.t_off + (int64_t)tr * N;
+  float *Tout = T2 + w.t_off + (int64_t)tr * N;
+  int const so = wsrc_off[g];
+  AgdEntry const e = went[g];
+  int const nin = adjoint ? N : nfk;
+  if (so < 0) {
+    // not multiplied: C2R(R2C(x)) = N x
+    float const fN = (float)N;
+    if (!adjoint) {
+      for (int t = lane; t < N; t += 64) Tio[t] = t < nfk ? Tio[t] * fN : 0.f;
+    } else {
+      for (int t = lane; t < N; t += 64) {
+        int const i2 = t - e.trace_start;
+        float const x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+        Tout[t] = t < nfk ? x * fN : 0.f;
+      }
+    }
+    return;
+  }
+  float2 const *const rt = roots + plans[w.wplan].roots_off;      // e^{-2 pi i m / MH}
+  float2 const *const rfull = roots + plans[w.wplan_full].roots_off;  // e^{-2 pi i m / MR}
+  float2 const *const hs = wspec + so;
+  auto sample = [&](int t) -> float {
+    if (t >= nin) return 0.f;
+    if (!adjoint) return Tio[t];
+    int const i2 = t - e.trace_start;
+    return (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+  };
+  for (int n = lane; n < MH; n += 64)
+    sa[(n / M2) * LD + n % M2] = make_float2(sample(2 * n), sample(2 * n + 1));
+  __syncthreads();
+  // forward: columns, twiddle, rows
+  if (lane < M2) {
+    float2 v[M1];
+#pragma unroll
+    for (int a = 0; a < M1; ++a) v[a] = sa[a * LD + lane];
+    agdfft::fft<M1, -1>(v);
+#pragma unroll
+    for (int a = 0; a < M1; ++a) sa[a * LD + lane] = cmul(v[a], rt[(lane * a) % MH]);
+  }
+  __syncthreads();
+  if (lane < M1) {
+    float2 v[M2];
+#pragma unroll
+    for (int b = 0; b < M2; ++b) v[b] = sa[lane * LD + b];
+    agdfft::fft<M2, -1>(v);
+#pragma unroll
+    for (int b = 0; b < M2; ++b) sa[lane * LD + b] = v[b];
+  }
+  __syncthreads();
+  // spectrum: real unpack, product with the wavelet, pack for the inverse,
+  // in place on the pairs (k, MH - k)
+  auto zpos = [](int k) { return (k % M1) * LD + k / M1; };
+  auto unpack = [&](float2 zk, float2 zo, int k) {
+    float2 const zc = make_float2(zo.x, -zo.y);
+    float2 const ev = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
+    float2 const o = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
+    float2 X = cadd(ev, cmul(o, rootf(rfull, MR, k, -1)));
+    if (k == 0 || k == MH) X.y = 0.f;
+    float2 h = hs[k];
+    if (adjoint) h.y = -h.y;
+    return cmul(X, h);
+  };
+  auto pack = [&](float2 xk, float2 xo, int k) {
+    if (k == 0) {
+      xk.y = 0.f;
+      xo.y = 0.f;
+    }
+    float2 const xc = make_float2(xo.x, -xo.y);
+    float2 const ev = cadd(xk, xc);
+    float2 const o = cmul(csub(xk, xc), rootf(rfull, MR, k, +1));
+    return make_float2(ev.x - o.y, ev.y + o.x);
+  };
+  for (int k = lane; k <= MH / 2; k += 64) {
+    int const kc = MH - k;
+    float2 const za = sa[zpos(k)];
+    float2 const zb = sa[zpos(kc % MH)];
+    float2 const hk = unpack(za, zb, k);
+    float2 const hc = unpack(zb, za, kc);
+    sa[zpos(k)] = pack(hk, hc, k);
+    if (k != 0 && k != kc) sa[zpos(kc)] = pack(hc, hk, kc);
+  }
+  __syncthreads();
+  // inverse: rows, twiddle, columns
+  if (lane < M1) {
+    float2 v[M2];
+#pragma unroll
+    for (int b = 0; b < M2; ++b) v[b] = sa[lane * LD + b];
+    agdfft::fft<M2, +1>(v);
+#pragma unroll
+    for (int b = 0; b < M2; ++b) {
+      float2 t = rt[(lane * b) % MH];
+      t.y = -t.y;
+      sa[lane * LD + b] = cmul(v[b], t);
+    }
+  }
+  __syncthreads();
+  if (lane < M2) {
+    float2 v[M1];
+#pragma unroll
+    for (int a = 0; a < M1; ++a) v[a] = sa[a * LD + lane];
+    agdfft::fft<M1, +1>(v);
+#pragma unroll
+    for (int a = 0; a < M1; ++a) sa[a * LD + lane] = v[a];
+  }
+  __syncthreads();
+  int const nout = adjoint ? N : N;
+  for (int t = lane; t < nout; t += 64) {
+    int const n = t >> 1;
+    float2 const z = sa[(n / M2) * LD + n % M2];
+    float const y = (t & 1) ? z.y : z.x;
+    if (!adjoint)
+      Tio[t] = y;
+    else
+      Tout[t] = t < nfk ? y : 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Output assembly and blend extraction
 // ---------------------------------------------------------------------------
 // Threshold of the next iteration of a captured loop: lamb_dev[0] =
@@ -2166,6 +2310,11 @@ int agdg_wavelet(AgdDev const *g, int adjoint) {
   int const first = g->wtrace0[g->w0];
   int const n = g->wtrace0[g->w1] - first;
   if (n <= 0) return 0;
+  if (g->wavelet_mh == 2800)
+    return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet_t<50, 56>, dim3(n),
+                  dim3(64), 0, g->win, g->trace_win, first, g->plans,
+                  (float2 const *)g->roots, (float2 const *)g->wspec, g->wsrc_off, adjoint,
+                  g->went, (float const *)g->r, g->T, g->T2);
   return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet, dim3(n),
                 dim3(NTHR), g->lds_wavelet, g->win, g->trace_win, first,
                 g->plans, (float2 const *)g->roots, (float2 const *)g->wspec,
