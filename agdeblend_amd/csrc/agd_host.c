@@ -1005,6 +1005,10 @@ static int const kSpatialSizes[][2] = {{1, 10},  {1, 20},  {1, 40},  {10, 10},
                                        {10, 20}, {10, 40}, {20, 10}, {20, 20},
                                        {20, 40}, {40, 10}, {40, 20}, {40, 40}};
 static int const kTimeSizes[][2] = {{5, 8}, {10, 8}, {10, 16}, {20, 16}};
+/* every (M1, M2) factorization with a kernel (the autotuner's candidates) */
+static int const kTimeCands[][2] = {{5, 8},   {8, 5},   {4, 10},  {10, 8},  {8, 10},
+                                    {16, 5},  {10, 16}, {16, 10}, {20, 8},  {8, 20},
+                                    {20, 16}, {16, 20}, {32, 10}, {10, 32}};
 
 static void spatial_key(AgdWin const *W, int *p0, int *p1) {
   size_t i;
@@ -1950,8 +1954,11 @@ EXPORT int agdx_get_model(AgdProblem *p, float *x) {
 }
 
 static float g_last_lambda0 = 0.0f;
+static int g_last_warm_fallback = 0;
 
 EXPORT float agdx_last_lambda0(void) { return g_last_lambda0; }
+
+EXPORT int agdx_last_warm_fallback(void) { return g_last_warm_fallback; }
 
 /* agdeblend.c:526-629 */
 /* Warm-start model s R2C(pad(y_w)) of a data estimate y (patch layout)
@@ -2057,9 +2064,10 @@ static int run_captured(AgdProblem *p, float alpha, float const *lamb, int n, in
 }
 
 /* Iterations 1 .. n_its - 1 with thresholds sched[it], from the current
- * model (forward not yet computed). */
+ * model; r_ready: dev.r already holds its residual A x - d (otherwise the
+ * first iteration computes it). */
 static int run_loop(AgdProblem *p, float alpha, float const *sched, int n_its,
-                    int print_freq) {
+                    int print_freq, int r_ready) {
   int it, rc = 0;
   /* from the second iteration on, every iteration launches the same
    * kernels with the same arguments except the threshold: capture one
@@ -2068,7 +2076,7 @@ static int run_loop(AgdProblem *p, float alpha, float const *sched, int n_its,
   int const graph = n_its > 4 && n_its - 2 <= p->sched_cap && print_freq <= 0 &&
                     !agdg_timing_enabled() && getenv("AGD_NO_GRAPH") == NULL;
   int const chunked = p->n_chunk > 1;
-  if (chunked && n_its > 1 && forward_record(p, 1)) rc = 1;
+  if (chunked && n_its > 1 && !r_ready && forward_record(p, 1)) rc = 1;
   for (it = 1; it < n_its && rc == 0; it++) {
     if (chunked) {
       if (graph && it == 2) {
@@ -2089,7 +2097,7 @@ static int run_loop(AgdProblem *p, float alpha, float const *sched, int n_its,
       rc = run_captured(p, alpha, sched + 2, n_its - 2, 0);
       break;
     }
-    if (forward_record(p, 1)) rc = 1;
+    if (!(it == 1 && r_ready) && forward_record(p, 1)) rc = 1;
     if (rc == 0 && print_freq > 0 && it % print_freq == 0) {
       double r2 = 0.0;
       if (agdg_norm2(&p->dev, p->dev.r, p->cfg.rec_len, &r2)) rc = 1;
@@ -2123,6 +2131,25 @@ EXPORT int agdx_run_warm_dev(AgdProblem *p, float f, int n_its, int print_freq,
   return run_impl(p, f, n_its, print_freq, NULL, NULL, warm_dev);
 }
 
+/* The zero start's first update x = 0 - alpha A^H d into dev.X and its
+ * threshold lambda0 = max |x| (agdeblend.c:584-590, model.c:1202-1227). */
+static int lambda0_update(AgdProblem *p, float alpha, float *lamb) {
+  unsigned int bits = 0;
+  float l2;
+  float *tmp;
+  int rc;
+  if (agdg_zero(p->dev.scalar, 64)) return 1;
+  /* the adjoint reads its record through dev.r: point it at d */
+  tmp = p->dev.r;
+  p->dev.r = p->dev.d;
+  rc = adjoint_update(p, alpha, 0.0f, 1);
+  p->dev.r = tmp;
+  if (rc || agdg_d2h(&bits, p->dev.scalar, sizeof(bits))) return 1;
+  memcpy(&l2, &bits, sizeof(l2));
+  *lamb = sqrtf(l2);
+  return 0;
+}
+
 static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float const *x0,
                     float const *const *warm, float const *warm_dev) {
   float const decay = expf(logf(0.001f / f) / (float)n_its);
@@ -2133,65 +2160,58 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
   float const alpha = p->stepsize;
   float lamb;
   int it;
+  int const warm_any = x0 == NULL && (warm != NULL || warm_dev != NULL);
+  int zero_start = x0 == NULL && !warm_any;
+  int r_ready = 0;
+  double r2_start = -1.0;
   if (!p->have_data) {
     fprintf(stderr, "ERROR: no data set\n");
     return 1;
   }
-  if (x0 == NULL && (warm != NULL || warm_dev != NULL) && n_its <= 1) {
+  g_last_warm_fallback = 0;
+  if (warm_any && n_its <= 1) {
     /* a warm start with no iteration after it uses no threshold: skip the
      * first update that only serves to compute lambda0 */
     lamb = 0.0f;
-  } else if (x0 == NULL) {
-    unsigned int bits = 0;
-    float l2;
-    /* x = 0 - alpha * A^H d; lambda0 = max |x| */
-    if (agdg_zero(p->dev.scalar, 64)) return 1;
-    {
-      /* the adjoint reads its record through dev.r: point it at d */
-      float *tmp = p->dev.r;
-      p->dev.r = p->dev.d;
-      if (adjoint_update(p, alpha, 0.0f, 1)) {
-        p->dev.r = tmp;
-        return 1;
-      }
-      p->dev.r = tmp;
-    }
-    if (agdg_d2h(&bits, p->dev.scalar, sizeof(bits))) return 1;
-    memcpy(&l2, &bits, sizeof(l2));
-    lamb = sqrtf(l2);
   } else {
-    unsigned int bits = 0;
-    float l2;
-    float *tmp;
-    /* lambda0 from the zero-start first update (same schedule scale as the
-     * plain inversion), then the warm start replaces the model */
-    if (agdg_zero(p->dev.scalar, 64)) return 1;
-    tmp = p->dev.r;
-    p->dev.r = p->dev.d;
-    if (adjoint_update(p, alpha, 0.0f, 1)) {
-      p->dev.r = tmp;
+    /* x = 0 - alpha * A^H d; lambda0 = max |x| (for a warm start or x0 the
+     * schedule keeps the scale of the zero start) */
+    if (lambda0_update(p, alpha, &lamb)) return 1;
+    if (x0 != NULL && agdg_h2d(p->dev.X, x0, (size_t)agdx_model_len(p) * 8)) return 1;
+  }
+  if (warm_any) {
+    double d2 = 0.0;
+    if (warm != NULL ? warm_model(p, warm) : warm_model_flat(p, warm_dev)) return 1;
+    /* guard: the warm model must fit the data better than the zero model,
+     * ||A x_warm - d|| < ||d||; its residual then serves the first
+     * iteration. Otherwise the plain zero-start inversion runs instead. */
+    if (forward_record(p, 1) || agdg_norm2(&p->dev, p->dev.r, p->cfg.rec_len, &r2_start) ||
+        agdg_norm2(&p->dev, p->dev.d, p->cfg.rec_len, &d2))
       return 1;
+    if (r2_start < d2) {
+      r_ready = 1;
+    } else {
+      printf("WARNING: the warm start fits the data worse than the zero start "
+             "(residual %g >= %g); starting from zero\n",
+             sqrt(r2_start), sqrt(d2));
+      g_last_warm_fallback = 1;
+      r2_start = -1.0;
+      zero_start = 1;
+      if (lambda0_update(p, alpha, &lamb)) return 1;
     }
-    p->dev.r = tmp;
-    if (agdg_d2h(&bits, p->dev.scalar, sizeof(bits))) return 1;
-    memcpy(&l2, &bits, sizeof(l2));
-    lamb = sqrtf(l2);
-    if (agdg_h2d(p->dev.X, x0, (size_t)agdx_model_len(p) * 8)) return 1;
   }
   g_last_lambda0 = lamb;
   slope *= lamb;
   lamb *= f;
-  if (warm != NULL) {
-    if (warm_model(p, warm)) return 1; /* sets u_valid */
-  } else if (warm_dev != NULL) {
-    if (warm_model_flat(p, warm_dev)) return 1;
-  } else {
-    if (x0 == NULL && agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
+  if (zero_start) {
+    if (agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
+    p->u_valid = 0;
+  } else if (x0 != NULL) {
     p->u_valid = 0;
   }
   if (print_freq > 0) {
-    double r2 = 0.0;
-    if (agdg_norm2(&p->dev, p->dev.d, p->cfg.rec_len, &r2)) return 1;
+    double r2 = r2_start;
+    if (r2 < 0.0 && agdg_norm2(&p->dev, p->dev.d, p->cfg.rec_len, &r2)) return 1;
     printf("Iteration 0 Residual %f\n", sqrt(r2));
   }
   {
@@ -2205,7 +2225,7 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
       else lamb = (float)(n_its - 1 - it) * slope;
       sched[it] = lamb;
     }
-    rc = run_loop(p, alpha, sched, n_its, print_freq);
+    rc = run_loop(p, alpha, sched, n_its, print_freq, r_ready);
     free(sched);
     if (rc) return 1;
   }
@@ -2282,11 +2302,99 @@ static double time_spatial(AgdProblem *p, int reps) {
   return ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) / reps;
 }
 
-/* Autotuner of the spatial kernels' tile: for each specialised spatial
- * class in turn (the others at their current choice), time one
+/* Mean seconds of one iteration's time stage (forward and adjoint time
+ * transforms of every window) over reps launches. Overwrites T and V only
+ * (intermediates that every iteration recomputes). */
+static double time_time(AgdProblem *p, int reps) {
+  struct timespec t0, t1;
+  int i, rc = 0;
+  for (i = -1; i < reps && rc == 0; i++) {
+    if (i == 0) {
+      if (agdg_sync()) rc = 1;
+      clock_gettime(CLOCK_MONOTONIC, &t0);
+    }
+    if (agdg_fwd_time(&p->dev) ||
+        agdg_adj_time(&p->dev, p->use_wavelets ? p->dev.T2 : NULL, 0))
+      rc = 1;
+  }
+  if (agdg_sync()) rc = 1;
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (rc) return -1.0;
+  return ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) / reps;
+}
+
+/* Radix choice of the time transforms: for each specialised time class,
+ * every (M1, M2) factorization of its half length with a kernel is timed
+ * on one iteration's time stage and the fastest kept. */
+static int autotune_time(AgdProblem *p, int reps) {
+  int c, k;
+  for (c = 0; c < p->n_tclass; c++) {
+    AgdClass *cl = &p->tclass[c];
+    int const M = cl->p0 * cl->p1;
+    int b0 = cl->p0, b1 = cl->p1;
+    double best_t = -1.0;
+    if (cl->p0 == 0) continue;
+    /* untimed pass first: the first candidate timed on a cold device
+     * (clocks, caches, code) would read slow */
+    if (time_time(p, reps) < 0.0) return 1;
+    for (k = 0; k < (int)(sizeof(kTimeCands) / sizeof(kTimeCands[0])); k++) {
+      double t;
+      if (kTimeCands[k][0] * kTimeCands[k][1] != M) continue;
+      cl->p0 = kTimeCands[k][0];
+      cl->p1 = kTimeCands[k][1];
+      t = time_time(p, reps);
+      if (t < 0.0) return 1;
+      if (getenv("AGD_TUNE_VERBOSE"))
+        fprintf(stderr, "autotune: time class M=%d (%d x %d): %.4f ms\n", M, cl->p0, cl->p1,
+                t * 1e3);
+      if (best_t < 0.0 || t < best_t) {
+        best_t = t;
+        b0 = cl->p0;
+        b1 = cl->p1;
+      }
+    }
+    cl->p0 = b0;
+    cl->p1 = b1;
+  }
+  return 0;
+}
+
+EXPORT int agdx_n_time_classes(AgdProblem const *p) { return p->n_tclass; }
+
+EXPORT int agdx_set_time_factors(AgdProblem *p, int c, int m1, int m2) {
+  int k;
+  if (c < 0 || c >= p->n_tclass || p->tclass[c].p0 == 0 ||
+      m1 * m2 != p->tclass[c].p0 * p->tclass[c].p1) {
+    fprintf(stderr, "ERROR: no time class %d of half length %d\n", c, m1 * m2);
+    return 1;
+  }
+  for (k = 0; k < (int)(sizeof(kTimeCands) / sizeof(kTimeCands[0])); k++)
+    if (kTimeCands[k][0] == m1 && kTimeCands[k][1] == m2) {
+      p->tclass[c].p0 = m1;
+      p->tclass[c].p1 = m2;
+      return 0;
+    }
+  fprintf(stderr, "ERROR: no time kernel for %d x %d\n", m1, m2);
+  return 1;
+}
+
+EXPORT int agdx_time_factors(AgdProblem const *p, int *out) {
+  int c;
+  if (out)
+    for (c = 0; c < p->n_tclass; c++) {
+      out[2 * c] = p->tclass[c].p0;
+      out[2 * c + 1] = p->tclass[c].p1;
+    }
+  return p->n_tclass;
+}
+
+/* Autotuner: first the radix choice of the time transforms
+ * (autotune_time), then the spatial kernels' tile: for each specialised
+ * spatial class in turn (the others at their current choice), time one
  * iteration's spatial stage for every candidate planes-per-task count that
- * fits the LDS, and keep the fastest. Clears V, leaves the tasks set to the
- * choice; the model X is unchanged. nk_out (n_sclass ints, or NULL)
+ * fits the LDS, and keep the fastest. Overwrites the intermediates T and V
+ * (V is left zeroed), leaves the tasks set to the choice; the model X is
+ * unchanged. nk_out (n_sclass ints, or NULL)
  * receives the choice per class (0 for generic classes); ms_out (same) the
  * time of the chosen configuration. Returns 0, or 1 on error. */
 EXPORT int agdx_autotune(AgdProblem *p, int reps, int *nk_out, double *ms_out) {
@@ -2294,6 +2402,7 @@ EXPORT int agdx_autotune(AgdProblem *p, int reps, int *nk_out, double *ms_out) {
   int c, k;
   if (p->host_only) return 1;
   if (reps < 1) reps = 3;
+  if (autotune_time(p, reps)) return 1;
   if (p->nk_req == NULL) {
     p->nk_req = (int *)calloc((size_t)(p->n_sclass > 0 ? p->n_sclass : 1), sizeof(int));
     if (p->nk_req == NULL) return 1;
@@ -2345,7 +2454,7 @@ EXPORT int agdx_iterate(AgdProblem *p, int n, float lamb) {
   sched = (float *)malloc(((size_t)n + 1) * sizeof(float));
   if (sched == NULL) return 1;
   for (it = 0; it <= n; it++) sched[it] = lamb;
-  rc = run_loop(p, p->stepsize, sched, n + 1, 0);
+  rc = run_loop(p, p->stepsize, sched, n + 1, 0, 0);
   free(sched);
   if (rc) return 1;
   return agdg_sync();

@@ -1950,7 +1950,11 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_x3(
 #define SPATIAL_SIZES(X) \
   X(1, 10) X(1, 20) X(1, 40) X(10, 10) X(10, 20) X(10, 40) X(20, 10) X(20, 20) \
   X(20, 40) X(40, 10) X(40, 20) X(40, 40)
-#define TIME_SIZES(X) X(5, 8) X(10, 8) X(10, 16) X(20, 16)
+// (M1, M2) factorizations of the half length M = Pt / 2 with a kernel: the
+// first of each M is the default, the others are the autotuner's candidates
+#define TIME_SIZES(X)                                                            \
+  X(5, 8) X(8, 5) X(4, 10) X(10, 8) X(8, 10) X(16, 5) X(10, 16) X(16, 10) X(20, 8) \
+  X(8, 20) X(20, 16) X(16, 20) X(32, 10) X(10, 32)
 
 // units of class c within the launch's window range: [*first, *first + n)
 static inline int class_range(AgdDev const *g, AgdClass const &c, int *first) {

@@ -66,6 +66,9 @@ def load():
     fn("agdx_set_spatial_precision", ci, vp, ci)
     fn("agdx_autotune", ci, vp, ci, vp, vp)
     fn("agdx_n_spatial_classes", ci, vp)
+    fn("agdx_n_time_classes", ci, vp)
+    fn("agdx_time_factors", ci, vp, vp)
+    fn("agdx_set_time_factors", ci, vp, ci, ci, ci)
     fn("agdx_data_dev", vp, vp)
     fn("agdx_data_len", ctypes.c_int64, vp)
     fn("agdx_warm_model_dev", ci, vp, vp)
@@ -75,6 +78,7 @@ def load():
     fn("agdx_get_model", ci, vp, vp)
     fn("agdx_iterate", ci, vp, ci, cf)
     fn("agdx_last_lambda0", cf)
+    fn("agdx_last_warm_fallback", ci)
     fn("agdx_timing", ci, ci)
     fn("agdx_timing_get", ci, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
        ctypes.POINTER(ctypes.c_long))
@@ -109,10 +113,12 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
            "agdx_warm_model", "agdx_set_spatial_precision", "agdx_autotune",
-           "agdx_n_spatial_classes", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
+           "agdx_n_spatial_classes", "agdx_n_time_classes", "agdx_time_factors",
+           "agdx_set_time_factors", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
            "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
-           "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_timing",
+           "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback",
+           "agdx_timing",
            "agdx_timing_get", "agdx_bytes",
            "agdu_conv3", "agdu_conv3_cat", "agdu_conv_in", "agdu_conv_out",
            "agdu_flip_weights", "agdu_wgrad_part_len", "agdu_conv3_wgrad",
@@ -275,14 +281,28 @@ class Problem:
         return n
 
     def autotune(self, reps=3):
-        """Pick the spatial kernels' planes per task by timing; returns the
-        choice per spatial class (0: generic class) and the times (ms)."""
+        """Pick the time transforms' factorizations, then the spatial
+        kernels' planes per task, by timing; returns the spatial choice per
+        spatial class (0: generic class) and the times (ms). The time
+        choice is read with time_factors()."""
         n = int(self.L.agdx_n_spatial_classes(self.h))
         nk = np.zeros(max(n, 1), C_INT)
         ms = np.zeros(max(n, 1), np.float64)
         if self.L.agdx_autotune(self.h, int(reps), _p(nk), _p(ms)):
             raise RuntimeError("agdx_autotune failed")
         return nk[:n].tolist(), ms[:n].tolist()
+
+    def time_factors(self):
+        """Current (M1, M2) factorization of each time class ((0, 0): the
+        generic kernels)."""
+        n = int(self.L.agdx_n_time_classes(self.h))
+        out = np.zeros(2 * max(n, 1), C_INT)
+        self.L.agdx_time_factors(self.h, _p(out))
+        return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(n)]
+
+    def set_time_factors(self, c, m1, m2):
+        if self.L.agdx_set_time_factors(self.h, int(c), int(m1), int(m2)):
+            raise ValueError("no time kernel %d x %d for class %d" % (m1, m2, c))
 
     def data_dev(self):
         """Device address of the patch data set by set_data (flat, patch
@@ -340,3 +360,8 @@ def timing_get(name):
 
 def last_lambda0():
     return float(load().agdx_last_lambda0())
+
+
+def last_warm_fallback():
+    """True when the last warm-started run fell back to the zero start."""
+    return bool(load().agdx_last_warm_fallback())

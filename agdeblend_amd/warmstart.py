@@ -366,10 +366,23 @@ def estimate(pm, net, out=None):
     return out
 
 
+def accepts(shapes):
+    """Whether the network can estimate patches of these shapes: 3-D, every
+    size a multiple of 4 (two 2x2x2 poolings)."""
+    return all(len(s) == 3 and all(int(v) % 4 == 0 for v in s) for s in shapes)
+
+
 def deblend_warm(pm, net, initial_factor, n_its, out=None):
     """The inversion of pm started from the network's estimate: estimate on
     the device, then agdx_run_warm_dev (the reference's loop from that
-    initial model). Returns the estimate tensor."""
+    initial model). The plain zero-start inversion is the fallback: it runs
+    instead when net is None or cannot take pm's patches (then None is
+    returned), and the library runs it when the estimate's model fits the
+    data no better than the zero model (lib.last_warm_fallback() tells).
+    Returns the estimate tensor."""
+    if net is None or not accepts(pm.shapes):
+        pm.run(initial_factor, n_its)
+        return None
     est = estimate(pm, net, out)
     torch.cuda.synchronize(net.dev)
     pm.run_warm_dev(initial_factor, n_its, est.data_ptr())

@@ -281,10 +281,12 @@ def main():
     if not args.no_autotune:
         t0 = time.perf_counter()
         nk, ms = pm.autotune(reps=3)
+        tf = pm.time_factors()
         tune = dict(planes_per_task=nk, spatial_ms=[round(t, 4) for t in ms],
+                    time_factors=["%dx%d" % f for f in tf],
                     tuning_s=round(time.perf_counter() - t0, 3))
-        log("autotune: planes per spatial task %s (%s ms), %.2f s" %
-            (nk, ", ".join("%.3f" % t for t in ms), tune["tuning_s"]))
+        log("autotune: time factorizations %s; planes per spatial task %s (%s ms), %.2f s" %
+            (tune["time_factors"], nk, ", ".join("%.3f" % t for t in ms), tune["tuning_s"]))
     n_windows = pm.n_windows
     by = pm.bytes()
     b_fk = 3 * by["M"] + 2 * by["T"]

@@ -93,7 +93,11 @@ int agdx_run(AgdProblem *p, float initial_factor, int n_its, int print_freq,
  * learned or other warm start plugs into.) */
 int agdx_warm_model(AgdProblem *p, float const *const *y);
 /* agdx_run starting from agdx_warm_model(warm) instead of the thresholded
- * first update; lambda0 and the schedule are those of the zero start. */
+ * first update; lambda0 and the schedule are those of the zero start.
+ * Guarded: when the warm model fits the data no better than the zero
+ * model (||A x_warm - d|| >= ||d||, FP64 norms) the plain zero-start
+ * inversion runs instead, with a WARNING on stdout, and
+ * agdx_last_warm_fallback() returns 1. */
 int agdx_run_warm(AgdProblem *p, float initial_factor, int n_its, int print_freq,
                   float const *const *warm);
 /* Spatial DFT precision: 0 FP32 register FFTs (default), 1 split-precision
@@ -101,15 +105,27 @@ int agdx_run_warm(AgdProblem *p, float initial_factor, int n_its, int print_freq
  * other windows keep FP32). Returns the number of window classes served by
  * the bf16 x 3 path, -1 on error. */
 int agdx_set_spatial_precision(AgdProblem *p, int mode);
-/* Autotuner of the spatial kernels' tile (planes per task): per
- * specialised spatial class, every candidate that fits the LDS is timed on
- * one iteration's spatial stage (reps launches) and the fastest kept.
- * Clears the adjoint intermediate; the model is unchanged. nk_out and
- * ms_out (one entry per spatial class, or NULL) receive the choice (0 for
- * generic classes) and its time per iteration. */
+/* Autotuner. First the radix choice of the time transforms: per
+ * specialised time class, every (M1, M2) factorization of the half length
+ * Pt / 2 with a kernel (e.g. 160 = 10 x 16, 16 x 10, 20 x 8, 8 x 20) is
+ * timed on one iteration's time stage and the fastest kept. Then the
+ * spatial kernels' tile (planes per task): per specialised spatial class,
+ * every candidate that fits the LDS is timed on one iteration's spatial
+ * stage (reps launches) and the fastest kept. Overwrites the intermediates;
+ * the model is unchanged. nk_out and ms_out (one entry per spatial class,
+ * or NULL) receive the spatial choice (0 for generic classes) and its time
+ * per iteration. */
 int agdx_autotune(AgdProblem *p, int reps, int *nk_out, double *ms_out);
 /* Number of spatial kernel classes of the problem. */
 int agdx_n_spatial_classes(AgdProblem const *p);
+/* Number of time kernel classes, and their current (M1, M2) factorization
+ * (out: 2 ints per class, or NULL; 0 0 for classes of the generic
+ * kernels). agdx_time_factors returns the number of classes. */
+int agdx_n_time_classes(AgdProblem const *p);
+int agdx_time_factors(AgdProblem const *p, int *out);
+/* Sets time class c to the factorization m1 x m2 of its half length (one
+ * of the autotuner's candidates). */
+int agdx_set_time_factors(AgdProblem *p, int c, int m1, int m2);
 /* Device copy of the patch data set by agdx_set_data: all patches
  * concatenated in patch order, agdx_data_len floats (valid until
  * agdx_warm_model, agdx_finish or agdx_get_output overwrite it). */
@@ -134,6 +150,8 @@ int agdx_get_model(AgdProblem *p, float *x);
 int agdx_iterate(AgdProblem *p, int n_iterations, float lamb);
 /* Threshold lambda0 of the last agdx_run (first update's max |x|). */
 float agdx_last_lambda0(void);
+/* 1 when the last warm-started run fell back to the zero start, else 0. */
+int agdx_last_warm_fallback(void);
 /* Per-kernel device times accumulated while timing is enabled. */
 int agdx_timing(int enable);
 int agdx_timing_get(char const *kernel, double *ms, long *count);

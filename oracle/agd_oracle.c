@@ -1402,8 +1402,10 @@ static void model_from_data(omodel const *m, real const *const *y,
 
 /* agdeblend.c:511-649. stats receives the first threshold and stepsize
  * when non-NULL. With warm (patch data, the layout of data) the start model
- * is model_from_data(warm) instead of the thresholded first update; the
- * threshold schedule still starts from the first update's max |x|. */
+ * is model_from_data(warm) instead of the thresholded first update, unless
+ * that model's residual is no smaller than the zero model's (then the zero
+ * start; stats[2] = 1 in that case, else 0); the threshold schedule still
+ * starts from the first update's max |x|. */
 /* Threshold schedule (agdeblend.c:526-533, :623-627): lamb decays by
  * `decay` per iteration until first_linear_it, then falls linearly to zero
  * at the last iteration; slope is per unit of the first threshold. */
@@ -1455,7 +1457,29 @@ static int deblend_impl(int np, int const *volumes, int const *n_dims,
   slope *= lamb;
   lamb *= f;
   if (warm != NULL) {
+    /* guard: keep the warm model only when it fits the data better than
+     * the zero model, ||A x_warm - d|| < ||d|| (FP64 sums); otherwise the
+     * zero start's thresholded first update */
+    double r2 = 0, d2 = 0;
+    size_t i;
+    int fallback;
     model_from_data(m, warm, shape, x);
+    model_forward(m, x, r);
+    for (i = 0; i < m->cfg.rec_len; i++) {
+      double const ri = (double)(r[i] - d[i]);
+      r2 += ri * ri;
+      d2 += (double)d[i] * d[i];
+    }
+    fallback = !(r2 < d2);
+    if (fallback) {
+      printf("WARNING: the warm start fits the data worse than the zero start "
+             "(residual %g >= %g); starting from zero\n",
+             sqrt(r2), sqrt(d2));
+      memset(x, 0, (m->model_len + 1) * sizeof(ocpx));
+      model_adjoint_update(m, d, step, x);
+      model_shrink(m, lamb, x);
+    }
+    if (stats) stats[2] = (double)fallback;
   } else {
     model_shrink(m, lamb, x);
   }

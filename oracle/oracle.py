@@ -222,8 +222,10 @@ def deblend(volumes, n_dims, window_shapes, coords, shapes, shottimes, channels,
             wavelet_lengths=None, wavelet_idxs=None, wavelets=None,
             double=False, threads=None, warm=None):
     """Run the oracle's agd_deblend. Returns (list of output arrays, stats)
-    where stats = (initial threshold lambda0, stepsize). warm: optional data
-    estimate (patch arrays) whose model_from_data replaces the start model."""
+    where stats = (initial threshold lambda0, stepsize) and, with warm,
+    a third entry: 1 when the warm start fell back to the zero start. warm:
+    optional data estimate (patch arrays) whose model_from_data replaces the
+    start model when it fits the data better than the zero model."""
     dt = _rdt(double)
     L = lib(double)
     if threads is not None:
@@ -232,7 +234,7 @@ def deblend(volumes, n_dims, window_shapes, coords, shapes, shottimes, channels,
                         channels, trace_types, wavelet_lengths, wavelet_idxs,
                         wavelets, double)
     d = PtrTable([np.array(x, dtype=dt, copy=True) for x in data], dt, writable=True)
-    stats = np.zeros(2, np.float64)
+    stats = np.zeros(3 if warm is not None else 2, np.float64)
     if warm is not None:
         wt = PtrTable([np.asarray(x, dtype=dt).reshape(-1) for x in warm], dt)
         rc = L.deblend_warm(*args.head(), initial_factor, n_its, print_freq, d.ptr,
