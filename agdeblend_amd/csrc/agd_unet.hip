@@ -87,34 +87,54 @@ __device__ __forceinline__ void st_swz(uint16_t *base, int row, int half, uint4 
   *(uint2 *)(base + swz_q(row, 2 * half + 1)) = make_uint2(v.z, v.w);
 }
 
-template <int CO, int CU>
+template <int CO, int CU, bool UPC = false>
 struct ConvDepth {
   // depth planes per block: the ring saves input reads, the accumulators
   // (DB x 4 x CO / 16 tiles) cost registers
-  static constexpr int DB = CO <= 16 ? (CU ? 2 : 4) : (CO <= 32 ? 2 : 1);
+  static constexpr int DB = CO <= 16 ? ((CU && !UPC) ? 2 : 4) : (CO <= 32 ? 2 : 1);
 };
+
+// Combined taps of a 3x3x3 convolution over a nearest x2 upsampled input.
+// Along one axis, output index o = 2 c + p reads input o + k - 1 (k = 0, 1,
+// 2), i.e. coarse c + floor((p + k - 1) / 2): for p = 0 the coarse offsets
+// are -1 (k = 0) and 0 (k = 1, 2); for p = 1 they are 0 (k = 0, 1) and +1
+// (k = 2). So per output parity the three taps become two, taps 0 and 1+2
+// (p = 0) or 0+1 and 2 (p = 1), and the two read the input rows o - 1 + p
+// and o + p. Applied along d and h (whose parities are uniform per plane
+// and per row of a block), the 27 taps become 4 (a, b) pairs x 3 w taps for
+// each of the 4 (pd, ph) parities: 48 combined taps, entry
+// ((pd * 2 + ph) * 4 + a * 2 + b) * 3 + kw.
+__host__ __device__ constexpr int up_tap(int pd, int ph, int a, int b, int kw) {
+  return (((pd * 2 + ph) * 4 + a * 2 + b) * 3) + kw;
+}
 
 // CU > 0: the input is the concatenation [upsample(xu), x] without being
 // formed: input channels [0, CU) are xu's ([n][D/2][H/2][W/2][CU], nearest
 // x2 upsampling, D, H, W even) and [CU, CI) are x's ([n][D][H][W][CI - CU]).
-template <int CI, int CO, int CU>
+// UPC: the xu channels use the 48 combined taps wcu ([48][CO][CU], built by
+// k_up_weights from wt) instead of wt's 27, 12 matrix-core steps per
+// 16-channel chunk instead of 27.
+template <int CI, int CO, int CU, bool UPC = false>
 __global__ __launch_bounds__(UT) void k_conv3(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ xu,
-    uint16_t const *__restrict__ wt, float const *__restrict__ bias,
-    uint16_t const *__restrict__ gate, uint16_t *__restrict__ y, int D, int H, int W,
-    int mode) {
+    uint16_t const *__restrict__ wt, uint16_t const *__restrict__ wcu,
+    float const *__restrict__ bias, uint16_t const *__restrict__ gate,
+    uint16_t *__restrict__ y, int D, int H, int W, int mode) {
   static_assert(CI % 16 == 0 && CO % 16 == 0 && CU % 16 == 0 && CU < CI,
                 "channels in multiples of 16");
+  static_assert(!UPC || CU > 0, "combined taps serve the upsampled channels");
   constexpr int CS = CI - CU;  // channels of x
   constexpr int NT = CO / 16;
-  constexpr int DB = ConvDepth<CO, CU>::DB;
+  constexpr int DB = ConvDepth<CO, CU, UPC>::DB;
   constexpr int PW = CT_W + 2, PH = CT_H + 2;
   constexpr int PLANE = PH * PW * 16;  // bf16 elements of one staged plane
   constexpr int NPI = PH * PW * 2;     // staging items (8 channels) per plane
   constexpr int NWI = 27 * CO * 2;
+  constexpr int NWU = 48 * CO * 2;     // staging items of the combined taps
+  constexpr int NTW = UPC ? 48 : 27;   // taps the weight region holds
   constexpr int OBS = CO + 4;  // output staging row: CO channels + 4 (bank spread)
-  constexpr int SMEM = 3 * PLANE + 27 * CO * 16 > 4 * 64 * OBS ? 3 * PLANE + 27 * CO * 16
-                                                                : 4 * 64 * OBS;
+  constexpr int SMEM = 3 * PLANE + NTW * CO * 16 > 4 * 64 * OBS ? 3 * PLANE + NTW * CO * 16
+                                                                 : 4 * 64 * OBS;
   __shared__ __align__(16) uint16_t smem[SMEM];
   uint16_t *const sx = smem;               // ring of 3 input planes
   uint16_t *const sw = smem + 3 * PLANE;   // weights of the chunk
@@ -191,13 +211,20 @@ __global__ __launch_bounds__(UT) void k_conv3(
       load_plane(d0 - 1, c, v0);
       load_plane(d0, c, v1);
       load_plane(d0 + 1, c, v2);
+      // the chunk's taps: wt's 27 ([27][CO][CI]) or, for the upsampled
+      // channels with UPC, the 48 combined ones ([48][CO][CU])
+      bool const upc = UPC && c < CU / 16;
+      int const nw = upc ? NWU : NWI;
+      uint16_t const *const wsrc = upc ? wcu : wt;
+      int const wstride = upc ? CU : CI;
+      constexpr int NWK = ((UPC ? NWU : NWI) + UT - 1) / UT;
 #pragma unroll
-      for (int k = 0; k < (NWI + UT - 1) / UT; ++k) {
+      for (int k = 0; k < NWK; ++k) {
         int const e0 = tid + k * UT;
-        int const e = e0 < NWI ? e0 : NWI - 1;
+        int const e = e0 < nw ? e0 : nw - 1;
         int const half = e & 1;
         int const tc = e >> 1;  // tap * CO + co
-        uint4 const v = *(uint4 const *)(wt + (int64_t)tc * CI + c * 16 + half * 8);
+        uint4 const v = *(uint4 const *)(wsrc + (int64_t)tc * wstride + c * 16 + half * 8);
         st_swz(sw, tc, half, v);  // past the end: the last item again
       }
       store_plane(d0 - 1, v0);
@@ -219,8 +246,17 @@ __global__ __launch_bounds__(UT) void k_conv3(
         bf16x4 a[3][4];
         bf16x4 b[3][NT];
       };
-      auto load_ops = [&](int kdh, Ops &o) {
-        int const kd = kdh / 3, kh = kdh - kd * 3;
+      // row r of taps: (kd, kh) = (r / 3, r % 3) over 9 rows, or with the
+      // combined taps (a, b) = (r / 2, r % 2) over 4 rows, which read the
+      // fine plane d - 1 + pd + a and fine row h - 1 + ph + b
+      bool const upc = UPC && c < CU / 16;
+      int const nrow = upc ? 4 : 9;
+      int const pd = d & 1, ph = (h0 + wave) & 1;
+      auto load_ops = [&](int r, Ops &o) {
+        int const a = upc ? r >> 1 : r / 3;
+        int const b = upc ? r & 1 : r - 3 * (r / 3);
+        int const kd = upc ? pd + a : a, kh = upc ? ph + b : b;
+        int const wrow = upc ? up_tap(pd, ph, a, b, 0) : r * 3;
         uint16_t const *const pl = sx + ((d + kd - 1 + 3) % 3) * PLANE;
         int const v0 = (wave + kh) * PW + i;
 #pragma unroll
@@ -228,7 +264,7 @@ __global__ __launch_bounds__(UT) void k_conv3(
 #pragma unroll
           for (int m = 0; m < 4; ++m) o.a[kw][m] = ld_swz(pl, v0 + 16 * m + kw, kq);
 #pragma unroll
-          for (int t = 0; t < NT; ++t) o.b[kw][t] = ld_swz(sw, (kdh * 3 + kw) * CO + t * 16 + i, kq);
+          for (int t = 0; t < NT; ++t) o.b[kw][t] = ld_swz(sw, (wrow + kw) * CO + t * 16 + i, kq);
         }
       };
       auto mma_ops = [&](Ops const &o) {
@@ -244,13 +280,13 @@ __global__ __launch_bounds__(UT) void k_conv3(
       Ops opa, opb;
       load_ops(0, opa);
 #pragma unroll 1
-      for (int p = 0; p < 4; ++p) {
-        load_ops(2 * p + 1, opb);
+      for (int r = 1; r < nrow; r += 2) {
+        load_ops(r, opb);
         mma_ops(opa);
-        load_ops(2 * p + 2, opa);
+        load_ops(r + 1 < nrow ? r + 1 : r, opa);  // even nrow: the last row again, unused
         mma_ops(opb);
       }
-      mma_ops(opa);
+      if (nrow & 1) mma_ops(opa);
       if (j + 1 < DB) {
         __syncthreads();  // every wave is done with plane d - 1
         store_plane(d + 2, nxt);
@@ -311,6 +347,33 @@ __global__ __launch_bounds__(UT) void k_conv3(
         }
       }
     }
+  }
+}
+
+// Combined taps of the first CU input channels of wt ([27][CO][CI]):
+// wcu[up_tap(pd, ph, a, b, kw)][co][cu] = bf16(sum over the d taps kd of
+// (pd, a) and the h taps kh of (ph, b) of wt[(kd * 3 + kh) * 3 + kw][co][cu]),
+// summed in fp32 in tap order. Taps of parity p and pair index a: p = 0:
+// {0}, {1, 2}; p = 1: {0, 1}, {2}.
+__global__ __launch_bounds__(UT) void k_up_weights(uint16_t const *__restrict__ wt,
+                                                   uint16_t *__restrict__ wcu, int CI, int CO,
+                                                   int CU) {
+  int const n = 48 * CO * CU;
+  for (int e = blockIdx.x * UT + threadIdx.x; e < n; e += gridDim.x * UT) {
+    int const cu = e % CU;
+    int const co = (e / CU) % CO;
+    int const tap = e / (CU * CO);
+    int const kw = tap % 3, ab = (tap / 3) % 4, pp = tap / 12;
+    int const pd = pp >> 1, ph = pp & 1, a = ab >> 1, b = ab & 1;
+    int const dlo = (pd == 0) ? (a == 0 ? 0 : 1) : (a == 0 ? 0 : 2);
+    int const dhi = (pd == 0) ? (a == 0 ? 0 : 2) : (a == 0 ? 1 : 2);
+    int const hlo = (ph == 0) ? (b == 0 ? 0 : 1) : (b == 0 ? 0 : 2);
+    int const hhi = (ph == 0) ? (b == 0 ? 0 : 2) : (b == 0 ? 1 : 2);
+    float acc = 0.f;
+    for (int kd = dlo; kd <= dhi; ++kd)
+      for (int kh = hlo; kh <= hhi; ++kh)
+        acc += bf2f(wt[((int64_t)((kd * 3 + kh) * 3 + kw) * CO + co) * CI + cu]);
+    wcu[e] = f2bf(acc);
   }
 }
 
@@ -1022,22 +1085,24 @@ static int conv_shape_ok(int N, int D, int H, int W, int CI, int CO) {
   return 1;
 }
 
-#define CONV_LAUNCH(a, b, u)                                                                \
+#define CONV_LAUNCH(a, b, u, p)                                                             \
   {                                                                                        \
-    int const db = ConvDepth<b, u>::DB;                                                    \
+    int const db = ConvDepth<b, u, p>::DB;                                                 \
     dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H),  \
                     (unsigned)(N * ((D + db - 1) / db)));                                  \
-    return ulaunch("conv3", k_conv3<a, b, u>, grid, dim3(UT), stream, (uint16_t const *)x,   \
-                   (uint16_t const *)xu, (uint16_t const *)wt, bias, (uint16_t const *)gate, \
-                   (uint16_t *)y, D, H, W, mode);                                          \
+    return ulaunch("conv3", k_conv3<a, b, u, p>, grid, dim3(UT), stream,                   \
+                   (uint16_t const *)x, (uint16_t const *)xu, (uint16_t const *)wt,         \
+                   (uint16_t const *)wcu, bias, (uint16_t const *)gate, (uint16_t *)y, D, H, \
+                   W, mode);                                                                \
   }
 
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
   void const *const xu = NULL;
+  void const *const wcu = NULL;
   if (!conv_shape_ok(N, D, H, W, CI, CO)) return 1;
 #define CC(a, b) \
-  if (CI == a && CO == b) CONV_LAUNCH(a, b, 0)
+  if (CI == a && CO == b) CONV_LAUNCH(a, b, 0, false)
   CONV_CASES(CC)
 #undef CC
   fprintf(stderr, "ERROR: no convolution kernel for %d -> %d channels\n", CI, CO);
@@ -1049,6 +1114,7 @@ int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gat
 int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *bias,
                    void const *gate, void *y, int N, int D, int H, int W, int CU, int CS,
                    int CO, int mode, void *stream) {
+  void const *const wcu = NULL;
   if (!conv_shape_ok(N, D, H, W, CU + CS, CO)) return 1;
   if ((D | H | W) & 1) {
     fprintf(stderr, "ERROR: upsampled concatenation needs even sizes, got %d x %d x %d\n", D,
@@ -1056,7 +1122,40 @@ int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *b
     return 1;
   }
 #define CT(u, c, o) \
-  if (CU == u && CS == c && CO == o) CONV_LAUNCH(u + c, o, u)
+  if (CU == u && CS == c && CO == o) CONV_LAUNCH(u + c, o, u, false)
+  CAT_CASES(CT)
+#undef CT
+  fprintf(stderr, "ERROR: no concatenating convolution for %d + %d -> %d channels\n", CU, CS,
+          CO);
+  return 1;
+}
+
+int agdu_up_weights(void const *wt, void *wcu, int CI, int CO, int CU, void *stream) {
+  if (CU < 16 || CU % 16 || CU >= CI || CO < 16 || CO % 16) {
+    fprintf(stderr, "ERROR: combined taps for %d of %d -> %d channels unsupported\n", CU, CI,
+            CO);
+    return 1;
+  }
+  int64_t const n = (int64_t)48 * CO * CU;
+  return ulaunch("up_weights", k_up_weights, dim3((unsigned)blocks_capped(n, UT)), dim3(UT),
+                 stream, (uint16_t const *)wt, (uint16_t *)wcu, CI, CO, CU);
+}
+
+int agdu_conv3_up(void const *xu, void const *x, void const *wt, void const *wcu,
+                  float const *bias, void const *gate, void *y, int N, int D, int H, int W,
+                  int CU, int CS, int CO, int mode, void *stream) {
+  if (!conv_shape_ok(N, D, H, W, CU + CS, CO)) return 1;
+  if ((D | H | W) & 1) {
+    fprintf(stderr, "ERROR: upsampled concatenation needs even sizes, got %d x %d x %d\n", D,
+            H, W);
+    return 1;
+  }
+  if (wcu == NULL) {
+    fprintf(stderr, "ERROR: combined taps missing\n");
+    return 1;
+  }
+#define CT(u, c, o) \
+  if (CU == u && CS == c && CO == o) CONV_LAUNCH(u + c, o, u, true)
   CAT_CASES(CT)
 #undef CT
   fprintf(stderr, "ERROR: no concatenating convolution for %d + %d -> %d channels\n", CU, CS,

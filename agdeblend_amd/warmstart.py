@@ -48,6 +48,14 @@ def _vp(t):
 class UNet:
     """Weights, optimizer state and activation buffers on one HIP device."""
 
+    # decoder layers whose upsampled channels run on the 48 combined taps in
+    # the inference pass (agdu_conv3_up: 12 instead of 27 matrix-core steps
+    # per 16 of those channels, combined weights rounded once to bf16); the
+    # others read the concatenation in place with all 27 taps. d2a stays on
+    # 27: its combined taps (48 x 32 x 64) leave room for one block per CU
+    # only, and it measures slower.
+    UP_LAYERS = ("d1a",)
+
     def __init__(self, seed=0, device="cuda", head_scale=0.1):
         if not torch.cuda.is_available():
             raise RuntimeError(
@@ -75,6 +83,8 @@ class UNet:
         self.g = torch.zeros(n, device=self.dev)
         self.step_count = 0
         self._bufs = {}
+        self.up_layers = set(self.UP_LAYERS)
+        self._wcu = {}
 
     # -- parameters ---------------------------------------------------------
     def _set_params(self, w):
@@ -239,15 +249,33 @@ class UNet:
                "agdu_maxpool")
         self._conv("b1", B["p2"], B["b1"], d2, 3)
         self._conv("b2", B["b1"], B["b2"], d2, 3)
-        _check(L.agdu_conv3_cat(_vp(B["b2"]), _vp(B["e2b"]), self._wptr("d2a"), self._bptr("d2a"),
-                                None, _vp(B["d2a"]), *d1, 64, 32, 32, 3, s), "agdu_conv3_cat d2a")
+        self._conv_cat("d2a", B["b2"], B["e2b"], B["d2a"], d1, 64, 32, 32)
         self._conv("d2b", B["d2a"], B["d2b"], d1, 3)
-        _check(L.agdu_conv3_cat(_vp(B["d2b"]), _vp(B["e1b"]), self._wptr("d1a"), self._bptr("d1a"),
-                                None, _vp(B["d1a"]), *d0, 32, 16, 16, 3, s), "agdu_conv3_cat d1a")
+        self._conv_cat("d1a", B["d2b"], B["e1b"], B["d1a"], d0, 32, 16, 16)
         self._conv("d1b", B["d1a"], B["d1b"], d0, 3)
         _check(L.agdu_conv_out(_vp(B["d1b"]), self._wptr("head"), 16, self._bptr("head"), src,
                                _vp(out), *d0, 1.0 / self.scale, s), "agdu_conv_out")
         return out
+
+    def _conv_cat(self, name, xu, xs, y, dims, cu, cs, co):
+        """Decoder layer on [upsample(xu), xs] (bias, ReLU), with combined
+        taps for the upsampled channels when name is in up_layers (built
+        from the current bf16 weights at every call: two small launches)."""
+        s = self._s()
+        if name in self.up_layers:
+            if name not in self._wcu:
+                self._wcu[name] = torch.empty(48 * co * cu, dtype=torch.bfloat16,
+                                              device=self.dev)
+            wcu = _vp(self._wcu[name])
+            _check(self.L.agdu_up_weights(self._wptr(name), wcu, cu + cs, co, cu, s),
+                   "agdu_up_weights " + name)
+            _check(self.L.agdu_conv3_up(_vp(xu), _vp(xs), self._wptr(name), wcu,
+                                        self._bptr(name), None, _vp(y), *dims, cu, cs, co, 3,
+                                        s), "agdu_conv3_up " + name)
+        else:
+            _check(self.L.agdu_conv3_cat(_vp(xu), _vp(xs), self._wptr(name), self._bptr(name),
+                                         None, _vp(y), *dims, cu, cs, co, 3, s),
+                   "agdu_conv3_cat " + name)
 
     def predict(self, field, out=None):
         """predict_ptr on a float32 device tensor [N, D, H, W]."""

@@ -30,6 +30,19 @@ int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gat
 int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *bias,
                    void const *gate, void *y, int N, int D, int H, int W, int CU, int CS,
                    int CO, int mode, void *stream);
+/* Combined taps of the first CU input channels of wt ([27][CO][CI]) for
+ * agdu_conv3_up: wcu is [48][CO][CU] bf16. A 3x3x3 convolution over a
+ * nearest x2 upsampled input reads, along each of d and h, only two
+ * coarse rows per output parity; the weights of taps that read the same
+ * coarse row are summed (fp32, rounded once to bf16). */
+int agdu_up_weights(void const *wt, void *wcu, int CI, int CO, int CU, void *stream);
+/* agdu_conv3_cat with the xu channels on the combined taps wcu
+ * (agdu_up_weights of wt): 12 instead of 27 matrix-core steps per 16 of
+ * those channels. Equal to agdu_conv3_cat up to the rounding of the
+ * combined weights to bf16. */
+int agdu_conv3_up(void const *xu, void const *x, void const *wt, void const *wcu,
+                  float const *bias, void const *gate, void *y, int N, int D, int H, int W,
+                  int CU, int CS, int CO, int mode, void *stream);
 /* First layer from a one-channel fp32 field:
  * y[v][co] = ReLU(bias[co] + sum_tap wt[tap][co][0] bf16(src[v + tap] scale)),
  * 16 output channels, wt [27][16][wci] (input channel 0 is used) */

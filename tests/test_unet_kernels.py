@@ -225,6 +225,40 @@ def test_conv3_cat_equals_formed_concatenation(cu, cs, co):
     assert torch.equal(y1, y2)
 
 
+@pytest.mark.parametrize("cu,cs,co", [(32, 16, 16), (64, 32, 32)])
+def test_conv3_up_combined_taps(cu, cs, co):
+    """The combined taps are bit for bit the float32 sums of the bf16
+    weights rounded once; the convolution on them matches the FP64
+    reference of the same arithmetic, and the formed concatenation's
+    convolution up to the rounding of the combined taps."""
+    rng = np.random.default_rng(3 * cu + cs)
+    N, D, H, W = 1, 6, 8, 70
+    xu = _bf(rng.standard_normal((N, D // 2, H // 2, W // 2, cu)))
+    xs = _bf(rng.standard_normal((N, D, H, W, cs)))
+    w = _bf(rng.standard_normal((27, co, cu + cs)) / np.sqrt(27 * (cu + cs)))
+    b = rng.standard_normal(co).astype(np.float32)
+    b_d = torch.tensor(b, device="cuda")
+    L = _lib()
+    wcu = torch.empty((48, co, cu), dtype=torch.bfloat16, device="cuda")
+    assert L.agdu_up_weights(_p(w), _p(wcu), cu + cs, co, cu, None) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(wcu), R.up_weights(_np(w), cu))
+    y = torch.empty((N, D, H, W, co), dtype=torch.bfloat16, device="cuda")
+    y2 = torch.empty_like(y)
+    assert L.agdu_conv3_up(_p(xu), _p(xs), _p(w), _p(wcu), _p(b_d), None, _p(y), N, D, H, W,
+                           cu, cs, co, 3, None) == 0
+    assert L.agdu_conv3_cat(_p(xu), _p(xs), _p(w), _p(b_d), None, _p(y2), N, D, H, W, cu, cs,
+                            co, 3, None) == 0
+    torch.cuda.synchronize()
+    ref = R.conv3_up(_np(xu), _np(xs), _np(w), _np(wcu), b.astype(np.float64), relu=True)
+    _close(_np(y), ref, 1e-2)
+    rel = np.linalg.norm(_np(y) - _np(y2)) / np.linalg.norm(_np(y2))
+    assert rel <= 1e-2, rel
+    # no kernel for other channel counts, odd sizes refused
+    assert L.agdu_conv3_up(_p(xu), _p(xs), _p(w), _p(wcu), _p(b_d), None, _p(y), N, D, H, W - 1,
+                           cu, cs, co, 3, None) == 1
+
+
 def test_first_layer_and_head_kernels():
     rng = np.random.default_rng(21)
     N, D, H, W = 1, 3, 5, 72  # the head needs W % 4 == 0

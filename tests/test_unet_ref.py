@@ -90,3 +90,28 @@ def test_network_gradients_match_finite_differences():
                 fd = (lp - lm) / (2 * h)
                 an = G[name][which].reshape(-1)[k]
                 assert abs(fd - an) <= 1e-5 * max(abs(fd), 1e-3), (name, which, fd, an)
+
+
+def test_combined_taps_equal_the_upsampled_convolution():
+    """A 3x3x3 convolution of [upsample(xu), xs] equals the xs part plus the
+    48 combined taps over xu (exact FP64 sums: an identity), for every
+    output parity and at the borders; rounded combined taps are the float32
+    sums of the bf16 weights, rounded once."""
+    rng = np.random.default_rng(17)
+    n, D, H, W, cu, cs, co = 1, 6, 4, 10, 3, 2, 4
+    xu = rng.standard_normal((n, D // 2, H // 2, W // 2, cu))
+    xs = rng.standard_normal((n, D, H, W, cs))
+    w = rng.standard_normal((27, co, cu + cs))
+    b = rng.standard_normal(co)
+    cat = np.concatenate([R.upsample(xu), xs], axis=-1)
+    ref = R.conv3(cat, w, b, relu=True)
+    got = R.conv3_up(xu, xs, w, R.up_weights(w, cu, rounded=False), b, relu=True)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # a plausible slip (the parity sets swapped) breaks it
+    wrong = R.up_weights(w[::-1].copy(), cu, rounded=False)
+    assert np.abs(R.conv3_up(xu, xs, w, wrong, b) - R.conv3(cat, w, b)).max() > 1e-3
+    wb = R.bf16_round(w)
+    wr = R.up_weights(wb, cu)
+    assert np.array_equal(wr, R.bf16_round(wr))
+    assert np.abs(wr - R.up_weights(wb, cu, rounded=False)).max() <= 2 ** -8 * np.abs(wr).max()
+

@@ -31,6 +31,68 @@ def conv3(x, w, b=None, relu=False):
     return y
 
 
+# Combined taps over a nearest x2 upsampled input: along one axis, output
+# parity p and pair index a read the fine row o - 1 + p + a, which carries
+# the taps UP_SETS[p, a] of the 3-tap kernel.
+UP_SETS = {(0, 0): (0,), (0, 1): (1, 2), (1, 0): (0, 1), (1, 1): (2,)}
+
+
+def up_tap(pd, ph, a, b, kw):
+    return ((pd * 2 + ph) * 4 + a * 2 + b) * 3 + kw
+
+
+def up_weights(w, cu, rounded=True):
+    """[48][co][cu] combined taps of the first cu input channels of w
+    [27][co][ci]: per (pd, ph, a, b, kw) the sum over kd in UP_SETS[pd, a]
+    and kh in UP_SETS[ph, b] of w[(kd * 3 + kh) * 3 + kw], in float32 in that
+    order and rounded to bf16 (the kernel's arithmetic), or exact in FP64
+    with rounded=False."""
+    co = w.shape[1]
+    out = np.zeros((48, co, cu))
+    for pd in range(2):
+        for ph in range(2):
+            for a in range(2):
+                for b in range(2):
+                    for kw in range(3):
+                        taps = [(kd * 3 + kh) * 3 + kw for kd in UP_SETS[pd, a]
+                                for kh in UP_SETS[ph, b]]
+                        if rounded:
+                            acc = np.zeros((co, cu), np.float32)
+                            for t in taps:
+                                acc = (acc + np.asarray(w[t][:, :cu], np.float32)).astype(
+                                    np.float32)
+                            out[up_tap(pd, ph, a, b, kw)] = bf16_round(acc)
+                        else:
+                            out[up_tap(pd, ph, a, b, kw)] = sum(
+                                np.asarray(w[t][:, :cu], np.float64) for t in taps)
+    return out
+
+
+def conv3_up(xu, xs, w, wcu, b=None, relu=False):
+    """conv3 of [upsample(xu), xs] (D, H, W even) with xu's channels on the
+    combined taps wcu (up_weights) and xs's on w's."""
+    n, D, H, W, cs = xs.shape
+    cu = xu.shape[-1]
+    y = conv3(xs, np.asarray(w)[:, :, cu:])
+    up = np.zeros((n, D + 2, H + 2, W + 2, cu))
+    up[:, 1:-1, 1:-1, 1:-1] = upsample(xu)
+    for pd in range(2):
+        for ph in range(2):
+            for ia in range(2):
+                for ib in range(2):
+                    for kw in range(3):
+                        # outputs d = pd + 2 k read padded plane d + pd + ia
+                        xs_ = up[:, 2 * pd + ia:2 * pd + ia + D:2,
+                                 2 * ph + ib:2 * ph + ib + H:2, kw:kw + W]
+                        y[:, pd::2, ph::2] += np.einsum("ndhwi,oi->ndhwo", xs_,
+                                                        wcu[up_tap(pd, ph, ia, ib, kw)])
+    if b is not None:
+        y += b
+    if relu:
+        y = np.maximum(y, 0.0)
+    return y
+
+
 def conv3_input_grad(dy, w):
     """d(sum dy * conv3(x, w)) / dx"""
     n, D, H, W, co = dy.shape
