@@ -673,9 +673,17 @@ struct AgdProblem {
   float *lamb_cur;  /* device threshold read by captured iterations */
   int sched_cap;    /* capacity of dev.lamb_sched (floats) */
   int u_valid;   /* U holds the forward spatial transform of X */
+  int t_valid;   /* T holds the forward time transform of that U (no wavelets) */
   int *nk_req;   /* per spatial class: planes per task (0: the default rule) */
   size_t u_len;  /* complex elements of U and of V */
 };
+
+/* Every change of X or U goes through here: U's state is v, and T no
+ * longer holds the forward of the current model. */
+static void set_u_valid(AgdProblem *p, int v) {
+  p->u_valid = v;
+  p->t_valid = 0;
+}
 
 static void *dev_alloc(AgdProblem *p, size_t bytes) {
   void *q;
@@ -1741,7 +1749,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
       goto err;
   }
   p->fuse = !host_only && agdg_can_fuse(&p->dev) && getenv("AGD_NO_FUSE") == NULL;
-  p->u_valid = 0;
+  set_u_valid(p, 0);
   /* stepsize (model.c:1230-1312): fold of the space tapers, max with 1 */
   if (!host_only) {
     unsigned int bits = 0;
@@ -1913,10 +1921,11 @@ EXPORT int agdx_set_data(AgdProblem *p, float const *const *data, float *d_out) 
 static int forward_record(AgdProblem *p, int resid) {
   if (!p->u_valid) {
     if (agdg_fwd_spatial(&p->dev, p->dev.X)) return 1;
-    p->u_valid = 1;
+    set_u_valid(p, 1);
   }
   if (agdg_fwd_time(&p->dev)) return 1;
   if (p->use_wavelets && agdg_wavelet(&p->dev, 0)) return 1;
+  p->t_valid = !p->use_wavelets;
   return agdg_blend_traces(&p->dev, resid);
 }
 
@@ -1930,14 +1939,14 @@ static int adjoint_update(AgdProblem *p, float alpha, float lamb, int mode) {
     if (agdg_adj_time(&p->dev, NULL, 0)) return 1;
   }
   if (agdg_adj_spatial(&p->dev, alpha, lamb, mode, fuse)) return 1;
-  p->u_valid = fuse;
+  set_u_valid(p, fuse);
   return 0;
 }
 
 EXPORT int agdx_forward(AgdProblem *p, float const *x, float *rec) {
   size_t const mb = (size_t)agdx_model_len(p) * 8;
   if (agdg_h2d(p->dev.X, x, mb)) return 1;
-  p->u_valid = 0;
+  set_u_valid(p, 0);
   if (forward_record(p, 0)) return 1;
   return agdg_d2h(rec, p->dev.r, (size_t)p->cfg.rec_len * 4);
 }
@@ -1969,7 +1978,7 @@ static int warm_model_flat(AgdProblem *p, float const *y_dev) {
   if (agdg_adj_time(&p->dev, p->dev.T, 1)) return 1;
   /* with the fused kernels the model's forward spatial stage comes along */
   if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, p->fuse)) return 1;
-  p->u_valid = p->fuse;
+  set_u_valid(p, p->fuse);
   return 0;
 }
 
@@ -2029,7 +2038,7 @@ static int chunked_iteration(AgdProblem *p, float alpha, float lamb, int last) {
   p->dev.w0 = 0;
   p->dev.w1 = p->n_win;
   if (rc == 0 && !last && agdg_blend_traces(&p->dev, 1)) rc = 1;
-  p->u_valid = p->fuse;
+  set_u_valid(p, p->fuse);
   return rc;
 }
 
@@ -2205,9 +2214,9 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
   lamb *= f;
   if (zero_start) {
     if (agdg_shrink(&p->dev, agdx_model_len(p), lamb)) return 1;
-    p->u_valid = 0;
+    set_u_valid(p, 0);
   } else if (x0 != NULL) {
-    p->u_valid = 0;
+    set_u_valid(p, 0);
   }
   if (print_freq > 0) {
     double r2 = r2_start;
@@ -2241,7 +2250,7 @@ EXPORT int agdx_set_spatial_precision(AgdProblem *p, int mode) {
   if (p->host_only || (mode != 0 && mode != 1)) return -1;
   p->dev.precision = mode;
   p->fuse = agdg_can_fuse(&p->dev) && getenv("AGD_NO_FUSE") == NULL;
-  p->u_valid = 0;
+  set_u_valid(p, 0);
   return mode ? agdg_x3_classes(&p->dev) : 0;
 }
 
@@ -2274,7 +2283,7 @@ static int rebuild_spatial(AgdProblem *p) {
   p->dev.tasks = (AgdTask const *)q;
   p->dev.n_tasks = (int)tasks.n;
   p->dev.lds_spatial = lds_sp;
-  p->u_valid = 0;
+  set_u_valid(p, 0);
   vec_free(&tasks);
   return 0;
 }
@@ -2328,6 +2337,7 @@ static double time_time(AgdProblem *p, int reps) {
  * on one iteration's time stage and the fastest kept. */
 static int autotune_time(AgdProblem *p, int reps) {
   int c, k;
+  p->t_valid = 0; /* the timed launches overwrite T */
   for (c = 0; c < p->n_tclass; c++) {
     AgdClass *cl = &p->tclass[c];
     int const M = cl->p0 * cl->p1;
@@ -2465,10 +2475,15 @@ EXPORT int agdx_iterate(AgdProblem *p, int n, float lamb) {
 EXPORT int agdx_finish(AgdProblem *p) {
   if (!p->u_valid) {
     if (agdg_fwd_spatial(&p->dev, p->dev.X)) return 1;
-    p->u_valid = 1;
+    set_u_valid(p, 1);
   }
-  if (agdg_fwd_time(&p->dev)) return 1;
-  if (p->use_wavelets && agdg_wavelet(&p->dev, 0)) return 1;
+  /* T is reused when the last forward was of the current model (e.g. a
+   * warm start's guard with no iteration after it) */
+  if (!p->t_valid) {
+    if (agdg_fwd_time(&p->dev)) return 1;
+    if (p->use_wavelets && agdg_wavelet(&p->dev, 0)) return 1;
+    p->t_valid = !p->use_wavelets;
+  }
   if (agdg_output(p->dev.T, p->n_out, p->d_o_off, p->d_o_nt, p->d_o_ptr, p->d_o_src,
                   p->d_data))
     return 1;
