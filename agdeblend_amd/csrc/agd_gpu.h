@@ -115,14 +115,11 @@ int agdg_wavelet(AgdDev const *g, int adjoint);
 int agdg_blend_traces(AgdDev const *g, int resid);
 /* adjoint pieces */
 /* V = time R2C of the windowed traces: of the residual record dev.r through
- * the blend entries (Tin = NULL) or of a trace buffer Tin ([ntr][nconv] per
- * window). plain = 0: the adjoint (scaler x space and time tapers, x2 on the
+ * the blend entries dev.went (Tin = NULL; any flat buffer with entries in
+ * the same terms, e.g. patch data) or of a trace buffer Tin ([ntr][nconv]
+ * per window). plain = 0: the adjoint (scaler x space and time tapers, x2 on the
  * interior bins); plain = 1: scaler only (the warm-start model). */
 int agdg_adj_time(AgdDev const *g, float const *Tin, int plain);
-/* window traces of flat patch data into a trace buffer (per global window
- * trace: data offset and length of its patch trace) */
-int agdg_window_traces(AgdDev const *g, int64_t const *src_off, int const *src_nt,
-                       float const *data, float *T);
 /* mode 0: X = X - alpha*G then shrink by lamb; mode 1: X = -alpha*G and
  * record max |X|^2 bits in scalar[0]; mode 2: X = X - alpha*G (no shrink);
  * mode 3: X = G. fuse: also write the forward spatial transform of the new X

@@ -656,8 +656,8 @@ struct AgdProblem {
   int64_t *d_o_off;
   int *d_o_nt, *d_o_ptr;
   int64_t *d_o_src;
-  int64_t *d_wsrc;     /* per global window trace: its patch trace's data offset */
-  int *d_wnt;          /* per global window trace: its patch's n_times */
+  AgdEntry *d_went_data; /* per global window trace: its patch trace in the
+                            patch data (sample 0 at window sample W_t / 2) */
   int n_out;
   float stepsize;
   int have_data;
@@ -1717,27 +1717,25 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->d_o_src = (int64_t *)dev_upload(p, o_src, (size_t)(ncontrib ? ncontrib : 1) * sizeof(int64_t));
     p->d_data = (float *)dev_alloc(p, (size_t)p->data_len * 4 + 16);
     {
-      /* window traces of patch data (warm start) */
-      int64_t *wsrc = (int64_t *)malloc(((size_t)p->n_wtr + 1) * sizeof(int64_t));
-      int *wnt = (int *)malloc(((size_t)p->n_wtr + 1) * sizeof(int));
-      if (wsrc == NULL || wnt == NULL) {
-        free(wsrc);
-        free(wnt);
-        goto err;
-      }
+      /* window traces of patch data (warm start), as entries the adjoint
+       * time kernel reads like the record's: window sample t is data
+       * sample t - W_t / 2 of the window trace's patch trace (set_output's
+       * placement, model.c:1425-1515), zero outside the trace */
+      AgdEntry *wd = (AgdEntry *)calloc((size_t)p->n_wtr + 1, sizeof(AgdEntry));
+      if (wd == NULL) goto err;
       for (w = 0; w < p->n_win; w++) {
         AgdWin const *W = &p->win[w];
+        int const nt = p->nt[W->patch];
         for (t = 0; t < W->ntr; t++) {
           int const gt = W->trace0 + t;
-          wsrc[gt] = p->data_off[W->patch] + (int64_t)p->wpatch_trace[gt] * p->nt[W->patch];
-          wnt[gt] = p->nt[W->patch];
+          wd[gt].rec = p->data_off[W->patch] + (int64_t)p->wpatch_trace[gt] * nt;
+          wd[gt].trace_start = W->half;
+          wd[gt].n = nt < W->nconv - W->half ? nt : W->nconv - W->half;
         }
       }
-      p->d_wsrc = (int64_t *)dev_upload(p, wsrc, (size_t)p->n_wtr * sizeof(int64_t));
-      p->d_wnt = (int *)dev_upload(p, wnt, (size_t)p->n_wtr * sizeof(int));
-      free(wsrc);
-      free(wnt);
-      if (!p->d_wsrc || !p->d_wnt) goto err;
+      p->d_went_data = (AgdEntry *)dev_upload(p, wd, (size_t)p->n_wtr * sizeof(AgdEntry));
+      free(wd);
+      if (!p->d_went_data) goto err;
     }
     free(o_off);
     free(o_nt);
@@ -1971,11 +1969,19 @@ EXPORT int agdx_last_warm_fallback(void) { return g_last_warm_fallback; }
 
 /* agdeblend.c:526-629 */
 /* Warm-start model s R2C(pad(y_w)) of a data estimate y (patch layout)
- * into dev.X: the window traces of y (placed as set_output places them),
- * the plain time R2C (scaler, no tapers, no x2) and the spatial DFTs. */
+ * into dev.X: the plain time R2C (scaler, no tapers, no x2) of the window
+ * traces of y, read in place through d_went_data (placed as set_output
+ * places them), and the spatial DFTs. */
 static int warm_model_flat(AgdProblem *p, float const *y_dev) {
-  if (agdg_window_traces(&p->dev, p->d_wsrc, p->d_wnt, y_dev, p->dev.T)) return 1;
-  if (agdg_adj_time(&p->dev, p->dev.T, 1)) return 1;
+  float *const r = p->dev.r;
+  AgdEntry const *const went = p->dev.went;
+  int rc;
+  p->dev.r = (float *)y_dev; /* read only */
+  p->dev.went = p->d_went_data;
+  rc = agdg_adj_time(&p->dev, NULL, 1);
+  p->dev.r = r;
+  p->dev.went = went;
+  if (rc) return 1;
   /* with the fused kernels the model's forward spatial stage comes along */
   if (agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, p->fuse)) return 1;
   set_u_valid(p, p->fuse);

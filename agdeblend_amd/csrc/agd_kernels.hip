@@ -1009,25 +1009,6 @@ __global__ void k_next_lamb(float const *__restrict__ sched, int *__restrict__ i
   }
 }
 
-// Window traces of patch data (the inverse placement of set_output,
-// model.c:1478-1484): trace buffer sample t of a window trace is data sample
-// t - W_t/2 of its patch trace, zero outside the data. One block per global
-// window trace.
-__global__ __launch_bounds__(NTHR) void k_window_traces(
-    AgdWin const *__restrict__ wins, int const *__restrict__ trace_win,
-    int64_t const *__restrict__ src_off, int const *__restrict__ src_nt,
-    float const *__restrict__ data, float *__restrict__ T) {
-  int const g = blockIdx.x;
-  AgdWin const &w = wins[trace_win[g]];
-  float const *src = data + src_off[g];
-  int const nt = src_nt[g];
-  float *dst = T + w.t_off + (int64_t)(g - w.trace0) * w.nconv;
-  for (int t = threadIdx.x; t < w.nconv; t += blockDim.x) {
-    int const ts = t - w.half;
-    dst[t] = (ts >= 0 && ts < nt) ? src[ts] : 0.f;
-  }
-}
-
 __global__ __launch_bounds__(NTHR) void k_output(
     float const *__restrict__ T, int64_t const *__restrict__ o_off,
     int const *__restrict__ o_nt, int const *__restrict__ o_ptr,
@@ -2384,12 +2365,6 @@ void agdg_graph_destroy(void *exec) {
 }
 
 int agdg_timing_enabled(void) { return g_timing; }
-
-int agdg_window_traces(AgdDev const *g, int64_t const *src_off, int const *src_nt,
-                       float const *data, float *T) {
-  return launch("window_traces", k_window_traces, dim3(g->n_wtr), dim3(NTHR), 0, g->win,
-                g->trace_win, src_off, src_nt, data, T);
-}
 
 int agdg_extract(AgdEntry const *ent, int n_ent, int const *trace_len, float const *rec,
                  float *out) {
