@@ -446,59 +446,80 @@ __global__ __launch_bounds__(NTHR) void k_blend(
     float const *__restrict__ taper, int taper_len,
     unsigned char const *__restrict__ mute, float const *__restrict__ d,
     float *__restrict__ out) {
+  // the tile's entries, NTHR at a time, staged in LDS once per block (every
+  // thread reads all of them; measured faster than each thread reading the
+  // entry table, and than eight entries in flight)
+  __shared__ int64_t s_rec[NTHR], s_src[NTHR];
+  __shared__ int s_n[NTHR], s_id[NTHR];
   int const e0 = tile_ptr[blockIdx.x];
   int const e1 = tile_ptr[blockIdx.x + 1];
   int64_t const base = (int64_t)blockIdx.x * tile;
-  for (int o = threadIdx.x; o < tile; o += blockDim.x) {
+  // uniform loops: every thread takes part in every staging pass
+  for (int ob = 0; ob < tile; ob += NTHR) {
+    int const o = ob + (int)threadIdx.x;
     int64_t const s = base + o;
-    if (s >= rec_len) break;
+    bool const live = o < tile && s < rec_len;
     float acc = 0.f;
     float cnt = 0.f;
-    // four entries at a time: their loads are issued together (clamped
-    // indices, values outside an entry's range unused), then added in the
-    // entries' order, so the sums are those of the serial loop
-    for (int q = e0; q < e1; q += 4) {
-      float v[4], wg[4];
-      bool ok[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        int const qq = min(q + u, e1 - 1);
-        int const id = tile_ent[qq];
+    for (int c0 = e0; c0 < e1; c0 += NTHR) {
+      int const nc = min(NTHR, e1 - c0);
+      __syncthreads();  // the previous pass's entries are no longer read
+      if ((int)threadIdx.x < nc) {
+        int const id = tile_ent[c0 + threadIdx.x];
         AgdEntry const e = ent[id];
-        int64_t const i = s - e.rec;
-        ok[u] = (q + u < e1) && i >= 0 && i < e.n;
-        int const ii = ok[u] ? (int)i : 0;
-        wg[u] = 1.f;
-        if (mode == 4) {
-          v[u] = cst[id];
-        } else {
-          v[u] = src[e.src + e.trace_start + ii];
+        s_rec[threadIdx.x] = e.rec;
+        s_src[threadIdx.x] = e.src + e.trace_start;
+        s_n[threadIdx.x] = e.n;
+        s_id[threadIdx.x] = id;
+      }
+      __syncthreads();
+      // four entries at a time: their loads are issued together (clamped
+      // indices, values outside an entry's range unused), then added in the
+      // entries' order, so the sums are those of the serial loop
+      for (int q = 0; q < nc; q += 4) {
+        float v[4], wg[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          int const qq = min(q + u, nc - 1);
+          int64_t const i = s - s_rec[qq];
+          int const n = s_n[qq];
+          ok[u] = live && (q + u < nc) && i >= 0 && i < n;
+          int const ii = ok[u] ? (int)i : 0;
+          wg[u] = 1.f;
+          if (mode == 4) {
+            v[u] = cst[s_id[qq]];
+          } else {
+            v[u] = src[s_src[qq] + ii];
+            if (mode == 1) {
+              int const to_end = n - 1 - ii;
+              int const edge = ii < to_end ? ii : to_end;
+              wg[u] = (taper_len > 1 && edge >= 0 && edge < taper_len) ? taper[edge] : 1.f;
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (!ok[u]) continue;
           if (mode == 1) {
-            int const to_end = e.n - 1 - ii;
-            int const edge = ii < to_end ? ii : to_end;
-            wg[u] = (taper_len > 1 && edge >= 0 && edge < taper_len) ? taper[edge] : 1.f;
+            acc += v[u] * wg[u];
+            cnt += wg[u];
+          } else if (mode == 2) {
+            acc = v[u];
+          } else {
+            acc += v[u];
           }
         }
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (!ok[u]) continue;
-        if (mode == 1) {
-          acc += v[u] * wg[u];
-          cnt += wg[u];
-        } else if (mode == 2) {
-          acc = v[u];
-        } else {
-          acc += v[u];
-        }
-      }
     }
-    if (mode == 1 && cnt > 0.f) acc /= cnt;
-    bool const muted = mute != nullptr && mute[s];
-    if (mode == 3) {
-      out[s] = muted ? 0.f : acc - d[s];
-    } else {
-      out[s] = muted ? 0.f : acc;
+    if (live) {
+      if (mode == 1 && cnt > 0.f) acc /= cnt;
+      bool const muted = mute != nullptr && mute[s];
+      if (mode == 3) {
+        out[s] = muted ? 0.f : acc - d[s];
+      } else {
+        out[s] = muted ? 0.f : acc;
+      }
     }
   }
 }
