@@ -203,28 +203,119 @@ __global__ __launch_bounds__(UT) void k_conv3(
       st_swz(dst, e >> 1, e & 1, v[k]);
     }
   };
+  // operands of one row of taps: A for the 4 voxel tiles and 3 kw shifts,
+  // B for the 3 taps and NT channel tiles
+  struct Ops {
+    bf16x4 a[3][4];
+    bf16x4 b[3][NT];
+  };
+  if constexpr (UPC) {
+    // The upsampled channels, on the 48 combined taps ([48][CO][CU]) and
+    // staged at their own (coarse) resolution, the whole block's region at
+    // once: coarse planes d0/2 - 1 .. (d0 + DB)/2, rows h0/2 - 1 ..
+    // (h0 + CT_H)/2, columns w0/2 - 1 .. (w0 + CT_W)/2 (d0, h0, w0 even),
+    // row (cd * CPH + ch) * CPW + cw of the ring's LDS.
+    constexpr int CPW = CT_W / 2 + 2, CPH = CT_H / 2 + 2, CPD = DB / 2 + 2;
+    constexpr int NCI = CPD * CPH * CPW * 2;  // staging items (8 channels)
+    constexpr int NCK = (NCI + UT - 1) / UT;
+    static_assert(DB % 2 == 0 && CT_H % 2 == 0 && NCI * 8 <= 3 * PLANE,
+                  "coarse staging: whole coarse planes and rows, within the ring");
 #pragma unroll 1
-  for (int c = 0; c < CI / 16; ++c) {
+    for (int c = 0; c < CU / 16; ++c) {
+      __syncthreads();
+      {
+        uint4 cv[NCK];
+#pragma unroll
+        for (int k = 0; k < NCK; ++k) {
+          int const e0 = tid + k * UT;
+          int const e = e0 < NCI ? e0 : NCI - 1;
+          int const half = e & 1, r = e >> 1;
+          int const cw = r % CPW, t = r / CPW;
+          int const gd = d0 / 2 - 1 + t / CPH, gh = h0 / 2 - 1 + t % CPH;
+          int const gw = w0 / 2 - 1 + cw;
+          bool const ok = gd >= 0 && gd < D / 2 && gh >= 0 && gh < Hu && gw >= 0 && gw < Wu;
+          uint16_t const *const src =
+              xun + (ok ? (int64_t)gd * planeu + (int64_t)(gh * Wu + gw) * CU : 0) + c * 16 +
+              half * 8;
+          uint32_t const msk = ok ? ~0u : 0u;  // a mask, not a branch
+          uint4 const v = *(uint4 const *)src;
+          cv[k] = make_uint4(v.x & msk, v.y & msk, v.z & msk, v.w & msk);
+        }
+#pragma unroll
+        for (int k = 0; k < (NWU + UT - 1) / UT; ++k) {
+          int const e0 = tid + k * UT;
+          int const e = e0 < NWU ? e0 : NWU - 1;
+          int const half = e & 1;
+          int const tc = e >> 1;  // combined tap * CO + co
+          uint4 const v = *(uint4 const *)(wcu + (int64_t)tc * CU + c * 16 + half * 8);
+          st_swz(sw, tc, half, v);
+        }
+#pragma unroll
+        for (int k = 0; k < NCK; ++k) {
+          int const e0 = tid + k * UT;
+          int const e = e0 < NCI ? e0 : NCI - 1;
+          st_swz(sx, e >> 1, e & 1, cv[k]);  // past the end: the last item again
+        }
+      }
+      __syncthreads();
+      int const ph = (h0 + wave) & 1;
+#pragma unroll
+      for (int j = 0; j < DB; ++j) {
+        int const pd = (d0 + j) & 1;
+        // rows (a, b): fine plane d - 1 + pd + a and fine row h - 1 + ph + b,
+        // i.e. coarse plane (j + 1) / 2 + a and row (wave + 1) / 2 + b of the
+        // staging; fine column w0 + x (x = 16 m + i + kw - 1 >= -1) is
+        // coarse column floor(x / 2) + 1
+        auto load_up = [&](int ab, Ops &o) {
+          int const a = ab >> 1, b = ab & 1;
+          int const v0 = (((j + 1) / 2 + a) * CPH + (wave + 1) / 2 + b) * CPW + 1;
+#pragma unroll
+          for (int kw = 0; kw < 3; ++kw) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+              o.a[kw][m] = ld_swz(sx, v0 + ((16 * m + i + kw - 1) >> 1), kq);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+              o.b[kw][t] = ld_swz(sw, up_tap(pd, ph, a, b, kw) * CO + t * 16 + i, kq);
+          }
+        };
+        auto mma_up = [&](Ops const &o) {
+#pragma unroll
+          for (int kw = 0; kw < 3; ++kw)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+              for (int m = 0; m < 4; ++m)
+                acc[j][m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                    o.a[kw][m], o.b[kw][t], acc[j][m][t], 0, 0, 0);
+        };
+        Ops opa, opb;
+        load_up(0, opa);
+#pragma unroll 1
+        for (int r = 1; r < 4; r += 2) {
+          load_up(r, opb);
+          mma_up(opa);
+          load_up(r + 1 < 4 ? r + 1 : r, opa);  // the last row again, unused
+          mma_up(opb);
+        }
+      }
+    }
+  }
+#pragma unroll 1
+  for (int c = UPC ? CU / 16 : 0; c < CI / 16; ++c) {
     __syncthreads();
     {
       uint4 v0[NPK], v1[NPK], v2[NPK];
       load_plane(d0 - 1, c, v0);
       load_plane(d0, c, v1);
       load_plane(d0 + 1, c, v2);
-      // the chunk's taps: wt's 27 ([27][CO][CI]) or, for the upsampled
-      // channels with UPC, the 48 combined ones ([48][CO][CU])
-      bool const upc = UPC && c < CU / 16;
-      int const nw = upc ? NWU : NWI;
-      uint16_t const *const wsrc = upc ? wcu : wt;
-      int const wstride = upc ? CU : CI;
-      constexpr int NWK = ((UPC ? NWU : NWI) + UT - 1) / UT;
 #pragma unroll
-      for (int k = 0; k < NWK; ++k) {
+      for (int k = 0; k < (NWI + UT - 1) / UT; ++k) {
         int const e0 = tid + k * UT;
-        int const e = e0 < nw ? e0 : nw - 1;
+        int const e = e0 < NWI ? e0 : NWI - 1;
         int const half = e & 1;
         int const tc = e >> 1;  // tap * CO + co
-        uint4 const v = *(uint4 const *)(wsrc + (int64_t)tc * wstride + c * 16 + half * 8);
+        uint4 const v = *(uint4 const *)(wt + (int64_t)tc * CI + c * 16 + half * 8);
         st_swz(sw, tc, half, v);  // past the end: the last item again
       }
       store_plane(d0 - 1, v0);
@@ -239,24 +330,11 @@ __global__ __launch_bounds__(UT) void k_conv3(
       // the slot of plane d - 1
       uint4 nxt[NPK];
       if (j + 1 < DB) load_plane(d + 2, c, nxt);
-      // operands of one (kd, kh) row of taps: A for the 4 voxel tiles and
-      // 3 kw shifts, B for the 3 taps and NT channel tiles; double-buffered
-      // so the LDS reads of the next row overlap this row's MFMAs
-      struct Ops {
-        bf16x4 a[3][4];
-        bf16x4 b[3][NT];
-      };
-      // row r of taps: (kd, kh) = (r / 3, r % 3) over 9 rows, or with the
-      // combined taps (a, b) = (r / 2, r % 2) over 4 rows, which read the
-      // fine plane d - 1 + pd + a and fine row h - 1 + ph + b
-      bool const upc = UPC && c < CU / 16;
-      int const nrow = upc ? 4 : 9;
-      int const pd = d & 1, ph = (h0 + wave) & 1;
+      // operands of row r = (kd, kh) of taps, double-buffered so the LDS
+      // reads of the next row overlap this row's MFMAs
       auto load_ops = [&](int r, Ops &o) {
-        int const a = upc ? r >> 1 : r / 3;
-        int const b = upc ? r & 1 : r - 3 * (r / 3);
-        int const kd = upc ? pd + a : a, kh = upc ? ph + b : b;
-        int const wrow = upc ? up_tap(pd, ph, a, b, 0) : r * 3;
+        int const kd = r / 3, kh = r - 3 * (r / 3);
+        int const wrow = r * 3;
         uint16_t const *const pl = sx + ((d + kd - 1 + 3) % 3) * PLANE;
         int const v0 = (wave + kh) * PW + i;
 #pragma unroll
@@ -280,13 +358,13 @@ __global__ __launch_bounds__(UT) void k_conv3(
       Ops opa, opb;
       load_ops(0, opa);
 #pragma unroll 1
-      for (int r = 1; r < nrow; r += 2) {
+      for (int r = 1; r < 9; r += 2) {
         load_ops(r, opb);
         mma_ops(opa);
-        load_ops(r + 1 < nrow ? r + 1 : r, opa);  // even nrow: the last row again, unused
+        load_ops(r + 1, opa);
         mma_ops(opb);
       }
-      if (nrow & 1) mma_ops(opa);
+      mma_ops(opa);
       if (j + 1 < DB) {
         __syncthreads();  // every wave is done with plane d - 1
         store_plane(d + 2, nxt);
