@@ -106,3 +106,35 @@ def test_product_tables_do_not_touch_oracle():
             if f.endswith((".py", ".c", ".h", ".hip")):
                 txt = open(os.path.join(root, f)).read()
                 assert not re.search(r"(from|import)\s+oracle|liboracle", txt), f
+
+
+@pytest.mark.parametrize("wt", [64, 128, 256, 512, 200])
+def test_time_factorization_tables(wt):
+    """Each specialised time class starts on the default factorization of
+    its half length Pt / 2; agdx_set_time_factors accepts exactly the
+    candidates with that product, and a class of the generic kernels
+    (Wt = 200: Pt = 250) has none."""
+    from tests.test_autotune import TIME_CAND
+    from tests.test_gpu_parity import _spec_geometry
+    kw = _kw(_spec_geometry((64, 2048), (32, wt), 50 + wt))
+    pm = L.Problem(host_only=True, **kw)
+    try:
+        fac = pm.time_factors()
+        assert len(fac) == 1
+        M = 5 * wt // 8
+        if M not in TIME_CAND:
+            assert fac == [(0, 0)]
+            with pytest.raises(ValueError):
+                pm.set_time_factors(0, 5, 25)
+            return
+        assert fac == [TIME_CAND[M][0]]
+        for m1, m2 in TIME_CAND[M]:
+            pm.set_time_factors(0, m1, m2)
+            assert pm.time_factors() == [(m1, m2)]
+        for m1, m2 in ((M, 1), (2, M // 2), (7, 7)):
+            with pytest.raises(ValueError):
+                pm.set_time_factors(0, m1, m2)
+        with pytest.raises(ValueError):
+            pm.set_time_factors(1, *TIME_CAND[M][0])
+    finally:
+        pm.close()
