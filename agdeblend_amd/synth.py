@@ -191,9 +191,10 @@ def config1_small(seed=1011):
     return p
 
 
-def config2(seed=1001):
-    """Same volume as config 1 with windows (32, 32, 512)."""
-    p = config1(seed, windows=(32, 32, 512), n_its=200)
+def config2(seed=1001, **kw):
+    """Same volume as config 1 (kw: its shot timing) with windows
+    (32, 32, 512)."""
+    p = config1(seed, windows=(32, 32, 512), n_its=200, **kw)
     p["name"] = "C2 3D (64 ch x 512 src x 2048 t), windows (32,32,512)"
     return p
 

@@ -60,9 +60,10 @@ def geometry_kwargs(g):
 
 def make_workload(cfg):
     from agdeblend_amd import synth
-    if cfg == 1:
+    if cfg in (1, 2):
         # shots every 4 s +- 1 s: two-fold blending, as in the 2-D workload
-        return synth.config1(interval_s=4.0, dither_s=1.0)
+        # (C2 is C1's volume with windows (32, 32, 512))
+        return synth.CONFIGS[cfg](interval_s=4.0, dither_s=1.0)
     return synth.CONFIGS[cfg]()
 
 
