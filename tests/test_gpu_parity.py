@@ -253,6 +253,26 @@ def test_agd_blend_random_matches_oracle(mode, taper):
                 np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("mode,taper", [(0, 0), (2, 0), (1, 0), (1, 5)])
+def test_agd_blend_deep_overlap_matches_oracle(mode, taper):
+    """600 traces of one channel within a few samples of each other: every
+    256-sample record tile has several hundred blend entries, so the kernel
+    stages them in several passes; the sums stay those of the serial order."""
+    rng = np.random.default_rng(7 + mode + taper)
+    ntr, nt = [600], [300]
+    shot = [rng.integers(0, 40, ntr[0])]
+    ch = [np.zeros(ntr[0], np.int64)]
+    tt = [np.zeros(ntr[0], np.int64)]
+    data = [rng.standard_normal((ntr[0], nt[0])).astype(np.float32)]
+    ref = O.blend(ntr, nt, shot, ch, tt, data, mode, taper, ntr, nt, shot, ch, tt)
+    got = agd.blend(shot, ch, tt, data, mode, taper_length=taper, trace_types_out=tt)
+    for a, b in zip(got, ref):
+        if mode == 1 and taper > 1:
+            np.testing.assert_allclose(a, b, rtol=3e-7, atol=1e-7)
+        else:
+            np.testing.assert_array_equal(a, b)
+
+
 def test_blend_input_record_matches_oracle():
     prob = synth.SMALL[1]()
     g = prob["geometry"]
