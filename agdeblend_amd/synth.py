@@ -278,7 +278,7 @@ def config4(seed=40000, n_ch=64, n_src=1024, n_t=2048):
     return p
 
 
-CONFIGS = {0: config0, 1: config1, 2: config2, 3: config3, 4: config4}
+WORKLOADS = {0: config0, 1: config1, 2: config2, 3: config3, 4: config4}
 SMALL = {0: config0_small, 1: config1_small, 3: config3_small}
 
 
@@ -286,7 +286,7 @@ def make_config(i, small=False, blend_fn=None):
     """Build workload i; the pseudo-deblended input is added as 'blended'
     using blend_fn (default: the oracle-free product blend is not assumed;
     pass agdeblend_amd.agdeblend.blend or the oracle's blend)."""
-    prob = (SMALL[i] if small else CONFIGS[i])()
+    prob = (SMALL[i] if small else WORKLOADS[i])()
     if blend_fn is not None:
         prob["blended"] = blend_input(prob, blend_fn)
     else:

@@ -63,8 +63,8 @@ def make_workload(cfg):
     if cfg in (1, 2):
         # shots every 4 s +- 1 s: two-fold blending, as in the 2-D workload
         # (C2 is C1's volume with windows (32, 32, 512))
-        return synth.CONFIGS[cfg](interval_s=4.0, dither_s=1.0)
-    return synth.CONFIGS[cfg]()
+        return synth.WORKLOADS[cfg](interval_s=4.0, dither_s=1.0)
+    return synth.WORKLOADS[cfg]()
 
 
 def traffic_from_summary(workload):
