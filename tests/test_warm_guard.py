@@ -99,3 +99,13 @@ def test_deblend_warm_without_network_runs_plain():
         assert np.array_equal(a, b)
     pm.close()
     assert not W.accepts([(32, 30, 256)]) and W.accepts([(32, 32, 256)])
+
+
+def test_network_patch_rule():
+    """The network takes 3-D patches with every size a multiple of 4 (two
+    2x2x2 poolings); anything else goes to the plain inversion."""
+    from agdeblend_amd import warmstart as W
+    assert W.accepts([(32, 32, 256), (8, 64, 512)])
+    assert not W.accepts([(32, 30, 256)])
+    assert not W.accepts([(32, 32, 256), (6, 32, 256)])
+    assert not W.accepts([(256, 1024)])
