@@ -919,14 +919,33 @@ __global__ __launch_bounds__(64) void k_wavelet_t(
   float2 const *const rt = roots + plans[w.wplan].roots_off;      // e^{-2 pi i m / MH}
   float2 const *const rfull = roots + plans[w.wplan_full].roots_off;  // e^{-2 pi i m / MR}
   float2 const *const hs = wspec + so;
+  // the trace's samples, all loads in flight together: one source (the trace
+  // buffer, or the record through the blend entry), clamped index, value
+  // masked outside the range (no branch around a load)
+  float const *const src = adjoint ? rec : Tio;
+  int64_t const sbase = adjoint ? e.rec : 0;
+  int const sstart = adjoint ? e.trace_start : 0;
+  int const scount = adjoint ? e.n : nin;
   auto sample = [&](int t) -> float {
-    if (t >= nin) return 0.f;
-    if (!adjoint) return Tio[t];
-    int const i2 = t - e.trace_start;
-    return (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
+    int const i2 = t - sstart;
+    bool const ok = t < nin && (unsigned)i2 < (unsigned)scount;
+    float const x = src[sbase + (ok ? i2 : 0)];
+    return ok ? x : 0.f;
   };
-  for (int n = lane; n < MH; n += 64)
-    sa[(n / M2) * LD + n % M2] = make_float2(sample(2 * n), sample(2 * n + 1));
+  {
+    constexpr int NL = (MH + 63) / 64;
+    float2 vin[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      int const n = lane + 64 * q;
+      vin[q] = make_float2(sample(2 * n), sample(2 * n + 1));
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      int const n = lane + 64 * q;
+      if (n < MH) sa[(n / M2) * LD + n % M2] = vin[q];
+    }
+  }
   __syncthreads();
   // forward: columns, twiddle, rows
   if (lane < M2) {
@@ -970,7 +989,13 @@ __global__ __launch_bounds__(64) void k_wavelet_t(
     float2 const o = cmul(csub(xk, xc), rootf(rfull, MR, k, +1));
     return make_float2(ev.x - o.y, ev.y + o.x);
   };
-  for (int k = lane; k <= MH / 2; k += 64) {
+  // unrolled over the compile-time trip count with the index clamped (a
+  // clamped lane repeats the last pair: the same values to the same places),
+  // so the spectrum and twiddle loads need not wait behind the LDS traffic
+  constexpr int KQ = (MH / 2 + 1 + 63) / 64;
+#pragma unroll
+  for (int q = 0; q < KQ; ++q) {
+    int const k = min(lane + 64 * q, MH / 2);
     int const kc = MH - k;
     float2 const za = sa[zpos(k)];
     float2 const zb = sa[zpos(kc % MH)];
