@@ -6,7 +6,8 @@ bytes and the grids have 10^5 blocks:
   * C4: the blended input record (agd_blend's mean blend and mute of the
     patch data) is bit-exact to the oracle's;
   * C1: the forward operator agrees with the FP64 oracle to 2e-6 relative,
-    and the GPU forward and adjoint pass the dot test in FP64 to 1e-6.
+    and the GPU forward and adjoint pass the dot test in FP64 to 1e-6;
+    agd_deblend after three iterations agrees to 1e-4.
 """
 import numpy as np
 import pytest
@@ -54,3 +55,23 @@ def test_c1_operators_at_full_size():
     lhs = float(np.dot(got, y))
     rhs = float(np.sum(x.real * gp.real + x.imag * gp.imag))
     assert abs(lhs - rhs) <= 1e-6 * np.linalg.norm(got) * np.linalg.norm(y), (lhs, rhs)
+
+
+def test_c1_deblend_at_full_size():
+    """agd_deblend on the full C1 volume (initial update, two thresholded
+    iterations, output) within 1e-4 relative L2 of the FP64 oracle."""
+    prob = synth.config1(interval_s=4.0, dither_s=1.0)
+    kw = _kw(prob["geometry"])
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    ref, _ = O.deblend(data=blended, initial_factor=1.0, n_its=3, double=True, **kw)
+    pm = L.Problem(**kw)
+    try:
+        pm.set_data(blended)
+        pm.run(1.0, 3)
+        out = pm.output()
+    finally:
+        pm.close()
+    num = sum(float(np.sum((np.asarray(a, np.float64) - b) ** 2)) for a, b in zip(out, ref))
+    den = sum(float(np.sum(np.asarray(b, np.float64) ** 2)) for b in ref)
+    rel = (num / den) ** 0.5
+    assert rel <= 1e-4, rel
