@@ -7,7 +7,9 @@ bytes and the grids have 10^5 blocks:
     patch data) is bit-exact to the oracle's;
   * C1: the forward operator agrees with the FP64 oracle to 2e-6 relative,
     and the GPU forward and adjoint pass the dot test in FP64 to 1e-6;
-    agd_deblend after three iterations agrees to 1e-4.
+    agd_deblend after three iterations agrees to 1e-4;
+  * C4: agd_deblend after two iterations, and the learned warm start (the
+    network's estimate given to both sides), agree to 1e-4.
 """
 import numpy as np
 import pytest
@@ -75,3 +77,43 @@ def test_c1_deblend_at_full_size():
     den = sum(float(np.sum(np.asarray(b, np.float64) ** 2)) for b in ref)
     rel = (num / den) ** 0.5
     assert rel <= 1e-4, rel
+
+
+def _rel(out, ref):
+    num = sum(float(np.sum((np.asarray(a, np.float64) - b) ** 2)) for a, b in zip(out, ref))
+    den = sum(float(np.sum(np.asarray(b, np.float64) ** 2)) for b in ref)
+    return (num / den) ** 0.5
+
+
+def test_c4_deblend_and_learned_warm_start_at_full_size():
+    """The headline workload: agd_deblend on the whole C4 volume (initial
+    update, one thresholded iteration, output) within 1e-4 of the FP64
+    oracle; and the learned warm start with one iteration (the network's
+    estimate handed to both the library and the oracle: warm-start model,
+    guard, output) within 1e-4 of the oracle from the same estimate."""
+    import torch
+    from agdeblend_amd import warmstart as W
+    torch.cuda.init()
+    prob = synth.config4()
+    kw = _kw(prob["geometry"])
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    pm = L.Problem(**kw)
+    try:
+        pm.set_data(blended)
+        pm.run(1.0, 2)
+        plain = pm.output()
+        net = W.UNet().load(W.WEIGHTS)
+        pm.set_data(blended)
+        est = W.deblend_warm(pm, net, 0.03, 1)
+        assert est is not None and not L.last_warm_fallback()
+        warm = pm.output()
+        shape = tuple(int(v) for v in pm.shapes[0])
+        est_host = [est.cpu().numpy().reshape(shape)]
+    finally:
+        pm.close()
+    ref, _ = O.deblend(data=blended, initial_factor=1.0, n_its=2, double=True, **kw)
+    assert _rel(plain, ref) <= 1e-4, _rel(plain, ref)
+    ref_w, st = O.deblend(data=blended, initial_factor=0.03, n_its=1, double=True,
+                          warm=est_host, **kw)
+    assert st[2] == 0
+    assert _rel(warm, ref_w) <= 1e-4, _rel(warm, ref_w)
