@@ -9,7 +9,9 @@ bytes and the grids have 10^5 blocks:
     and the GPU forward and adjoint pass the dot test in FP64 to 1e-6;
     agd_deblend after three iterations agrees to 1e-4;
   * C4: agd_deblend after two iterations, and the learned warm start (the
-    network's estimate given to both sides), agree to 1e-4.
+    network's estimate given to both sides), agree to 1e-4;
+  * C2 with the bf16 x 3 spatial DFTs: agd_deblend after two iterations
+    agrees to 1e-4.
 """
 import numpy as np
 import pytest
@@ -117,3 +119,28 @@ def test_c4_deblend_and_learned_warm_start_at_full_size():
                           warm=est_host, **kw)
     assert st[2] == 0
     assert _rel(warm, ref_w) <= 1e-4, _rel(warm, ref_w)
+
+
+def test_c2_split_precision_deblend_at_full_size():
+    """C2 with the bf16 x 3 matrix-core spatial DFTs: agd_deblend after two
+    iterations within 1e-4 of the FP64 oracle (the error is printed)."""
+    prob = synth.config2(interval_s=4.0, dither_s=1.0)
+    kw = _kw(prob["geometry"])
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    pm = L.Problem(**kw)
+    try:
+        assert pm.set_spatial_precision(1) >= 1
+        pm.set_data(blended)
+        pm.run(1.0, 2)
+        x3 = pm.output()
+        assert pm.set_spatial_precision(0) == 0
+        pm.set_data(blended)
+        pm.run(1.0, 2)
+        fp32 = pm.output()
+    finally:
+        pm.close()
+    ref, _ = O.deblend(data=blended, initial_factor=1.0, n_its=2, double=True, **kw)
+    print("C2 full size, agd_deblend 2 iterations vs FP64: bf16x3 %.2e, FP32 %.2e"
+          % (_rel(x3, ref), _rel(fp32, ref)))
+    assert _rel(x3, ref) <= 1e-4
+    assert _rel(fp32, ref) <= 1e-4
