@@ -11,7 +11,9 @@ bytes and the grids have 10^5 blocks:
   * C4: agd_deblend after two iterations, and the learned warm start (the
     network's estimate given to both sides), agree to 1e-4;
   * C2 with the bf16 x 3 spatial DFTs: agd_deblend after two iterations
-    agrees to 1e-4.
+    agrees to 1e-4;
+  * C0 (all 100 iterations) and C3 (three iterations, with the wavelet):
+    agd_deblend agrees to 1e-4.
 """
 import numpy as np
 import pytest
@@ -144,3 +146,24 @@ def test_c2_split_precision_deblend_at_full_size():
           % (_rel(x3, ref), _rel(fp32, ref)))
     assert _rel(x3, ref) <= 1e-4
     assert _rel(fp32, ref) <= 1e-4
+
+
+@pytest.mark.parametrize("cfg,n_its", [(0, 100), (3, 3)])
+def test_full_size_deblend_small_configs(cfg, n_its):
+    """C0 (the 2-D workload, its full 100 iterations) and C3 (vibroseis,
+    with the wavelet convolution and a recording gap) within 1e-4 of the
+    FP64 oracle."""
+    prob = synth.WORKLOADS[cfg]()
+    kw = _kw(prob["geometry"])  # with the wavelets, when the workload has them
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    pm = L.Problem(**kw)
+    try:
+        pm.set_data(blended)
+        pm.run(1.0, n_its)
+        out = pm.output()
+    finally:
+        pm.close()
+    ref, _ = O.deblend(data=blended, initial_factor=1.0, n_its=n_its, double=True, **kw)
+    rel = _rel(out, ref)
+    print("C%d full size, agd_deblend %d iterations vs FP64: %.2e" % (cfg, n_its, rel))
+    assert rel <= 1e-4, rel
