@@ -119,7 +119,7 @@ __global__ __launch_bounds__(UT) void k_conv3(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ xu,
     uint16_t const *__restrict__ wt, uint16_t const *__restrict__ wcu,
     float const *__restrict__ bias, uint16_t const *__restrict__ gate,
-    uint16_t *__restrict__ y, int D, int H, int W, int mode) {
+    uint16_t *__restrict__ y, uint16_t *__restrict__ pool, int D, int H, int W, int mode) {
   static_assert(CI % 16 == 0 && CO % 16 == 0 && CU % 16 == 0 && CU < CI,
                 "channels in multiples of 16");
   static_assert(!UPC || CU > 0, "combined taps serve the upsampled channels");
@@ -133,8 +133,9 @@ __global__ __launch_bounds__(UT) void k_conv3(
   constexpr int NWU = 48 * CO * 2;     // staging items of the combined taps
   constexpr int NTW = UPC ? 48 : 27;   // taps the weight region holds
   constexpr int OBS = CO + 4;  // output staging row: CO channels + 4 (bank spread)
-  constexpr int SMEM = 3 * PLANE + NTW * CO * 16 > 4 * 64 * OBS ? 3 * PLANE + NTW * CO * 16
-                                                                 : 4 * 64 * OBS;
+  // output staging: 4 x 64 rows of OBS, then the pooling partials [2][32][CO]
+  constexpr int OSM = 4 * 64 * OBS + 2 * 32 * CO;
+  constexpr int SMEM = 3 * PLANE + NTW * CO * 16 > OSM ? 3 * PLANE + NTW * CO * 16 : OSM;
   __shared__ __align__(16) uint16_t smem[SMEM];
   uint16_t *const sx = smem;               // ring of 3 input planes
   uint16_t *const sw = smem + 3 * PLANE;   // weights of the chunk
@@ -422,6 +423,53 @@ __global__ __launch_bounds__(UT) void k_conv3(
             val = make_uint4(vw[0], vw[1], vw[2], vw[3]);
           }
           *(uint4 *)(y + o) = val;
+        }
+      }
+    }
+    if constexpr (DB % 2 == 0) {
+      if (mode & 8) {
+        // 2x2x2 max pooling of the output (D, H, W even; d0, h0, w0 even):
+        // item (pooled row pr, column pc, 8 channels) scans its voxels in
+        // (d, h, w) order with a strict >, as the pooling kernel does; the
+        // even plane's partial (one of the bf16 values, so exact) waits in
+        // LDS for the odd plane, handled by the same thread
+        constexpr int C8 = CO / 8;
+        uint16_t *const pm = smem + 4 * 64 * OBS;
+        for (int e = tid; e < 2 * 32 * C8; e += UT) {
+          int const c8 = e % C8, pc = (e / C8) % 32, pr = e / (C8 * 32);
+          float best[8];
+          if (j % 2 == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) best[k] = -INFINITY;
+          } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) best[k] = bf2f(pm[(pr * 32 + pc) * CO + c8 * 8 + k]);
+          }
+#pragma unroll
+          for (int o = 0; o < 4; ++o) {
+            uint16_t const *const r =
+                smem + (2 * pr + (o >> 1)) * 64 * OBS + (2 * pc + (o & 1)) * OBS + c8 * 8;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              float const f = bf2f(r[k]);
+              if (f > best[k]) best[k] = f;
+            }
+          }
+          if (j % 2 == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) pm[(pr * 32 + pc) * CO + c8 * 8 + k] = f2bf(best[k]);
+          } else {
+            int const dp = d / 2, hp = h0 / 2 + pr, wp = w0 / 2 + pc;
+            if (d < D && 2 * hp < H && 2 * wp < W) {
+              uint32_t q[4];
+#pragma unroll
+              for (int k = 0; k < 4; ++k)
+                q[k] = (uint32_t)f2bf(best[2 * k]) | ((uint32_t)f2bf(best[2 * k + 1]) << 16);
+              int64_t const po =
+                  ((((int64_t)n * (D / 2) + dp) * (H / 2) + hp) * (W / 2) + wp) * CO + c8 * 8;
+              *(uint4 *)(pool + po) = make_uint4(q[0], q[1], q[2], q[3]);
+            }
+          }
         }
       }
     }
@@ -1170,14 +1218,16 @@ static int conv_shape_ok(int N, int D, int H, int W, int CI, int CO) {
                     (unsigned)(N * ((D + db - 1) / db)));                                  \
     return ulaunch("conv3", k_conv3<a, b, u, p>, grid, dim3(UT), stream,                   \
                    (uint16_t const *)x, (uint16_t const *)xu, (uint16_t const *)wt,         \
-                   (uint16_t const *)wcu, bias, (uint16_t const *)gate, (uint16_t *)y, D, H, \
-                   W, mode);                                                                \
+                   (uint16_t const *)wcu, bias, (uint16_t const *)gate, (uint16_t *)y,       \
+                   (uint16_t *)pool, D, H, W, mode);                                        \
   }
 
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
   void const *const xu = NULL;
   void const *const wcu = NULL;
+  void *const pool = NULL;
+  mode &= 7;
   if (!conv_shape_ok(N, D, H, W, CI, CO)) return 1;
 #define CC(a, b) \
   if (CI == a && CO == b) CONV_LAUNCH(a, b, 0, false)
@@ -1189,10 +1239,35 @@ int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gat
 
 #define CAT_CASES(X) X(32, 16, 16) X(64, 32, 32)
 
+/* agdu_conv3 that also writes the 2x2x2 max pooling of its output (the
+   values agdu_maxpool gives, without the arg), for the layers whose blocks
+   hold whole plane pairs */
+int agdu_conv3_pool(void const *x, void const *wt, float const *bias, void *y, void *pool,
+                    int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
+  void const *const xu = NULL;
+  void const *const wcu = NULL;
+  void const *const gate = NULL;
+  if (!conv_shape_ok(N, D, H, W, CI, CO)) return 1;
+  if ((D | H | W) & 1 || pool == NULL) {
+    fprintf(stderr, "ERROR: pooling needs even sizes and an output, got %d x %d x %d\n", D, H,
+            W);
+    return 1;
+  }
+  mode = (mode & 3) | 8;
+#define CP(a, b) \
+  if (CI == a && CO == b && ConvDepth<b, 0>::DB % 2 == 0) CONV_LAUNCH(a, b, 0, false)
+  CONV_CASES(CP)
+#undef CP
+  fprintf(stderr, "ERROR: no pooling convolution kernel for %d -> %d channels\n", CI, CO);
+  return 1;
+}
+
 int agdu_conv3_cat(void const *xu, void const *x, void const *wt, float const *bias,
                    void const *gate, void *y, int N, int D, int H, int W, int CU, int CS,
                    int CO, int mode, void *stream) {
   void const *const wcu = NULL;
+  void *const pool = NULL;
+  mode &= 7;
   if (!conv_shape_ok(N, D, H, W, CU + CS, CO)) return 1;
   if ((D | H | W) & 1) {
     fprintf(stderr, "ERROR: upsampled concatenation needs even sizes, got %d x %d x %d\n", D,
@@ -1232,6 +1307,8 @@ int agdu_conv3_up(void const *xu, void const *x, void const *wt, void const *wcu
     fprintf(stderr, "ERROR: combined taps missing\n");
     return 1;
   }
+  void *const pool = NULL;
+  mode &= 7;
 #define CT(u, c, o) \
   if (CU == u && CS == c && CO == o) CONV_LAUNCH(u + c, o, u, true)
   CAT_CASES(CT)

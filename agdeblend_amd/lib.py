@@ -86,6 +86,7 @@ def load():
     i64 = ctypes.c_int64
     fn("agdu_conv3", ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, ci, vp)
     fn("agdu_conv3_cat", ci, vp, vp, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, ci, ci, vp)
+    fn("agdu_conv3_pool", ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, ci, vp)
     fn("agdu_up_weights", ci, vp, vp, ci, ci, ci, vp)
     fn("agdu_conv3_up", ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, ci, ci, ci, ci, ci, ci, vp)
     fn("agdu_conv_in", ci, vp, vp, ci, vp, vp, ci, ci, ci, ci, cf, vp)
@@ -122,7 +123,7 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
            "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback",
            "agdx_timing",
            "agdx_timing_get", "agdx_bytes",
-           "agdu_conv3", "agdu_conv3_cat", "agdu_up_weights", "agdu_conv3_up", "agdu_conv_in", "agdu_conv_out",
+           "agdu_conv3", "agdu_conv3_pool", "agdu_conv3_cat", "agdu_up_weights", "agdu_conv3_up", "agdu_conv_in", "agdu_conv_out",
            "agdu_flip_weights", "agdu_wgrad_part_len", "agdu_conv3_wgrad",
            "agdu_maxpool", "agdu_maxpool_bwd", "agdu_upsample",
            "agdu_upsample_bwd", "agdu_copy_channels", "agdu_pack_input",

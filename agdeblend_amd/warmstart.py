@@ -227,9 +227,10 @@ class UNet:
         device address src_ptr (float32 [N, D, H, W], e.g. the patch data of
         a deblending problem, agdx_data_dev): field + head[0] / scale, into
         the float32 device tensor out. The inference pass: the first layer
-        reads the field itself, the decoder convolutions read the upsampled
-        and skip tensors in place of their concatenation, and the head
-        computes only its channel 0, in fp32, added to the field."""
+        reads the field itself, each level's last encoder layer writes its
+        2x2x2 max pooling as well, the decoder convolutions read the
+        upsampled and skip tensors in place of their concatenation, and the
+        head computes only its channel 0, in fp32, added to the field."""
         N, D, H, W = shape
         if tuple(out.shape) != tuple(shape) or out.dtype != torch.float32:
             raise ValueError("out must be float32 of shape %s" % (shape,))
@@ -240,13 +241,10 @@ class UNet:
         src = ctypes.c_void_p(src_ptr)
         _check(L.agdu_conv_in(src, self._wptr("e1a"), 16, self._bptr("e1a"), _vp(B["e1a"]), *d0,
                               self.scale, s), "agdu_conv_in")
-        self._conv("e1b", B["e1a"], B["e1b"], d0, 3)
-        _check(L.agdu_maxpool(_vp(B["e1b"]), _vp(B["p1"]), _vp(B["a1"]), *d0, 16, s),
-               "agdu_maxpool")
+        # the encoder's last layer of each level writes its pooling too
+        self._conv_pool("e1b", B["e1a"], B["e1b"], B["p1"], d0, 16, 16)
         self._conv("e2a", B["p1"], B["e2a"], d1, 3)
-        self._conv("e2b", B["e2a"], B["e2b"], d1, 3)
-        _check(L.agdu_maxpool(_vp(B["e2b"]), _vp(B["p2"]), _vp(B["a2"]), *d1, 32, s),
-               "agdu_maxpool")
+        self._conv_pool("e2b", B["e2a"], B["e2b"], B["p2"], d1, 32, 32)
         self._conv("b1", B["p2"], B["b1"], d2, 3)
         self._conv("b2", B["b1"], B["b2"], d2, 3)
         self._conv_cat("d2a", B["b2"], B["e2b"], B["d2a"], d1, 64, 32, 32)
@@ -256,6 +254,13 @@ class UNet:
         _check(L.agdu_conv_out(_vp(B["d1b"]), self._wptr("head"), 16, self._bptr("head"), src,
                                _vp(out), *d0, 1.0 / self.scale, s), "agdu_conv_out")
         return out
+
+    def _conv_pool(self, name, x, y, pool, dims, ci, co):
+        """Layer name (bias, ReLU) into y, and its 2x2x2 max pooling into
+        pool (the inference pass: no pooling arg)."""
+        _check(self.L.agdu_conv3_pool(_vp(x), self._wptr(name), self._bptr(name), _vp(y),
+                                      _vp(pool), *dims, ci, co, 3, self._s()),
+               "agdu_conv3_pool " + name)
 
     def _conv_cat(self, name, xu, xs, y, dims, cu, cs, co):
         """Decoder layer on [upsample(xu), xs] (bias, ReLU), with combined

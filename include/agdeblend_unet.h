@@ -24,6 +24,12 @@ extern "C" {
  * (gate > 0) if mode & 4 (gate has y's layout) */
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream);
+/* agdu_conv3 (mode bits 1 and 2) that also writes pool, the 2x2x2 max
+ * pooling of y ([N][D/2][H/2][W/2][CO], the values of agdu_maxpool, no
+ * arg); D, H, W even. Kernels whose blocks hold whole plane pairs only
+ * (16 and 32 output channels). */
+int agdu_conv3_pool(void const *x, void const *wt, float const *bias, void *y, void *pool,
+                    int N, int D, int H, int W, int CI, int CO, int mode, void *stream);
 /* agdu_conv3 on the concatenation [upsample(xu), x] without forming it:
  * xu is [N][D/2][H/2][W/2][CU] (nearest x2 upsampling; D, H, W even), x is
  * [N][D][H][W][CS]; wt is [27][CO][CU + CS] */

@@ -225,6 +225,43 @@ def test_conv3_cat_equals_formed_concatenation(cu, cs, co):
     assert torch.equal(y1, y2)
 
 
+@pytest.mark.parametrize("ci,co", [(16, 16), (32, 32)])
+def test_conv3_pool_equals_conv_then_maxpool(ci, co):
+    """The convolution that also writes its 2x2x2 max pooling gives the
+    convolution's bits and the pooling kernel's bits (partial tiles in w,
+    blocks past the last plane included)."""
+    rng = np.random.default_rng(ci + co)
+    N, D, H, W = 1, 6, 8, 132
+    x = _bf(rng.standard_normal((N, D, H, W, ci)))
+    w = _bf(rng.standard_normal((27, co, ci)) / np.sqrt(27 * ci))
+    b = torch.tensor(rng.standard_normal(co).astype(np.float32), device="cuda")
+    L = _lib()
+    y1 = torch.empty((N, D, H, W, co), dtype=torch.bfloat16, device="cuda")
+    y2 = torch.empty_like(y1)
+    p1 = torch.empty((N, D // 2, H // 2, W // 2, co), dtype=torch.bfloat16, device="cuda")
+    p2 = torch.empty_like(p1)
+    arg = torch.empty((N, D // 2, H // 2, W // 2, co), dtype=torch.uint8, device="cuda")
+    assert L.agdu_conv3(_p(x), _p(w), _p(b), None, _p(y1), N, D, H, W, ci, co, 3, None) == 0
+    assert L.agdu_maxpool(_p(y1), _p(p1), _p(arg), N, D, H, W, co, None) == 0
+    assert L.agdu_conv3_pool(_p(x), _p(w), _p(b), _p(y2), _p(p2), N, D, H, W, ci, co, 3,
+                             None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    assert torch.equal(p1.view(torch.int16), p2.view(torch.int16))
+    # odd sizes and channel pairs without a pooling kernel (64 output
+    # channels: one plane per block) are refused; the buffers match the
+    # channels asked for, so a wrong acceptance could not go out of bounds
+    assert L.agdu_conv3_pool(_p(x), _p(w), _p(b), _p(y2), _p(p2), N, D, H, W - 1, ci, co, 3,
+                             None) == 1
+    x64 = torch.zeros((N, D, H, W, 32), dtype=torch.bfloat16, device="cuda")
+    w64 = torch.zeros((27, 64, 32), dtype=torch.bfloat16, device="cuda")
+    b64 = torch.zeros(64, dtype=torch.float32, device="cuda")
+    y64 = torch.empty((N, D, H, W, 64), dtype=torch.bfloat16, device="cuda")
+    p64 = torch.empty((N, D // 2, H // 2, W // 2, 64), dtype=torch.bfloat16, device="cuda")
+    assert L.agdu_conv3_pool(_p(x64), _p(w64), _p(b64), _p(y64), _p(p64), N, D, H, W, 32, 64,
+                             3, None) == 1
+
+
 @pytest.mark.parametrize("cu,cs,co", [(32, 16, 16), (64, 32, 32)])
 def test_conv3_up_combined_taps(cu, cs, co):
     """The combined taps are bit for bit the float32 sums of the bf16
