@@ -619,7 +619,7 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
 // a ring of three (swizzled rows, as in k_conv3), one E buffer reused per kd.
 // grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / HDB)); block 256.
 // ---------------------------------------------------------------------------
-#define HDB 4
+#define HDB 16
 __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x,
                                                   uint16_t const *__restrict__ wt, int wco,
                                                   float const *__restrict__ bias,
