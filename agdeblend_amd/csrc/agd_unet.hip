@@ -512,7 +512,7 @@ __global__ __launch_bounds__(UT) void k_up_weights(uint16_t const *__restrict__ 
 // (bias, ReLU, 16-byte stores through per-wave LDS rows).
 // grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / FDB)); block 256.
 // ---------------------------------------------------------------------------
-#define FDB 4
+#define FDB 16
 __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src,
                                                    uint16_t const *__restrict__ wt, int wci,
                                                    float const *__restrict__ bias,
