@@ -3,9 +3,18 @@
 Mirrors the reference Python wrapper's ``deblend`` and ``blend``
 (wrappers/python/agdeblend/agdeblend.py:13-147, 150-323): same argument
 names, order, defaults, shape conventions and return values, with ctypes in
-place of cffi and float32 data (the library's AGD_TYPE). The MPI ``comm``
-argument is accepted only as None.
+place of cffi. The MPI ``comm`` argument is accepted only as None.
+
+Data types follow the reference wrapper's rule (``_get_fdtype``,
+wrappers/python/agdeblend/agdeblend.py:326-336): the first data patch must be
+float32 or float64 and the results come back in that type. The reference
+serves float64 from its double-precision library (``_get_agd_lib``,
+agdeblend.py:339-387); this library computes in float32 on the GPU only, so
+float64 patches are computed in float32 and converted back to float64, with
+a warning.
 """
+import warnings
+
 import numpy as np
 
 from . import lib as _lib
@@ -22,6 +31,18 @@ AGDBlendOverwrite = 2
 def _no_mpi(comm):
     if comm is not None:
         raise RuntimeError("MPI is not supported by the MI355X library")
+
+
+def _get_fdtype(patch):
+    """The results' dtype: that of the first data patch (float32 or float64)."""
+    fdtype = np.asarray(patch).dtype
+    if fdtype not in (np.float32, np.float64):
+        raise RuntimeError("Data must be float32 or float64")
+    if fdtype == np.float64:
+        warnings.warn("libagdeblend (MI355X) computes in float32: float64 data is "
+                      "converted to float32 and the results back to float64",
+                      RuntimeWarning, stacklevel=3)
+    return fdtype
 
 
 def _check_n_patches(n_patches, var, name):
@@ -41,6 +62,7 @@ def deblend(volumes, window_shapes, coords, shottimes, channels, trace_types,
     _no_mpi(comm)
     L = _lib.load()
     n_patches = len(data)
+    fdtype = _get_fdtype(data[0]) if n_patches > 0 else np.dtype(np.float32)
     data_np = [np.array(d, dtype=np.float32, copy=True, order="C") for d in data]
     if len(volumes) != n_patches:
         raise RuntimeError("volumes must have the same length as data")
@@ -71,7 +93,7 @@ def deblend(volumes, window_shapes, coords, shottimes, channels, trace_types,
     if L.agd_deblend(*geo.args(), float(initial_factor), int(n_its), int(print_freq),
                      dt.ptr):
         raise RuntimeError("Error in call to agd_deblend")
-    return [a.copy() for a in dt.np]
+    return [a.astype(fdtype) for a in dt.np]
 
 
 def blend(shottimes, channels, trace_types, data, blend_mode, taper_length=0,
@@ -82,6 +104,7 @@ def blend(shottimes, channels, trace_types, data, blend_mode, taper_length=0,
     L = _lib.load()
     if blend_mode not in (AGDBlendSum, AGDBlendMean, AGDBlendOverwrite):
         raise RuntimeError("Invalid blend_mode")
+    fdtype = _get_fdtype(data[0]) if len(data) > 0 else np.dtype(np.float32)
     data_np = [np.require(d, np.float32, ["C", "A"]) for d in data]
     shottimes_out = shottimes if shottimes_out is None else shottimes_out
     channels_out = channels if channels_out is None else channels_out
@@ -134,7 +157,7 @@ def blend(shottimes, channels, trace_types, data, blend_mode, taper_length=0,
                    int(taper_length), n_patches_out, ntro_p, nto_p, so.ptr, co.ptr,
                    to.ptr, out.ptr):
         raise RuntimeError("Error in call to agd_blend")
-    result = [o.reshape(sh) for o, sh in zip(out.np, out_shapes)]
+    result = [o.reshape(sh).astype(fdtype, copy=False) for o, sh in zip(out.np, out_shapes)]
     if data_out is not None:
         for dst, src in zip(data_out, result):
             dst[...] = src

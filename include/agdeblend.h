@@ -2,9 +2,10 @@
  * Public C interface of the MI355X deblending library (libagdeblend.so).
  *
  * Drop-in replacement for the interface declared in the reference's
- * include/agdeblend.h:1-46: same enums and values, the same AGD_TYPE switch,
- * the same tuning constants and the same two entry points with identical
- * argument order, types and meaning. Existing C callers, the cffi Python
+ * include/agdeblend.h:1-46: same enums and values, AGD_TYPE = float (the
+ * reference's default), the same tuning constants and the same two entry
+ * points with identical argument order, types and meaning. Existing C
+ * callers built without AGD_DOUBLE / AGD_MPI, the cffi Python
  * wrapper (wrappers/python/agdeblend_build.py:7,19) and the Julia/Fortran
  * shims bind to these symbols unchanged.
  *
@@ -25,6 +26,21 @@
 #ifdef AGD_MPI
 #error "libagdeblend (MI355X) is single-process: the AGD_MPI interface is not provided"
 #endif
+/* The reference builds a second library, libagdeblend_double.so, for
+ * callers compiled with AGD_DOUBLE (reference Makefile:13-27). This library
+ * computes in float (the reference's default AGD_TYPE) and has no double
+ * build: a caller compiled with AGD_DOUBLE would pass double buffers to
+ * float code, so that configuration is refused here. */
+#ifdef AGD_DOUBLE
+#error "libagdeblend (MI355X) computes in float: the AGD_DOUBLE interface (libagdeblend_double) is not provided"
+#endif
+
+/* Planner flags the reference passes to FFTW (reference
+ * include/agdeblend.h:20). This library has no FFTW: its transforms are
+ * its own HIP kernels, so the value is accepted and ignored. */
+#ifndef AGD_FFTW_FLAGS
+#define AGD_FFTW_FLAGS 0
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -33,11 +49,7 @@ extern "C" {
 enum AGDTraceType { AGDLive, AGDBad, AGDMissing };
 enum AGDBlendMode { AGDBlendSum, AGDBlendMean, AGDBlendOverwrite };
 
-#ifdef AGD_DOUBLE
-#define AGD_TYPE double
-#else
 #define AGD_TYPE float
-#endif
 
 /* Fraction of zero padding added to each window dimension before the
  * Fourier transform (reference include/agdeblend.h:17). */
