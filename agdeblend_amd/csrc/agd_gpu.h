@@ -99,6 +99,10 @@ typedef struct {
 } AgdGather;
 
 int agdg_init(void);
+/* The device address and size of the library's fixed scratch array on the
+ * current device, which must be the library's device (the device current
+ * at its first call). Used by the scratch-free entry point; no allocation. */
+int agdg_scratch(void **ptr, size_t *bytes);
 void *agdg_alloc(size_t bytes);
 void agdg_free(void *p);
 int agdg_h2d(void *dst, void const *src, size_t bytes);

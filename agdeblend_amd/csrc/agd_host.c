@@ -2624,22 +2624,31 @@ err:
   return 1;
 }
 
-/* allocate, upload and remember a device buffer (freed by the caller) */
-static void *upload_tracked(void **list, int *n, void const *src, size_t bytes) {
-  void *q = agdg_alloc(bytes > 0 ? bytes : 16);
-  list[(*n)++] = q;
-  if (q != NULL && bytes > 0 && agdg_h2d(q, src, bytes)) return NULL;
-  return q;
-}
+/* Device buffers of one agd_blend call */
+enum {
+  BL_TILE_PTR,
+  BL_TILE_ENT,
+  BL_ENT_IN,
+  BL_MUTE,
+  BL_TAPER,
+  BL_ENT_OUT,
+  BL_TRACE_LEN,
+  BL_IN,
+  BL_OUT,
+  BL_REC,
+  BL_N
+};
 
-/* agdeblend.c:800-938 */
-EXPORT int agd_blend(int np, int const *ntr, int const *nt,
-                     long int const *const *shot, int const *const *ch,
-                     enum AGDTraceType const *const *tt, float *const *data,
-                     enum AGDBlendMode mode, int taper, int npo, int const *ntro,
-                     int const *nto, long int const *const *shoto,
-                     int const *const *cho, enum AGDTraceType const *const *tto,
-                     float *const *datao) {
+/* agdeblend.c:800-938. noalloc = 0: the device buffers are allocated for the
+ * call and freed before it returns (agd_blend); noalloc = 1: they are carved
+ * from the library's fixed scratch array (agd_blend_noalloc), and a call
+ * whose buffers do not fit is refused before any device work. */
+static int blend_impl(int np, int const *ntr, int const *nt, long int const *const *shot,
+                      int const *const *ch, enum AGDTraceType const *const *tt,
+                      float *const *data, enum AGDBlendMode mode, int taper, int npo,
+                      int const *ntro, int const *nto, long int const *const *shoto,
+                      int const *const *cho, enum AGDTraceType const *const *tto,
+                      float *const *datao, int noalloc) {
   BlendCfg cfg;
   TraceCoord *ci = NULL, *co = NULL;
   AgdEntry *ei = NULL, *eo = NULL;
@@ -2647,15 +2656,18 @@ EXPORT int agd_blend(int np, int const *ntr, int const *nt,
   int64_t *doff = NULL, *ooff = NULL;
   int64_t din = 0, dout = 0;
   int nin = 0, nout = 0, nt_tiles = 0;
-  int p, t, g, m;
+  int p, t, g, m, b;
   float *tap = NULL;
   unsigned char *mute = NULL;
-  void *dbuf[16];
-  int nd = 0;
+  size_t sz[BL_N];
+  void *buf[BL_N];
+  void const *host[BL_N];
+  int owned = 0;
   float *d_in, *d_out, *d_rec;
   AgdGather gt;
   int rc = 1;
   memset(&cfg, 0, sizeof(cfg));
+  memset(buf, 0, sizeof(buf));
   if (check_blend_inputs(np, ntr, nt, shot, ch, tt, data, mode, taper, npo, ntro, nto,
                          shoto, cho, tto, datao))
     goto fail;
@@ -2707,41 +2719,76 @@ EXPORT int agd_blend(int np, int const *ntr, int const *nt,
     for (i = 0; i < taper; i++)
       tap[i] = cosf(0.5f * (float)M_PI * (float)(taper - i) / (float)(taper + 1));
   }
-#define UP(ptr, bytes) upload_tracked(dbuf, &nd, (ptr), (bytes))
+  /* sizes and host sources of the device buffers (NULL: written on the
+   * device; every element of those is written by the kernels) */
+  sz[BL_TILE_PTR] = ((size_t)nt_tiles + 1) * sizeof(int);
+  sz[BL_TILE_ENT] = (size_t)tp[nt_tiles] * sizeof(int);
+  sz[BL_ENT_IN] = (size_t)nin * sizeof(AgdEntry);
+  sz[BL_MUTE] = (size_t)cfg.rec_len;
+  sz[BL_TAPER] = tap ? (size_t)taper * sizeof(float) : 0;
+  sz[BL_ENT_OUT] = (size_t)nout * sizeof(AgdEntry);
+  sz[BL_TRACE_LEN] = (size_t)nout * sizeof(int);
+  sz[BL_IN] = (size_t)din * 4;
+  sz[BL_OUT] = (size_t)dout * 4;
+  sz[BL_REC] = (size_t)cfg.rec_len * 4;
+  host[BL_TILE_PTR] = tp;
+  host[BL_TILE_ENT] = te;
+  host[BL_ENT_IN] = ei;
+  host[BL_MUTE] = mute;
+  host[BL_TAPER] = tap;
+  host[BL_ENT_OUT] = eo;
+  host[BL_TRACE_LEN] = tl;
+  host[BL_IN] = NULL;
+  host[BL_OUT] = NULL;
+  host[BL_REC] = NULL;
+  if (noalloc) {
+    void *base = NULL;
+    size_t cap = 0, off = 0;
+    if (agdg_scratch(&base, &cap)) goto fail;
+    for (b = 0; b < BL_N; b++) {
+      buf[b] = (char *)base + off;
+      off += (sz[b] + 255) & ~(size_t)255;
+    }
+    if (off > cap) {
+      fprintf(stderr,
+              "ERROR: agd_blend_noalloc needs %zu bytes of device scratch, more than its "
+              "fixed %zu\n",
+              off, cap);
+      goto fail;
+    }
+  } else {
+    owned = 1;
+    for (b = 0; b < BL_N; b++)
+      if ((buf[b] = agdg_alloc(sz[b] > 0 ? sz[b] : 16)) == NULL) goto fail;
+  }
+  for (b = 0; b < BL_N; b++)
+    if (host[b] != NULL && agdg_h2d(buf[b], host[b], sz[b])) goto fail;
   gt.rec_len = cfg.rec_len;
   gt.tile = GATHER_TILE;
   gt.n_tiles = nt_tiles;
-  gt.tile_ptr = (int const *)UP(tp, ((size_t)nt_tiles + 1) * sizeof(int));
-  gt.tile_ent = (int const *)UP(te, (size_t)tp[nt_tiles] * sizeof(int));
-  gt.ent = (AgdEntry const *)UP(ei, (size_t)nin * sizeof(AgdEntry));
-  gt.mute = (unsigned char const *)UP(mute, (size_t)cfg.rec_len);
-  gt.taper = (float const *)UP(tap, tap ? (size_t)taper * sizeof(float) : 0);
+  gt.tile_ptr = (int const *)buf[BL_TILE_PTR];
+  gt.tile_ent = (int const *)buf[BL_TILE_ENT];
+  gt.ent = (AgdEntry const *)buf[BL_ENT_IN];
+  gt.mute = (unsigned char const *)buf[BL_MUTE];
+  gt.taper = (float const *)buf[BL_TAPER];
   gt.taper_len = taper;
-  {
-    AgdEntry const *d_eo = (AgdEntry const *)UP(eo, (size_t)nout * sizeof(AgdEntry));
-    int const *d_tl = (int const *)UP(tl, (size_t)nout * sizeof(int));
-    dbuf[nd] = agdg_alloc((size_t)din * 4 + 16);
-    d_in = (float *)dbuf[nd++];
-    dbuf[nd] = agdg_alloc((size_t)dout * 4 + 16);
-    d_out = (float *)dbuf[nd++];
-    dbuf[nd] = agdg_alloc((size_t)cfg.rec_len * 4 + 16);
-    d_rec = (float *)dbuf[nd++];
-    for (p = 0; p < nd; p++)
-      if (dbuf[p] == NULL) goto fail;
-    for (p = 0; p < np; p++)
-      if (agdg_h2d(d_in + doff[p], data[p], (size_t)ntr[p] * nt[p] * 4)) goto fail;
-    if (agdg_gather(&gt, d_in,
-                    mode == AGDBlendSum ? 0 : (mode == AGDBlendMean ? 1 : 2), d_rec))
-      goto fail;
-    if (agdg_extract(d_eo, nout, d_tl, d_rec, d_out)) goto fail;
-    if (agdg_sync()) goto fail;
-    for (p = 0; p < npo; p++)
-      if (agdg_d2h(datao[p], d_out + ooff[p], (size_t)ntro[p] * nto[p] * 4)) goto fail;
-  }
-#undef UP
+  d_in = (float *)buf[BL_IN];
+  d_out = (float *)buf[BL_OUT];
+  d_rec = (float *)buf[BL_REC];
+  for (p = 0; p < np; p++)
+    if (agdg_h2d(d_in + doff[p], data[p], (size_t)ntr[p] * nt[p] * 4)) goto fail;
+  if (agdg_gather(&gt, d_in, mode == AGDBlendSum ? 0 : (mode == AGDBlendMean ? 1 : 2), d_rec))
+    goto fail;
+  if (agdg_extract((AgdEntry const *)buf[BL_ENT_OUT], nout, (int const *)buf[BL_TRACE_LEN],
+                   d_rec, d_out))
+    goto fail;
+  if (agdg_sync()) goto fail;
+  for (p = 0; p < npo; p++)
+    if (agdg_d2h(datao[p], d_out + ooff[p], (size_t)ntro[p] * nto[p] * 4)) goto fail;
   rc = 0;
 fail:
-  for (p = 0; p < nd; p++) agdg_free(dbuf[p]);
+  if (owned)
+    for (b = 0; b < BL_N; b++) agdg_free(buf[b]);
   free(ci);
   free(co);
   free(ei);
@@ -2754,8 +2801,29 @@ fail:
   free(tap);
   free(mute);
   blendcfg_free(&cfg);
-  if (rc) fprintf(stderr, "ERROR in agd_blend\n");
+  if (rc) fprintf(stderr, noalloc ? "ERROR in agd_blend_noalloc\n" : "ERROR in agd_blend\n");
   return rc;
+}
+
+EXPORT int agd_blend(int np, int const *ntr, int const *nt, long int const *const *shot,
+                     int const *const *ch, enum AGDTraceType const *const *tt,
+                     float *const *data, enum AGDBlendMode mode, int taper, int npo,
+                     int const *ntro, int const *nto, long int const *const *shoto,
+                     int const *const *cho, enum AGDTraceType const *const *tto,
+                     float *const *datao) {
+  return blend_impl(np, ntr, nt, shot, ch, tt, data, mode, taper, npo, ntro, nto, shoto, cho,
+                    tto, datao, 0);
+}
+
+EXPORT int agd_blend_noalloc(int np, int const *ntr, int const *nt,
+                             long int const *const *shot, int const *const *ch,
+                             enum AGDTraceType const *const *tt, float *const *data,
+                             enum AGDBlendMode mode, int taper, int npo, int const *ntro,
+                             int const *nto, long int const *const *shoto,
+                             int const *const *cho, enum AGDTraceType const *const *tto,
+                             float *const *datao) {
+  return blend_impl(np, ntr, nt, shot, ch, tt, data, mode, taper, npo, ntro, nto, shoto, cho,
+                    tto, datao, 1);
 }
 
 EXPORT int agdeblend_blend(int np, int const *ntr, int const *nt,

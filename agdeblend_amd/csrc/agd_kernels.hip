@@ -2054,6 +2054,35 @@ static int fwd_spatial_sized(AgdDev const *g, AgdClass const &c, float const *X)
               : launch_fwd_spatial<P0, P1, false>(g, c, X);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of the scratch-free entry point (agd_blend_noalloc): every
+// device buffer of such a call is carved from this array, so the call makes
+// no allocation. Each device holds its own copy from module load until the
+// process ends. Host code never passes `scratch` itself (in host code the
+// name is a host-side stand-in, not a device address): the device address
+// of each device's copy is looked up once, with that device current, and
+// kept in g_scratch_dev.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static void *g_scratch_dev[kMaxDevices];
+static int g_device = -1;  // the device of g_stream
+
+// device address of `scratch` on the current device (looked up once per device)
+static int scratch_lookup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    fprintf(stderr, "ERROR: device %d is beyond the scratch table\n", dev);
+    return 1;
+  }
+  if (g_scratch_dev[dev] != nullptr) return 0;
+  void *p = nullptr;
+  if (hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+    return 1;
+  g_scratch_dev[dev] = p;
+  return 0;
+}
+
 extern "C" {
 
 int agdg_init(void) {
@@ -2064,6 +2093,8 @@ int agdg_init(void) {
     fprintf(stderr, "ERROR: no HIP device available\n");
     return 1;
   }
+  if (hip_check(hipGetDevice(&g_device), "hipGetDevice")) return 1;
+  if (scratch_lookup(g_device)) return 1;
   if (hip_check(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking),
                 "hipStreamCreate"))
     return 1;
@@ -2089,6 +2120,21 @@ int agdg_init(void) {
 #undef ATTR_T
     (void)hipGetLastError();
   }
+  return 0;
+}
+
+int agdg_scratch(void **ptr, size_t *bytes) {
+  int dev = -1;
+  if (agdg_init()) return 1;
+  if (hip_check(hipGetDevice(&dev), "hipGetDevice")) return 1;
+  if (dev != g_device) {
+    fprintf(stderr, "ERROR: the library runs on device %d, the current device is %d\n",
+            g_device, dev);
+    return 1;
+  }
+  if (scratch_lookup(dev)) return 1;
+  *ptr = g_scratch_dev[dev];
+  *bytes = kScratchBytes;
   return 0;
 }
 

@@ -45,6 +45,7 @@ def load():
     fn("agd_blend", ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, ci, vp, vp, vp, vp, vp, vp)
     fn("agdeblend_deblend", ci, *geo, cf, ci, ci, vp)
     fn("agdeblend_blend", ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, ci, vp, vp, vp, vp, vp, vp)
+    fn("agd_blend_noalloc", ci, ci, vp, vp, vp, vp, vp, vp, ci, ci, ci, vp, vp, vp, vp, vp, vp)
     fn("agdx_create", vp, *geo)
     fn("agdx_create_host", vp, *geo)
     fn("agdx_destroy", None, vp)
@@ -110,7 +111,7 @@ def load():
     return L
 
 
-EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend",
+EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend", "agd_blend_noalloc",
            "agdx_create", "agdx_create_host", "agdx_destroy", "agdx_model_len",
            "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",

@@ -83,6 +83,26 @@ int agd_blend(int n_patches, int const *n_traces, int const *n_times,
               enum AGDTraceType const *const *trace_types_out,
               AGD_TYPE *const *data_out);
 
+/* agd_blend without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory during the call
+ * (code under HIP graph capture, tight serving or training-data loops,
+ * harnesses that forbid hipMalloc). Same arguments, checks and results as
+ * agd_blend (bit for bit), but every device buffer of the call (input copy,
+ * tables, record, output staging) comes from one fixed 64 MiB array that
+ * each GPU holds from library load until the process ends; a call whose
+ * buffers need more is refused (return 1, "ERROR: ..." on stderr) before
+ * any device work. Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (several host threads) race and must not be issued. */
+int agd_blend_noalloc(int n_patches, int const *n_traces, int const *n_times,
+                      long int const *const *shottimes, int const *const *channels,
+                      enum AGDTraceType const *const *trace_types,
+                      AGD_TYPE *const *data, enum AGDBlendMode blend_mode,
+                      int taper_length, int n_patches_out, int const *n_traces_out,
+                      int const *n_times_out, long int const *const *shottimes_out,
+                      int const *const *channels_out,
+                      enum AGDTraceType const *const *trace_types_out,
+                      AGD_TYPE *const *data_out);
+
 /* Aliases with the library-prefixed names; identical behaviour. */
 int agdeblend_deblend(int n_patches, int const *volumes, int const *n_dims,
                       int const *const *window_shapes,
