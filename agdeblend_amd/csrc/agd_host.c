@@ -676,6 +676,10 @@ struct AgdProblem {
   int t_valid;   /* T holds the forward time transform of that U (no wavelets) */
   int *nk_req;   /* per spatial class: planes per task (0: the default rule) */
   size_t u_len;  /* complex elements of U and of V */
+  int64_t t_len; /* floats of the trace buffer T (and T2) */
+  AgdEntry *d_went_ident; /* single-operator tests: each window trace read
+                             whole from a trace-buffer layout (lazy) */
+  int *d_trace_len;       /* per global window trace: nconv (lazy) */
 };
 
 /* Every change of X or U goes through here: U's state is v, and T no
@@ -1591,6 +1595,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->dev.U = (float *)dev_alloc(p, (size_t)uo * 8);
     p->dev.V = (float *)dev_alloc(p, (size_t)uo * 8);
     p->u_len = (size_t)uo;
+    p->t_len = to;
     p->dev.T = (float *)dev_alloc(p, (size_t)to * 4);
     p->dev.T2 = p->use_wavelets ? (float *)dev_alloc(p, (size_t)to * 4) : p->dev.T;
     p->dev.d = (float *)dev_alloc(p, (size_t)p->cfg.rec_len * 4 + 16);
@@ -1882,6 +1887,74 @@ EXPORT int agdx_window_info(AgdProblem const *p, int w, int *info) {
   return 0;
 }
 
+/* model.c:642-653: taper flags (left, right) per space dim of window w */
+EXPORT int agdx_window_taper(AgdProblem const *p, int w, int *use_taper) {
+  int d;
+  if (w < 0 || w >= p->n_win) return 1;
+  for (d = 0; d < p->sw[w].nsp; d++) {
+    use_taper[2 * d] = p->sw[w].use_taper[2 * d];
+    use_taper[2 * d + 1] = p->sw[w].use_taper[2 * d + 1];
+  }
+  return 0;
+}
+
+/* model.c:748-805: the patch trace index of each trace of window w */
+EXPORT int agdx_window_traces(AgdProblem const *p, int w, int *out) {
+  int t;
+  if (w < 0 || w >= p->n_win) return 1;
+  for (t = 0; t < p->win[w].ntr; t++) out[t] = p->wpatch_trace[p->win[w].trace0 + t];
+  return 0;
+}
+
+/* blend.c:113-342, 585-760: the blend tables agd_blend builds, on the host */
+EXPORT int agdx_blend_tables(int np, int const *ntr, int const *nt,
+                             long int const *const *shot, int const *const *ch,
+                             enum AGDTraceType const *const *tt, int nq, int q_ntimes,
+                             long int const *q_shot, int const *q_ch,
+                             enum AGDTraceType const *q_tt, int *counts,
+                             long long *intervals, long long *mutes, int *channels,
+                             long long *coords, long long *q_coords) {
+  BlendCfg cfg;
+  int c, i, k, p, t, g;
+  if (blendcfg_build(np, ntr, nt, shot, ch, tt, &cfg)) return 1;
+  k = 0;
+  for (c = 0; c < cfg.n_channels; c++)
+    for (i = 0; i < cfg.chan[c].n; i++, k++)
+      if (intervals) {
+        intervals[3 * k] = c;
+        intervals[3 * k + 1] = cfg.chan[c].iv[i].start;
+        intervals[3 * k + 2] = cfg.chan[c].iv[i].stop;
+      }
+  counts[0] = k;
+  counts[1] = cfg.n_mute;
+  counts[2] = cfg.n_channels;
+  if (mutes)
+    for (i = 0; i < 4 * cfg.n_mute; i++) mutes[i] = cfg.mute_iv[i];
+  if (channels) memcpy(channels, cfg.uniq, (size_t)cfg.n_channels * sizeof(int));
+  g = 0;
+  for (p = 0; p < np && coords; p++)
+    for (t = 0; t < ntr[p]; t++, g++) {
+      TraceCoord tc;
+      trace_coord(&cfg, shot[p][t], nt[p], ch[p][t], tt[p][t], &tc);
+      coords[5 * g] = tc.channel;
+      coords[5 * g + 1] = tc.channel < 0 ? 0 : tc.interval;
+      coords[5 * g + 2] = tc.channel < 0 ? 0 : tc.interval_start;
+      coords[5 * g + 3] = tc.channel < 0 ? 0 : tc.trace_start;
+      coords[5 * g + 4] = tc.channel < 0 ? 0 : tc.n;
+    }
+  for (t = 0; t < nq && q_coords; t++) {
+    TraceCoord tc;
+    trace_coord(&cfg, q_shot[t], q_ntimes, q_ch[t], q_tt[t], &tc);
+    q_coords[5 * t] = tc.channel;
+    q_coords[5 * t + 1] = tc.channel < 0 ? 0 : tc.interval;
+    q_coords[5 * t + 2] = tc.channel < 0 ? 0 : tc.interval_start;
+    q_coords[5 * t + 3] = tc.channel < 0 ? 0 : tc.trace_start;
+    q_coords[5 * t + 4] = tc.channel < 0 ? 0 : tc.n;
+  }
+  blendcfg_free(&cfg);
+  return 0;
+}
+
 EXPORT int agdx_window_blend(AgdProblem const *p, int w, long long *out) {
   AgdWin const *W;
   int t;
@@ -1954,6 +2027,90 @@ EXPORT int agdx_adjoint(AgdProblem *p, float const *rec, float *g) {
   if (agdg_h2d(p->dev.r, rec, (size_t)p->cfg.rec_len * 4)) return 1;
   if (adjoint_update(p, 0.0f, 0.0f, 3)) return 1;
   return agdg_d2h(g, p->dev.X, mb);
+}
+
+EXPORT long agdx_trace_len(AgdProblem const *p) { return (long)p->t_len; }
+
+/* Single operators of A = mute . blend . wavelet . F, each on its own, for
+ * the per-operator adjoint tests (the loop composes them and fuses the
+ * adjoints: blend^H into the time or wavelet kernels). Model vectors are
+ * complex (model_len), trace vectors are the trace buffer (trace_len floats,
+ * per window [ntr][nconv]), records record_len floats. */
+static int ident_entries(AgdProblem *p) {
+  AgdEntry *e;
+  int *tl;
+  int w, t;
+  if (p->d_went_ident != NULL) return 0;
+  e = (AgdEntry *)malloc(((size_t)p->n_wtr + 1) * sizeof(AgdEntry));
+  tl = (int *)malloc(((size_t)p->n_wtr + 1) * sizeof(int));
+  if (e == NULL || tl == NULL) {
+    free(e);
+    free(tl);
+    return 1;
+  }
+  for (w = 0; w < p->n_win; w++)
+    for (t = 0; t < p->win[w].ntr; t++) {
+      AgdEntry *q = &e[p->win[w].trace0 + t];
+      q->rec = p->win[w].t_off + (int64_t)t * p->win[w].nconv;
+      q->src = q->rec;
+      q->trace_start = 0;
+      q->n = p->win[w].nconv;
+      tl[p->win[w].trace0 + t] = p->win[w].nconv;
+    }
+  p->d_went_ident = (AgdEntry *)dev_upload(p, e, (size_t)p->n_wtr * sizeof(AgdEntry));
+  p->d_trace_len = (int *)dev_upload(p, tl, (size_t)p->n_wtr * sizeof(int));
+  free(e);
+  free(tl);
+  return p->d_went_ident == NULL || p->d_trace_len == NULL;
+}
+
+EXPORT int agdx_operator(AgdProblem *p, int op, int adjoint, float const *in, float *out) {
+  size_t const mb = (size_t)agdx_model_len(p) * 8;
+  size_t const tb = (size_t)p->t_len * 4;
+  size_t const rb = (size_t)p->cfg.rec_len * 4;
+  int rc = 0;
+  if (p->host_only || op < 0 || op > 2) return 1;
+  if ((op == 1 && !p->use_wavelets) || (op != 0 && ident_entries(p))) return 1;
+  set_u_valid(p, 0);
+  if (op == 0 && !adjoint) {
+    /* F: model -> traces */
+    if (agdg_h2d(p->dev.X, in, mb) || agdg_fwd_spatial(&p->dev, p->dev.X) ||
+        agdg_fwd_time(&p->dev) || agdg_d2h(out, p->dev.T, tb))
+      rc = 1;
+  } else if (op == 0) {
+    /* F^H: traces -> model */
+    if (agdg_h2d(p->dev.T, in, tb) || agdg_adj_time(&p->dev, p->dev.T, 0) ||
+        agdg_adj_spatial(&p->dev, 0.0f, 0.0f, 3, 0) || agdg_d2h(out, p->dev.X, mb))
+      rc = 1;
+  } else if (op == 1 && !adjoint) {
+    /* wavelet convolution, in place on the traces */
+    if (agdg_h2d(p->dev.T, in, tb) || agdg_wavelet(&p->dev, 0) || agdg_d2h(out, p->dev.T, tb))
+      rc = 1;
+  } else if (op == 1) {
+    /* its adjoint reads whole traces through identity entries (in place of
+     * the record's blend entries) into T2 */
+    float *const r = p->dev.r;
+    AgdEntry const *const went = p->dev.went;
+    if (agdg_h2d(p->dev.T, in, tb)) return 1;
+    p->dev.r = p->dev.T;
+    p->dev.went = p->d_went_ident;
+    rc = agdg_wavelet(&p->dev, 1);
+    p->dev.r = r;
+    p->dev.went = went;
+    if (rc || agdg_d2h(out, p->dev.T2, tb)) rc = 1;
+  } else if (!adjoint) {
+    /* blend into the record and mute */
+    if (agdg_h2d(p->dev.T, in, tb) || agdg_blend_traces(&p->dev, 0) ||
+        agdg_d2h(out, p->dev.r, rb))
+      rc = 1;
+  } else {
+    /* blend^H: record slices back into the window traces (no mute) */
+    if (agdg_h2d(p->dev.r, in, rb) ||
+        agdg_extract(p->dev.went, p->n_wtr, p->d_trace_len, p->dev.r, p->dev.T) ||
+        agdg_sync() || agdg_d2h(out, p->dev.T, tb))
+      rc = 1;
+  }
+  return rc || agdg_sync();
 }
 
 EXPORT int agdx_get_model(AgdProblem *p, float *x) {

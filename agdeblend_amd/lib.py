@@ -54,6 +54,10 @@ def load():
     fn("agdx_n_windows", ci, vp)
     fn("agdx_window_info", ci, vp, ci, vp)
     fn("agdx_window_blend", ci, vp, ci, vp)
+    fn("agdx_window_taper", ci, vp, ci, vp)
+    fn("agdx_window_traces", ci, vp, ci, vp)
+    fn("agdx_blend_tables", ci, ci, vp, vp, vp, vp, vp, ci, ci, vp, vp, vp, vp, vp, vp, vp,
+       vp, vp)
     fn("agdx_intervals", ci, vp, vp)
     fn("agdx_mutes", ci, vp, vp)
     fn("agdx_unique_channels", ci, vp, vp)
@@ -77,6 +81,8 @@ def load():
     fn("agdx_get_output", ci, vp, vp)
     fn("agdx_finish", ci, vp)
     fn("agdx_get_model", ci, vp, vp)
+    fn("agdx_trace_len", ctypes.c_long, vp)
+    fn("agdx_operator", ci, vp, ci, ci, vp, vp)
     fn("agdx_iterate", ci, vp, ci, cf)
     fn("agdx_last_lambda0", cf)
     fn("agdx_last_warm_fallback", ci)
@@ -114,6 +120,7 @@ def load():
 EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend", "agd_blend_noalloc",
            "agdx_create", "agdx_create_host", "agdx_destroy", "agdx_model_len",
            "agdx_record_len", "agdx_n_windows", "agdx_window_info", "agdx_window_blend",
+           "agdx_window_taper", "agdx_window_traces", "agdx_blend_tables",
            "agdx_intervals", "agdx_mutes", "agdx_unique_channels", "agdx_stepsize",
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
            "agdx_warm_model", "agdx_set_spatial_precision", "agdx_autotune",
@@ -121,7 +128,7 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend", "
            "agdx_set_time_factors", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
            "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
-           "agdx_get_model", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback",
+           "agdx_get_model", "agdx_trace_len", "agdx_operator", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback",
            "agdx_timing",
            "agdx_timing_get", "agdx_bytes",
            "agdu_conv3", "agdu_conv3_pool", "agdu_conv3_cat", "agdu_up_weights", "agdu_conv3_up", "agdu_conv_in", "agdu_conv_out",
@@ -206,6 +213,21 @@ class Problem:
                     W=[int(v) for v in info[16:16 + nsp]],
                     coords=[int(v) for v in info[20:20 + nsp]])
 
+    def window_taper(self, w):
+        """(left, right) taper flags per space dim of window w."""
+        nsp = self.window(w)["nsp"]
+        out = np.zeros(max(1, 2 * nsp), C_INT)
+        if self.L.agdx_window_taper(self.h, w, _p(out)):
+            raise IndexError(w)
+        return [int(v) for v in out[:2 * nsp]]
+
+    def window_traces(self, w):
+        """patch trace index of each trace of window w"""
+        out = np.zeros(max(1, self.window(w)["ntr"]), C_INT)
+        if self.L.agdx_window_traces(self.h, w, _p(out)):
+            raise IndexError(w)
+        return out[:self.window(w)["ntr"]]
+
     def blend_coords(self, w):
         n = self.window(w)["ntr"]
         out = np.zeros((n, 5), np.int64)
@@ -253,6 +275,27 @@ class Problem:
         if self.L.agdx_adjoint(self.h, _p(rec), _p(g)):
             raise RuntimeError("agdx_adjoint failed")
         return g[: self.model_len]
+
+    @property
+    def trace_len(self):
+        return int(self.L.agdx_trace_len(self.h))
+
+    def operator(self, op, adjoint, v):
+        """op 0 FK (model <-> traces), 1 wavelet (traces <-> traces),
+        2 blend with mute (traces -> record; adjoint without the mute)."""
+        n_in, n_out, cin, cout = {
+            (0, 0): (self.model_len, self.trace_len, np.complex64, np.float32),
+            (0, 1): (self.trace_len, self.model_len, np.float32, np.complex64),
+            (1, 0): (self.trace_len, self.trace_len, np.float32, np.float32),
+            (1, 1): (self.trace_len, self.trace_len, np.float32, np.float32),
+            (2, 0): (self.trace_len, self.record_len, np.float32, np.float32),
+            (2, 1): (self.record_len, self.trace_len, np.float32, np.float32)}[(op, int(adjoint))]
+        v = np.ascontiguousarray(v, cin)
+        assert v.size == n_in
+        out = np.zeros(max(1, n_out), cout)
+        if self.L.agdx_operator(self.h, int(op), int(adjoint), _p(v), _p(out)):
+            raise RuntimeError("agdx_operator failed")
+        return out[:n_out]
 
     def run(self, initial_factor, n_its, print_freq=0, x0=None):
         x0p = null()
@@ -369,3 +412,38 @@ def last_lambda0():
 def last_warm_fallback():
     """True when the last warm-started run fell back to the zero start."""
     return bool(load().agdx_last_warm_fallback())
+
+
+def blend_tables(n_times, shottimes, channels, trace_types, q_ntimes=0, q_shottimes=(),
+                 q_channels=(), q_trace_types=()):
+    """The blend tables agd_blend builds for per-patch traces (host only):
+    dict of intervals [k][3] (channel, start, stop), mutes [k][4], unique
+    channels, coords of each input trace [n][5] and of the query traces
+    (q_ntimes samples each) [nq][5] (channel, interval, interval_start,
+    trace_start, n)."""
+    Lb = load()
+    shot = [np.asarray(s, np.int64).reshape(-1) for s in shottimes]
+    ntr = [len(s) for s in shot]
+    ntr_a, ntr_p = int_array(ntr)
+    nt_a, nt_p = int_array(n_times)
+    s = PtrTable(shot, C_LONG)
+    c = PtrTable([np.asarray(x).reshape(-1) for x in channels], C_INT)
+    t = PtrTable([np.asarray(x).reshape(-1) for x in trace_types], C_INT)
+    qs = np.require(np.asarray(q_shottimes, np.int64).reshape(-1), requirements=["C"])
+    qc = np.require(np.asarray(q_channels, C_INT).reshape(-1), requirements=["C"])
+    qt = np.require(np.asarray(q_trace_types, C_INT).reshape(-1), requirements=["C"])
+    nq = len(qs)
+    counts = np.zeros(3, C_INT)
+    args = (len(shot), ntr_p, nt_p, s.ptr, c.ptr, t.ptr, nq, int(q_ntimes), _p(qs), _p(qc),
+            _p(qt), _p(counts))
+    if Lb.agdx_blend_tables(*args, null(), null(), null(), null(), null()):
+        raise RuntimeError("agdx_blend_tables failed")
+    iv = np.zeros((max(1, counts[0]), 3), np.int64)
+    mu = np.zeros((max(1, counts[1]), 4), np.int64)
+    ch = np.zeros(max(1, counts[2]), C_INT)
+    co = np.zeros((max(1, sum(ntr)), 5), np.int64)
+    qo = np.zeros((max(1, nq), 5), np.int64)
+    if Lb.agdx_blend_tables(*args, _p(iv), _p(mu), _p(ch), _p(co), _p(qo)):
+        raise RuntimeError("agdx_blend_tables failed")
+    return dict(intervals=iv[:counts[0]], mutes=mu[:counts[1]], channels=ch[:counts[2]],
+                coords=co[:sum(ntr)], q_coords=qo[:nq])

@@ -65,6 +65,25 @@ int agdx_n_windows(AgdProblem const *p);
 int agdx_window_info(AgdProblem const *p, int w, int *info);
 /* per window trace: channel, interval, interval_start, trace_start, n */
 int agdx_window_blend(AgdProblem const *p, int w, long long *out);
+/* taper flags of window w: left, right per space dim (2 * nsp ints;
+ * model.c:642-653) */
+int agdx_window_taper(AgdProblem const *p, int w, int *use_taper);
+/* patch trace index of each trace of window w (ntr ints; model.c:748-805) */
+int agdx_window_traces(AgdProblem const *p, int w, int *out);
+/* The blend tables agd_blend builds for the given traces (host only, no
+ * device; blend.c:113-342, 585-760): counts[3] = (intervals, mutes,
+ * channels); intervals (channel, start, stop) in the reference's order,
+ * mutes (channel, interval, interval_start, n), unique channel values, the
+ * coordinates (channel, interval, interval_start, trace_start, n) of every
+ * input trace and of nq query traces of q_ntimes samples (channel -1: not
+ * blended). Arrays are filled only when not NULL. */
+int agdx_blend_tables(int n_patches, int const *n_traces, int const *n_times,
+                      long int const *const *shottimes, int const *const *channels,
+                      enum AGDTraceType const *const *trace_types, int nq, int q_ntimes,
+                      long int const *q_shottimes, int const *q_channels,
+                      enum AGDTraceType const *q_trace_types, int *counts,
+                      long long *intervals, long long *mutes, int *channels_out,
+                      long long *coords, long long *q_coords);
 /* stepsize computed on the device (0.5 / max(1, fold)) */
 float agdx_stepsize(AgdProblem const *p);
 
@@ -142,6 +161,16 @@ int agdx_run_warm_dev(AgdProblem *p, float initial_factor, int n_its, int print_
 int agdx_finish(AgdProblem *p);
 /* agdx_finish, then copies the patches into data. */
 int agdx_get_output(AgdProblem *p, float *const *data);
+/* Floats of the trace buffer (per window [n_traces][n_times_conv]). */
+long agdx_trace_len(AgdProblem const *p);
+/* One operator of the forward model A = mute . blend . wavelet . F alone, on
+ * host buffers (the per-operator adjoint tests): op 0 the FK operator F
+ * (model -> traces; adjoint traces -> model, fk.c:307-452), 1 the wavelet
+ * convolution (traces -> traces, wavelet.c:239-301; only on problems with
+ * wavelets), 2 the blend with the mute (traces -> record; adjoint record ->
+ * traces without the mute, blend.c:862-875, 936-960, 1262-1292). Leaves
+ * the model, intermediates and traces overwritten. */
+int agdx_operator(AgdProblem *p, int op, int adjoint, float const *in, float *out);
 /* Copies the current model to host (model_len complex). */
 int agdx_get_model(AgdProblem *p, float *x);
 
