@@ -145,6 +145,8 @@ int agdg_output(float const *T, int n_out, int64_t const *o_off,
 /* gather sources into a record: mode 0 sum, 1 mean, 2 overwrite;
  * muted samples set to 0 when gt->mute is set */
 int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec);
+/* rec = mute(sum gather of src) - d */
+int agdg_gather_resid(AgdGather const *gt, float const *src, float const *d, float *rec);
 /* extract record slices into traces (blend adjoint, blend.c:1262-1292):
  * out[e.src + t] = rec[e.rec + t - e.trace_start] for t in
  * [trace_start, trace_start + n), 0 for other t < trace_len[i] */

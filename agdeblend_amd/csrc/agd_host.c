@@ -661,6 +661,8 @@ struct AgdProblem {
   int n_out;
   float stepsize;
   int have_data;
+  double input_fit2;   /* ||mute(sum blend(data)) - d||^2 of the data set
+                          last (FP64): the warm-start guard's baseline */
   double bytes_M, bytes_T, bytes_R;
   int use_wavelets;
   int host_only; /* tables only, no device (CPU tests of the geometry) */
@@ -1982,6 +1984,12 @@ EXPORT int agdx_set_data(AgdProblem *p, float const *const *data, float *d_out) 
     if (agdg_h2d(p->d_data + p->data_off[a], data[a], n * 4)) return 1;
   }
   if (agdg_gather(&p->dgather, p->d_data, 1, p->dev.d)) return 1;
+  /* the data as its own predictor of the record (the forward of its
+   * warm-start model where the window tapers sum to one): the residual the
+   * warm-start guard compares with; dev.r is recomputed before any use */
+  if (agdg_gather_resid(&p->dgather, p->d_data, p->dev.d, p->dev.r) ||
+      agdg_norm2(&p->dev, p->dev.r, p->cfg.rec_len, &p->input_fit2))
+    return 1;
   if (d_out && agdg_d2h(d_out, p->dev.d, (size_t)p->cfg.rec_len * 4)) return 1;
   p->have_data = 1;
   return agdg_sync();
@@ -2119,10 +2127,13 @@ EXPORT int agdx_get_model(AgdProblem *p, float *x) {
 
 static float g_last_lambda0 = 0.0f;
 static int g_last_warm_fallback = 0;
+static double g_last_warm_ratio = -1.0;
 
 EXPORT float agdx_last_lambda0(void) { return g_last_lambda0; }
 
 EXPORT int agdx_last_warm_fallback(void) { return g_last_warm_fallback; }
+
+EXPORT double agdx_last_warm_ratio(void) { return g_last_warm_ratio; }
 
 /* agdeblend.c:526-629 */
 /* Warm-start model s R2C(pad(y_w)) of a data estimate y (patch layout)
@@ -2341,6 +2352,7 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
     return 1;
   }
   g_last_warm_fallback = 0;
+  g_last_warm_ratio = -1.0;
   if (warm_any && n_its <= 1) {
     /* a warm start with no iteration after it uses no threshold: skip the
      * first update that only serves to compute lambda0 */
@@ -2354,18 +2366,21 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
   if (warm_any) {
     double d2 = 0.0;
     if (warm != NULL ? warm_model(p, warm) : warm_model_flat(p, warm_dev)) return 1;
-    /* guard: the warm model must fit the data better than the zero model,
-     * ||A x_warm - d|| < ||d||; its residual then serves the first
-     * iteration. Otherwise the plain zero-start inversion runs instead. */
+    /* guard: the warm model must fit the data better than the zero model
+     * and better than the data itself does as a prediction of the record,
+     * ||A x_warm - d|| < min(||d||, ||mute(sum blend(data)) - d||); its
+     * residual then serves the first iteration. Otherwise the plain
+     * zero-start inversion runs instead. */
     if (forward_record(p, 1) || agdg_norm2(&p->dev, p->dev.r, p->cfg.rec_len, &r2_start) ||
         agdg_norm2(&p->dev, p->dev.d, p->cfg.rec_len, &d2))
       return 1;
-    if (r2_start < d2) {
+    g_last_warm_ratio = d2 > 0.0 ? sqrt(r2_start / d2) : -1.0;
+    if (r2_start < d2 && r2_start < p->input_fit2) {
       r_ready = 1;
     } else {
-      printf("WARNING: the warm start fits the data worse than the zero start "
-             "(residual %g >= %g); starting from zero\n",
-             sqrt(r2_start), sqrt(d2));
+      printf("WARNING: the warm start fits the data no better than the zero start or the "
+             "data itself (residual %g, zero start %g, data %g); starting from zero\n",
+             sqrt(r2_start), sqrt(d2), sqrt(p->input_fit2));
       g_last_warm_fallback = 1;
       r2_start = -1.0;
       zero_start = 1;

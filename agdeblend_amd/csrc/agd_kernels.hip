@@ -2419,6 +2419,11 @@ int agdg_output(float const *T, int n_out, int64_t const *o_off, int const *o_nt
                 o_src, out);
 }
 
+int agdg_gather_resid(AgdGather const *gt, float const *src, float const *d, float *rec) {
+  return launch("gather", k_blend, dim3(gt->n_tiles), dim3(NTHR), 0, 3, gt->rec_len, gt->tile,
+                gt->tile_ptr, gt->tile_ent, gt->ent, src, (float const *)nullptr, gt->taper,
+                gt->taper_len, gt->mute, d, rec);
+}
 int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec) {
   return launch("gather", k_blend, dim3(gt->n_tiles), dim3(NTHR), 0, mode, gt->rec_len,
                 gt->tile, gt->tile_ptr, gt->tile_ent, gt->ent, src,

@@ -86,6 +86,7 @@ def load():
     fn("agdx_iterate", ci, vp, ci, cf)
     fn("agdx_last_lambda0", cf)
     fn("agdx_last_warm_fallback", ci)
+    fn("agdx_last_warm_ratio", ctypes.c_double)
     fn("agdx_timing", ci, ci)
     fn("agdx_timing_get", ci, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
        ctypes.POINTER(ctypes.c_long))
@@ -128,7 +129,7 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend", "
            "agdx_set_time_factors", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
            "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
-           "agdx_get_model", "agdx_trace_len", "agdx_operator", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback",
+           "agdx_get_model", "agdx_trace_len", "agdx_operator", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback", "agdx_last_warm_ratio",
            "agdx_timing",
            "agdx_timing_get", "agdx_bytes",
            "agdu_conv3", "agdu_conv3_pool", "agdu_conv3_cat", "agdu_up_weights", "agdu_conv3_up", "agdu_conv_in", "agdu_conv_out",
@@ -407,6 +408,13 @@ def timing_get(name):
 
 def last_lambda0():
     return float(load().agdx_last_lambda0())
+
+
+def last_warm_ratio():
+    """||A x_warm - d|| / ||d|| measured by the last warm-started run's guard
+    (None when it had no warm start)."""
+    v = float(load().agdx_last_warm_ratio())
+    return None if v < 0 else v
 
 
 def last_warm_fallback():

@@ -369,12 +369,41 @@ class UNet:
         return B["lpart"]
 
 
+def _level(name):
+    return 2 if name in ("b1", "b2") else (1 if name in ("e2a", "e2b", "d2a", "d2b") else 0)
+
+
 def flops_per_voxel():
-    """Multiply-adds x 2 of one forward pass per full-resolution voxel."""
+    """Multiply-adds x 2 of one forward pass per full-resolution voxel, as
+    the architecture counts them (27 taps x all input x all output
+    channels of every layer)."""
+    return sum(2.0 * 27 * ci * co / 8 ** _level(name) for name, ci, co in LAYERS)
+
+
+# upsampled input channels of the decoder layers that read [upsample(x), skip]
+UP_CHANNELS = {"d2a": 64, "d1a": 32}
+
+
+def executed_flops_per_voxel():
+    """Matrix-core multiply-adds x 2 per full-resolution voxel that the
+    inference pass (UNet.predict_ptr) executes: the first layer with its 27
+    taps as the K of one 32-long product per output channel (one live input
+    channel); the head's channel 0 as three 16 x 16 tap-partial products per
+    voxel (one per depth offset, 16 channels by 9 taps padded to 16); the
+    layers in UNet.UP_LAYERS with 12 instead of 27 steps for their upsampled
+    channels; every other layer 27 taps x all channels."""
     f = 0.0
     for name, ci, co in LAYERS:
-        lv = 2 if name in ("b1", "b2") else (1 if name in ("e2a", "e2b", "d2a", "d2b") else 0)
-        f += 2.0 * 27 * ci * co / 8 ** lv
+        if name == "e1a":
+            macs = 32 * co
+        elif name == "head":
+            macs = 3 * 16 * 16
+        elif name in UNet.UP_LAYERS:
+            cu = UP_CHANNELS[name]
+            macs = (12 * cu + 27 * (ci - cu)) * co
+        else:
+            macs = 27 * ci * co
+        f += 2.0 * macs / 8 ** _level(name)
     return f
 
 

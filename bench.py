@@ -23,8 +23,10 @@ Reported (one JSON line):
                the same kernels from the FETCH_SIZE / WRITE_SIZE counter runs
                summarised in profiles/, when that summary matches the workload
   cpu_baseline the oracle (float32, OpenMP over windows, the reference's
-               algorithm with a project FFT) timed on this host on a bounded
-               sample and scaled to the same n_its
+               algorithm with a project FFT) timed on this host: the whole
+               volume on all affinity cores for n_its 1 and 2 (the measured
+               per-iteration cost, extended to the same n_its), and one
+               thread on a sub-volume scaled by window count
 """
 import argparse
 import json
@@ -130,6 +132,7 @@ def warm_start_leg(pm, prob, args, synth):
     network estimate + agdx_run_warm_dev + output assembly. Also the
     network's own SNR and its inference rate against the MFMA peak."""
     import torch
+    from agdeblend_amd import lib as L
     from agdeblend_amd import warmstart as W
     net = W.UNet().load(W.WEIGHTS)
     pm.set_data(prob["blended"])
@@ -143,8 +146,9 @@ def warm_start_leg(pm, prob, args, synth):
     snr_net = synth.snr(prob, [est.cpu().numpy().reshape(shape)])
     nvox = est.numel()
     tflops = W.flops_per_voxel() * nvox / t_inf / 1e12
-    log("network: inference %.4f s (%.1f TFLOP/s), estimate SNR %.2f dB" %
-        (t_inf, tflops, snr_net))
+    tflops_x = W.executed_flops_per_voxel() * nvox / t_inf / 1e12
+    log("network: inference %.4f s (%.1f TFLOP/s executed, %.1f as the architecture counts), "
+        "estimate SNR %.2f dB" % (t_inf, tflops_x, tflops, snr_net))
     res, best = [], None
     for n in [int(v) for v in args.warm_sweep.split(",") if v]:
         pm.set_data(prob["blended"])
@@ -154,8 +158,18 @@ def warm_start_leg(pm, prob, args, synth):
         pm.finish()
         el = time.perf_counter() - t0
         snr = synth.snr(prob, pm.output())
-        res.append((n, round(el, 4), round(snr, 2)))
-        log("  warm start n_its %3d  %.4f s  SNR %.2f dB" % (n, el, snr))
+        ratio, fell = L.last_warm_ratio(), L.last_warm_fallback()
+        # the zero start at the same iteration count, for comparison
+        pm.set_data(prob["blended"])
+        pm.run(1.0, n)
+        snr_plain = synth.snr(prob, pm.output())
+        res.append(dict(n_its=n, s=round(el, 4), snr_db=round(snr, 2),
+                        zero_start_snr_db=round(snr_plain, 2),
+                        guard_ratio=None if ratio is None else round(ratio, 4),
+                        fell_back=bool(fell)))
+        log("  warm start n_its %3d  %.4f s  SNR %.2f dB (zero start %.2f dB; guard ratio %s%s)"
+            % (n, el, snr, snr_plain, "-" if ratio is None else "%.3f" % ratio,
+               ", fell back" if fell else ""))
         if snr >= args.target_db:
             best = (el, n, snr)
             break
@@ -164,19 +178,21 @@ def warm_start_leg(pm, prob, args, synth):
                 n_its_to_target=None if best is None else best[1],
                 snr_db=None if best is None else round(best[2], 2), sweep=res,
                 network_snr_db=round(snr_net, 2), inference_s=round(t_inf, 4),
-                inference_tflops=round(tflops, 1),
-                inference_mfma_frac=round(tflops / 2500.0, 4))
+                inference_tflops_executed=round(tflops_x, 1),
+                inference_tflops_architecture=round(tflops, 1),
+                inference_mfma_frac=round(tflops_x / 2500.0, 4))
 
 
-def subvolume(prob, n_first):
-    """The first n_first entries of the leading space axis of every patch
-    (same seed, same geometry otherwise)."""
+def subvolume(prob, axis, n_first):
+    """The first n_first entries of space axis `axis` of every patch (same
+    seed, same geometry otherwise)."""
     g = dict(prob["geometry"])
-    cut = lambda a: [np.asarray(x)[:n_first] for x in a]  # noqa: E731
+    idx = (slice(None),) * axis + (slice(0, n_first),)
+    cut = lambda a: [np.asarray(x)[idx] for x in a]  # noqa: E731
     for k in ("shottimes", "channels", "trace_types", "wavelet_idxs"):
         if k in g:
             g[k] = cut(g[k])
-    g["shapes"] = [[n_first] + list(s[1:]) for s in g["shapes"]]
+    g["shapes"] = [list(s[:axis]) + [n_first] + list(s[axis + 1:]) for s in g["shapes"]]
     sub = dict(prob)
     sub["geometry"] = g
     sub["blended"] = cut(prob["blended"])
@@ -184,47 +200,83 @@ def subvolume(prob, n_first):
     return sub
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def fft_backend():
+    """The oracle's transforms: its own mixed-radix Stockham and Bluestein
+    FFTs (oracle/oracle_fft.c). FFTW, which the reference links, is named
+    when this host has it, but the oracle does not use it."""
+    import ctypes.util
+    have = ctypes.util.find_library("fftw3f") is not None
+    return ("project FFT (oracle_fft.c: Stockham radix <= 31, Bluestein beyond); "
+            "libfftw3f %s on this host" % ("present, not used" if have else "absent"))
+
+
+def _oracle_run(oracle, sub, kw, threads):
+    oracle.lib(False).set_threads(threads)
+    times = []
+    for n in (1, 2):
+        t0 = time.perf_counter()
+        oracle.deblend(data=sub["blended"], initial_factor=1.0, n_its=n, double=False, **kw)
+        times.append(time.perf_counter() - t0)
+    return times
+
+
 def cpu_baseline(prob, n_its_target, n_windows_full):
-    """Oracle float32 on this host on a bounded sample: a sub-volume (the
-    first channels) run with n_its=1 and n_its=2; the per-iteration and fixed
-    costs are scaled by the window count to the full volume and to
-    n_its_target iterations (the work per window is identical)."""
+    """The oracle (float32, the reference's algorithm, OpenMP over windows)
+    on this host. All affinity cores: the whole volume with n_its = 1 and 2,
+    so the per-iteration cost is measured, not scaled. One thread: a
+    sub-volume (the first lines of the leading axis) with n_its = 1 and 2,
+    scaled by its window count to the whole volume (the work per window is
+    the same). value: the all-core time for n_its_target iterations."""
     from oracle import oracle
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    oracle.lib(False).set_threads(cores)
-    lead = int(prob["geometry"]["shapes"][0][0])
-    total = int(np.prod(prob["geometry"]["shapes"][0]))
-    parts = max(2, int(np.ceil(total / float(1 << 25))))
-    # never narrower than one window along the cut axis
-    win0 = int(prob["geometry"]["window_shapes"][0][0])
-    n_sub = lead if len(prob["geometry"]["shapes"][0]) < 3 else \
-        min(lead, max(2, win0, lead // parts))
-    sub = subvolume(prob, n_sub)
-    kw = geometry_kwargs(sub["geometry"])
-    m = oracle.Model(double=False, **kw)
+    kw = geometry_kwargs(prob["geometry"])
+    t1, t2 = _oracle_run(oracle, prob, kw, cores)
+    per_it = max(t2 - t1, 1e-9)
+    fixed = max(t1 - per_it, 0.0)
+    value = fixed + per_it * n_its_target
+    # one thread on a bounded sub-volume: the last space axis cut to about
+    # 1/16 of its length, never below two windows
+    shape = [int(v) for v in prob["geometry"]["shapes"][0]]
+    axis = len(shape) - 2
+    lead = shape[axis]
+    win = int(prob["geometry"]["window_shapes"][0][axis])
+    n_sub = min(lead, max(2 * win, lead // 16))
+    sub = subvolume(prob, axis, n_sub)
+    skw = geometry_kwargs(sub["geometry"])
+    m = oracle.Model(double=False, **skw)
     n_windows_sub = m.n_windows
     del m
-    t0 = time.perf_counter()
-    oracle.deblend(data=sub["blended"], initial_factor=1.0, n_its=1, double=False, **kw)
-    t1 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.deblend(data=sub["blended"], initial_factor=1.0, n_its=2, double=False, **kw)
-    t2 = time.perf_counter() - t0
+    s1, s2 = _oracle_run(oracle, sub, skw, 1)
     scale = n_windows_full / max(1, n_windows_sub)
-    per_it = max(t2 - t1, 1e-9) * scale
-    fixed = max(t1 * scale - per_it, 0.0)
-    value = fixed + per_it * n_its_target
+    per_it_1 = max(s2 - s1, 1e-9) * scale
+    oracle.lib(False).set_threads(cores)
     return dict(value=round(value, 3), unit="s", cores=cores, kind="port",
-                sample="oracle float32 (%d threads) on the first %d of %d lines of the "
-                       "volume (%d of %d windows): n_its=1 %.2f s, n_its=2 %.2f s; scaled "
-                       "x%.3f to the full volume (%.3f s per iteration) and to %d "
-                       "iterations" % (cores, n_sub, lead, n_windows_sub, n_windows_full,
-                                       t1, t2, scale, per_it, n_its_target),
-                per_iteration_s=round(per_it, 4))
+                sample="oracle float32 on the whole volume (%d windows), %d threads: n_its=1 "
+                       "%.2f s, n_its=2 %.2f s -> %.3f s per iteration, %.2f s fixed; %d "
+                       "iterations. One thread on the first %d of %d entries of space axis %d (%d windows): "
+                       "n_its=1 %.2f s, n_its=2 %.2f s, x%.2f by windows -> %.2f s per "
+                       "iteration" % (n_windows_full, cores, t1, t2, per_it, fixed,
+                                      n_its_target, n_sub, lead, axis, n_windows_sub, s1, s2,
+                                      scale,
+                                      per_it_1),
+                per_iteration_s=round(per_it, 4), threads=cores,
+                per_iteration_s_1thread=round(per_it_1, 4),
+                per_iteration_1thread_kind="scaled from a sub-volume by window count",
+                cpu_model=cpu_model(), fft_backend=fft_backend())
 
 
 def main():
@@ -236,7 +288,9 @@ def main():
     ap.add_argument("--target-db", type=float, default=20.0)
     ap.add_argument("--sweep", type=str, default=None,
                     help="n_its of the plain inversion, ascending")
-    ap.add_argument("--warm-sweep", type=str, default="1,2,3,5,8,12,20,30")
+    ap.add_argument("--warm-sweep", type=str, default="2,3,5,8,12,20,30",
+                    help="n_its of the warm-started runs; 2 and up run at least one "
+                         "thresholded iteration after the start model")
     ap.add_argument("--warm-factor", type=float, default=0.03)
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",

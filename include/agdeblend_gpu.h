@@ -181,6 +181,9 @@ int agdx_iterate(AgdProblem *p, int n_iterations, float lamb);
 float agdx_last_lambda0(void);
 /* 1 when the last warm-started run fell back to the zero start, else 0. */
 int agdx_last_warm_fallback(void);
+/* ||A x_warm - d|| / ||d|| of the last warm-started run's guard (FP64
+ * norms), or -1 when the last run had no warm start. */
+double agdx_last_warm_ratio(void);
 /* Per-kernel device times accumulated while timing is enabled. */
 int agdx_timing(int enable);
 int agdx_timing_get(char const *kernel, double *ms, long *count);

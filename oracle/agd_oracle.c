@@ -1150,6 +1150,23 @@ static void model_blend_input(omodel const *m, real const *const *data,
   free(bc);
 }
 
+/* The patch data summed into the record (blend.c:936-960 on the patch
+ * traces) and muted: the data as its own prediction of the record, the
+ * warm-start guard's baseline (no counterpart in the reference). */
+static void model_blend_sum_input(omodel const *m, real const *const *data,
+                                  long const *const *shot, int const *const *ch,
+                                  int const *const *tt, real *rec) {
+  int p;
+  memset(rec, 0, m->cfg.rec_len * sizeof(real));
+  for (p = 0; p < m->n_patches; p++) {
+    obc *bc = (obc *)malloc((size_t)(m->ntr[p] + 1) * sizeof(obc));
+    blend_coords(m->ntr[p], m->nt[p], shot[p], ch[p], tt[p], &m->cfg, bc);
+    blend_sum(data[p], m->ntr[p], m->nt[p], bc, &m->cfg, rec);
+    free(bc);
+  }
+  blend_mute(&m->cfg, rec);
+}
+
 /* model.c:1425-1515 */
 static void model_output(omodel const *m, ocpx const *x,
                          int const *const *shape, real *const *data) {
@@ -1403,8 +1420,9 @@ static void model_from_data(omodel const *m, real const *const *y,
 /* agdeblend.c:511-649. stats receives the first threshold and stepsize
  * when non-NULL. With warm (patch data, the layout of data) the start model
  * is model_from_data(warm) instead of the thresholded first update, unless
- * that model's residual is no smaller than the zero model's (then the zero
- * start; stats[2] = 1 in that case, else 0); the threshold schedule still
+ * that model's residual is no smaller than the zero model's or than the
+ * residual of the data itself summed into the record (then the zero start;
+ * stats[2] = 1 in that case, else 0); the threshold schedule still
  * starts from the first update's max |x|. */
 /* Threshold schedule (agdeblend.c:526-533, :623-627): lamb decays by
  * `decay` per iteration until first_linear_it, then falls linearly to zero
@@ -1458,11 +1476,17 @@ static int deblend_impl(int np, int const *volumes, int const *n_dims,
   lamb *= f;
   if (warm != NULL) {
     /* guard: keep the warm model only when it fits the data better than
-     * the zero model, ||A x_warm - d|| < ||d|| (FP64 sums); otherwise the
-     * zero start's thresholded first update */
-    double r2 = 0, d2 = 0;
+     * the zero model and better than the data itself as a prediction of
+     * the record, ||A x_warm - d|| < min(||d||, ||mute(sum(data)) - d||)
+     * (FP64 sums); otherwise the zero start's thresholded first update */
+    double r2 = 0, d2 = 0, in2 = 0;
     size_t i;
     int fallback;
+    model_blend_sum_input(m, (real const *const *)data, shot, ch, tt, r);
+    for (i = 0; i < m->cfg.rec_len; i++) {
+      double const ri = (double)(r[i] - d[i]);
+      in2 += ri * ri;
+    }
     model_from_data(m, warm, shape, x);
     model_forward(m, x, r);
     for (i = 0; i < m->cfg.rec_len; i++) {
@@ -1470,11 +1494,11 @@ static int deblend_impl(int np, int const *volumes, int const *n_dims,
       r2 += ri * ri;
       d2 += (double)d[i] * d[i];
     }
-    fallback = !(r2 < d2);
+    fallback = !(r2 < d2 && r2 < in2);
     if (fallback) {
-      printf("WARNING: the warm start fits the data worse than the zero start "
-             "(residual %g >= %g); starting from zero\n",
-             sqrt(r2), sqrt(d2));
+      printf("WARNING: the warm start fits the data no better than the zero start or the "
+             "data itself (residual %g, zero start %g, data %g); starting from zero\n",
+             sqrt(r2), sqrt(d2), sqrt(in2));
       memset(x, 0, (m->model_len + 1) * sizeof(ocpx));
       model_adjoint_update(m, d, step, x);
       model_shrink(m, lamb, x);

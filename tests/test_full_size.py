@@ -13,8 +13,15 @@ bytes and the grids have 10^5 blocks:
   * C2 with the bf16 x 3 spatial DFTs: agd_deblend after two iterations
     agrees to 1e-4;
   * C0 (all 100 iterations) and C3 (three iterations, with the wavelet):
-    agd_deblend agrees to 1e-4.
+    agd_deblend agrees to 1e-4;
+  * C3 over its full 150 iterations and C1 over 20 iterations against the
+    FP64 oracle's output kept in tests/golden/full_size_deblend.npz (made
+    by tests/golden/make_full_size.py: the oracle needs minutes for these):
+    the kept traces within 1e-4 relative L2, the whole output's energy and
+    random projections within the bounds a 1e-4 relative error allows.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -167,3 +174,40 @@ def test_full_size_deblend_small_configs(cfg, n_its):
     rel = _rel(out, ref)
     print("C%d full size, agd_deblend %d iterations vs FP64: %.2e" % (cfg, n_its, rel))
     assert rel <= 1e-4, rel
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "full_size_deblend.npz")
+
+
+@pytest.mark.parametrize("name", ["c3", "c1"])
+def test_long_full_size_deblend_against_fp64_golden(name):
+    from tests.golden import make_full_size as G
+    z = np.load(GOLDEN, allow_pickle=False)
+    make, n_its, stride = G.CASES[name]
+    assert int(z[name + "_n_its"]) == n_its and int(z[name + "_stride"]) == stride
+    prob = make()
+    kw = _kw(prob["geometry"])
+    blended = synth.blend_input(prob, O.blend_same_geometry)
+    assert G.input_hash(blended) == str(z[name + "_hash"])  # the fixture's input, bit for bit
+    print("%s full size, %d iterations" % (name, n_its), flush=True)
+    pm = L.Problem(**kw)
+    try:
+        pm.set_data(blended)
+        pm.run(1.0, n_its)
+        out = pm.output()
+    finally:
+        pm.close()
+    traces, energy, proj = G.summary(out, stride)
+    ref_traces = z[name + "_traces"].astype(np.float64)
+    rel = np.linalg.norm(traces - ref_traces) / np.linalg.norm(ref_traces)
+    ref_energy = float(z[name + "_energy"])
+    n = sum(int(np.prod(np.shape(o))) for o in out)
+    print("%s, %d iterations vs FP64 oracle: kept traces %.2e, energy %.2e, projections %s"
+          % (name, n_its, rel, abs(energy - ref_energy) / ref_energy,
+             np.abs(proj - z[name + "_proj"]) / np.sqrt(ref_energy * n)))
+    assert rel <= 1e-4
+    # a relative L2 error e of the whole output moves its energy by at most
+    # about 2e and each +-1 projection by at most e ||out|| sqrt(n)
+    assert abs(energy - ref_energy) <= 2.0e-4 * ref_energy
+    assert np.all(np.abs(proj - z[name + "_proj"]) <= 1e-4 * np.sqrt(ref_energy * n))

@@ -1,10 +1,13 @@
 """The plain zero-start inversion as the warm start's fallback.
 
 A warm start is kept only when its model fits the data better than the zero
-model, ||A x_warm - d|| < ||d|| (FP64 norms); otherwise the inversion runs
-from the zero start exactly as agd_deblend does (agdeblend.c:584-629). The
-oracle and the library apply the same rule. An estimate between the blended
-input and the truth passes it; a large random field does not.
+model and better than the data itself as a prediction of the record,
+||A x_warm - d|| < min(||d||, ||mute(sum blend(data)) - d||) (FP64 norms);
+otherwise the inversion runs from the zero start exactly as agd_deblend does
+(agdeblend.c:584-629). The oracle and the library apply the same rule. An
+estimate between the blended input and the truth passes it; the input pushed
+further from the truth (better than zero, worse than the input) and a large
+random field do not.
 """
 import numpy as np
 import pytest
@@ -30,7 +33,8 @@ def _case(cfg):
     rng = np.random.default_rng(5)
     scale = max(float(np.abs(np.asarray(a)).max()) for a in blended)
     bad = [20.0 * scale * rng.standard_normal(np.shape(a)) for a in blended]
-    return kw, blended, good, bad
+    poor = [1.5 * np.asarray(a, np.float64) - 0.5 * b for a, b in zip(blended, prob["truth"])]
+    return kw, blended, good, poor, bad
 
 
 def _rel(out, ref):
@@ -41,14 +45,16 @@ def _rel(out, ref):
 
 @pytest.mark.parametrize("cfg", [0, 1])
 def test_oracle_guard(cfg):
-    """Oracle: the random field falls back and then gives the plain run's
-    output bit for bit; the intermediate estimate is kept."""
-    kw, blended, good, bad = _case(cfg)
+    """Oracle: the random field and the estimate worse than the input fall
+    back and then give the plain run's output bit for bit; the intermediate
+    estimate is kept."""
+    kw, blended, good, poor, bad = _case(cfg)
     plain, _ = O.deblend(data=blended, n_its=N_ITS, threads=1, **kw)
-    out, st = O.deblend(data=blended, n_its=N_ITS, threads=1, warm=bad, **kw)
-    assert st[2] == 1
-    for a, b in zip(out, plain):
-        assert np.array_equal(a, b)
+    for warm in (bad, poor):
+        out, st = O.deblend(data=blended, n_its=N_ITS, threads=1, warm=warm, **kw)
+        assert st[2] == 1
+        for a, b in zip(out, plain):
+            assert np.array_equal(a, b)
     out, st = O.deblend(data=blended, n_its=N_ITS, threads=1, warm=good, **kw)
     assert st[2] == 0
     assert _rel(out, plain) > 1e-3  # the warm start changed the result
@@ -61,12 +67,12 @@ def test_library_guard_matches_oracle(cfg):
     fallback the output is bit-identical to the plain GPU inversion and
     within 1e-4 of the FP64 oracle."""
     from agdeblend_amd import lib as L
-    kw, blended, good, bad = _case(cfg)
+    kw, blended, good, poor, bad = _case(cfg)
     pm = L.Problem(**kw)
     pm.set_data(blended)
     pm.run(1.0, N_ITS)
     plain = pm.output()
-    for warm, expect in ((bad, 1), (good, 0)):
+    for warm, expect in ((bad, 1), (poor, 1), (good, 0)):
         ref, st = O.deblend(data=blended, n_its=N_ITS, double=True, warm=warm, **kw)
         assert st[2] == expect
         pm.set_data(blended)
@@ -88,7 +94,7 @@ def test_deblend_warm_without_network_runs_plain():
     from agdeblend_amd import lib as L
     from agdeblend_amd import warmstart as W
     torch.cuda.init()
-    kw, blended, good, bad = _case(1)
+    kw, blended, good, poor, bad = _case(1)
     pm = L.Problem(**kw)
     pm.set_data(blended)
     pm.run(1.0, N_ITS)
