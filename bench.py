@@ -371,7 +371,7 @@ def main():
             log("  %-12s %9.4f ms/launch  %9.4f ms/iteration" % (k, ms / n, ms / args.steps))
     fk_ms = sum(per_it.get(k, 0.0) for k in FK_KERNELS)
     achieved = b_fk / (fk_ms * 1e-3) / 1e9
-    traffic = traffic_from_summary(prob["name"])
+    traffic = traffic_from_summary(prob["name"] + ("_bf16x3" if args.precision else ""))
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel="FK operator per iteration (%s)" %
