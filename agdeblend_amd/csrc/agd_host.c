@@ -1197,7 +1197,7 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
     int *wpos = (int *)malloc(((size_t)p->n_win + 1) * sizeof(int));
     if (wpos == NULL) return 1;
     cl->wpos = wpos;
-    cl->pad = 0;
+    cl->var = 0;
     for (tb = tb_max; tb >= 1; tb /= 2) {
       size_t need = 0;
       for (w = 0; w < p->n_win; w++) {
@@ -2519,30 +2519,35 @@ static int autotune_time(AgdProblem *p, int reps) {
   for (c = 0; c < p->n_tclass; c++) {
     AgdClass *cl = &p->tclass[c];
     int const M = cl->p0 * cl->p1;
-    int b0 = cl->p0, b1 = cl->p1;
+    int b0 = cl->p0, b1 = cl->p1, bv = cl->var, v;
     double best_t = -1.0;
     if (cl->p0 == 0) continue;
     /* untimed pass first: the first candidate timed on a cold device
      * (clocks, caches, code) would read slow */
     if (time_time(p, reps) < 0.0) return 1;
     for (k = 0; k < (int)(sizeof(kTimeCands) / sizeof(kTimeCands[0])); k++) {
-      double t;
       if (kTimeCands[k][0] * kTimeCands[k][1] != M) continue;
-      cl->p0 = kTimeCands[k][0];
-      cl->p1 = kTimeCands[k][1];
-      t = time_time(p, reps);
-      if (t < 0.0) return 1;
-      if (getenv("AGD_TUNE_VERBOSE"))
-        fprintf(stderr, "autotune: time class M=%d (%d x %d): %.4f ms\n", M, cl->p0, cl->p1,
-                t * 1e3);
-      if (best_t < 0.0 || t < best_t) {
-        best_t = t;
-        b0 = cl->p0;
-        b1 = cl->p1;
+      for (v = 0; v < 2; v++) {
+        double t;
+        cl->p0 = kTimeCands[k][0];
+        cl->p1 = kTimeCands[k][1];
+        cl->var = v;
+        t = time_time(p, reps);
+        if (t < 0.0) return 1;
+        if (getenv("AGD_TUNE_VERBOSE"))
+          fprintf(stderr, "autotune: time class M=%d (%d x %d, variant %d): %.4f ms\n", M,
+                  cl->p0, cl->p1, v, t * 1e3);
+        if (best_t < 0.0 || t < best_t) {
+          best_t = t;
+          b0 = cl->p0;
+          b1 = cl->p1;
+          bv = v;
+        }
       }
     }
     cl->p0 = b0;
     cl->p1 = b1;
+    cl->var = bv;
   }
   return 0;
 }
@@ -2564,6 +2569,22 @@ EXPORT int agdx_set_time_factors(AgdProblem *p, int c, int m1, int m2) {
     }
   fprintf(stderr, "ERROR: no time kernel for %d x %d\n", m1, m2);
   return 1;
+}
+
+EXPORT int agdx_set_time_variant(AgdProblem *p, int c, int var) {
+  if (c < 0 || c >= p->n_tclass || p->tclass[c].p0 == 0 || (var != 0 && var != 1)) {
+    fprintf(stderr, "ERROR: no specialised time class %d or variant %d\n", c, var);
+    return 1;
+  }
+  p->tclass[c].var = var;
+  return 0;
+}
+
+EXPORT int agdx_time_variants(AgdProblem const *p, int *out) {
+  int c;
+  if (out)
+    for (c = 0; c < p->n_tclass; c++) out[c] = p->tclass[c].var;
+  return p->n_tclass;
 }
 
 EXPORT int agdx_time_factors(AgdProblem const *p, int *out) {

@@ -1291,10 +1291,13 @@ template <int M>
 __host__ __device__ constexpr int time_threads() {
   return M >= 160 ? 256 : 128;
 }
-// dynamic LDS bytes of the two kernels
-template <int M>
+// dynamic LDS bytes of the two kernels (forward: LT selects the lean-tile
+// variant, see k_fwd_time_t)
+template <int M, bool LT>
 __host__ __device__ constexpr size_t fwd_time_lds() {
-  return (size_t)8 * (2 * TG * (M + 1) + M + (M + 1)) + (size_t)4 * TG * (4 * M / 5) + 4 * TG;
+  return LT ? (size_t)8 * (TG * (M + 1) + M + (M + 1)) + 4 * TG
+            : (size_t)8 * (2 * TG * (M + 1) + M + (M + 1)) + (size_t)4 * TG * (4 * M / 5) +
+                  4 * TG;
 }
 template <int M>
 __host__ __device__ constexpr size_t adj_time_lds() {
@@ -1307,7 +1310,17 @@ __device__ __forceinline__ int fft2_pos(int n) {
   return (n % M1) * M2 + n / M1;
 }
 
-template <int M1, int M2>
+// Two variants, chosen per time class by the autotuner:
+//   LT = false: the input tile [Ft][TG] and the transform buffer [TG][LDL]
+//     are separate LDS arrays, the overlap carry sits in LDS, and the next
+//     time window's tile is loaded into registers while this one is
+//     transformed;
+//   LT = true ("lean tile"): one LDS tile serves as the input and, once the
+//     first phase has read all of it into registers, as the transform
+//     buffer; the carry stays in registers and the tile is loaded at the
+//     start of its window: about half the LDS and fewer registers, so more
+//     blocks share a CU and hide each other's loads.
+template <int M1, int M2, bool LT>
 __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
     AgdWin const *__restrict__ wins, int const *__restrict__ blocks, int block0,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
@@ -1322,14 +1335,17 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
   constexpr int NJ = TG / JS;            // traces per thread in the output pass
   constexpr int NCH = Ft * TG / 2;       // 16-byte chunks per tile
   constexpr int R = (NCH + NT - 1) / NT;
+  constexpr int NI = TG * M2;            // first-phase items
+  constexpr int ITEMS = (NI + NT - 1) / NT;
   static_assert(NT % half == 0 && TG % JS == 0, "output mapping");
+  static_assert(!LT || ITEMS <= 2, "first-phase items per thread");
   extern __shared__ float2 sm[];
-  float2 *IN = sm;            // [Ft][TG] tile
-  float2 *Z = IN + Ft * TG;   // [TG][LDL]
-  float2 *TW = Z + TG * LDL;  // twiddles [M2][M1]
-  float2 *RP = TW + M;        // exp(+2 pi i k / Pt), k <= M
-  float *CA = (float *)(RP + Ft);  // [TG][half] overlap carry
-  float *SC = CA + TG * half;      // [TG] per-trace scale
+  float2 *IN = sm;                         // [Ft][TG] tile
+  float2 *Z = LT ? sm : IN + Ft * TG;      // [TG][LDL]
+  float2 *TW = Z + TG * LDL;               // twiddles [M2][M1]
+  float2 *RP = TW + M;                     // exp(+2 pi i k / Pt), k <= M
+  float *CA = (float *)(RP + Ft);          // [TG][half] overlap carry (LT: unused)
+  float *SC = LT ? CA : CA + TG * half;    // [TG] per-trace scale
   int const bid = block0 + blockIdx.x;
   AgdWin const &w = wins[blocks[2 * bid]];
   int const t0 = blocks[2 * bid + 1];
@@ -1344,38 +1360,52 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
       TW[m] = make_float2(r.x, -r.y);
     }
     for (int k = tid; k < Ft; k += NT) RP[k] = make_float2(rP[k].x, -rP[k].y);
-    for (int i = tid; i < TG * half; i += NT) CA[i] = 0.f;
+    if constexpr (!LT)
+      for (int i = tid; i < TG * half; i += NT) CA[i] = 0.f;
     if (tid < TG) SC[tid] = tscale[w.trace0 + t0 + tid];
   }
   int const s = tid % half;
   int const j0 = tid / half;
   float const tap0 = tapers[w.tt_off + s];
   float const tap1 = tapers[w.tt_off + half + s];
+  float ca[LT ? NJ : 1];  // LT: the overlap carry in registers
+#pragma unroll
+  for (int r = 0; r < (LT ? NJ : 1); ++r) ca[r] = 0.f;
   float *Tb = T + w.t_off + (int64_t)t0 * nconv;
   typedef float f4 __attribute__((ext_vector_type(4)));
   f4 const *tile = (f4 const *)(U + w.u_off + (int64_t)(t0 / TG) * Ft * TG);
   int64_t const tw_step = (int64_t)(w.ntr / TG) * Ft * TG / 2;  // float4 per time window
   // chunk r of this thread: min(tid + NT r, NCH - 1) (the clamped ones
   // repeat the last chunk and are not stored)
-  f4 pre[R];
+  f4 pre[LT ? 1 : R];
+  if constexpr (!LT) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) pre[r] = tile[min(tid + NT * r, NCH - 1)];
+    for (int r = 0; r < R; ++r) pre[r] = tile[min(tid + NT * r, NCH - 1)];
+  }
   for (int tw = 0; tw < ntw; ++tw) {
+    if constexpr (LT) {
+      if (tw > 0) __syncthreads();  // the previous window's output pass has read Z
+      f4 const *nt = tile + (int64_t)tw * tw_step;
+      f4 ld[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      if (r + 1 < R || tid + NT * r < NCH) ((f4 *)IN)[tid + NT * r] = pre[r];
-    if (tw + 1 < ntw) {
-      f4 const *nt = tile + (int64_t)(tw + 1) * tw_step;
+      for (int r = 0; r < R; ++r) ld[r] = nt[min(tid + NT * r, NCH - 1)];
 #pragma unroll
-      for (int r = 0; r < R; ++r) pre[r] = nt[min(tid + NT * r, NCH - 1)];
+      for (int r = 0; r < R; ++r)
+        if (r + 1 < R || tid + NT * r < NCH) ((f4 *)IN)[tid + NT * r] = ld[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r + 1 < R || tid + NT * r < NCH) ((f4 *)IN)[tid + NT * r] = pre[r];
+      if (tw + 1 < ntw) {
+        f4 const *nt = tile + (int64_t)(tw + 1) * tw_step;
+#pragma unroll
+        for (int r = 0; r < R; ++r) pre[r] = nt[min(tid + NT * r, NCH - 1)];
+      }
     }
     __syncthreads();
     // packing of the C2R (DC / Nyquist imaginary parts ignored), length-M1
     // FFTs over k1 of Z[k1 M2 + k2], twiddle
-    for (int i = tid; i < TG * M2; i += NT) {
-      int const k2 = i / TG;
-      int const j = i - k2 * TG;
-      float2 v[M1];
+    auto pack = [&](int k2, int j, float2 (&v)[M1]) {
 #pragma unroll
       for (int k1 = 0; k1 < M1; ++k1) {
         int const k = k1 * M2 + k2;
@@ -1390,12 +1420,36 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
         float2 const o = agdfft::c_mul(agdfft::c_sub(xk, xc), RP[k]);
         v[k1] = make_float2(e.x - o.y, e.y + o.x);
       }
+    };
+    auto phase1_out = [&](int k2, int j, float2 (&v)[M1]) {
       agdfft::fft<M1, +1>(v);
       float2 *zz = Z + j * LDL + k2;
       float2 const *tw = TW + k2 * M1;
 #pragma unroll
       for (int n1 = 0; n1 < M1; ++n1)
         zz[n1 * M2] = (n1 == 0) ? v[n1] : agdfft::c_mul(v[n1], tw[n1]);
+    };
+    if constexpr (LT) {
+      // every input of the tile read into registers before the tile, as Z,
+      // is overwritten
+      float2 v[ITEMS][M1];
+#pragma unroll
+      for (int q = 0; q < ITEMS; ++q) {
+        int const i = min(tid + q * NT, NI - 1);
+        pack(i / TG, i - (i / TG) * TG, v[q]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < ITEMS; ++q) {
+        int const i = tid + q * NT;
+        if (i < NI) phase1_out(i / TG, i - (i / TG) * TG, v[q]);
+      }
+    } else {
+      for (int i = tid; i < NI; i += NT) {
+        float2 v[M1];
+        pack(i / TG, i - (i / TG) * TG, v);
+        phase1_out(i / TG, i - (i / TG) * TG, v);
+      }
     }
     __syncthreads();
     for (int i = tid; i < TG * M1; i += NT) {
@@ -1421,14 +1475,20 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
       float const y0 = (s & 1) ? z0.y : z0.x;
       float const y1 = (s & 1) ? z1.y : z1.x;
       float const sc = SC[j];
-      Tb[j * nconv + tw * half + s] = CA[j * half + s] + y0 * sc * tap0;
-      CA[j * half + s] = y1 * sc * tap1;
+      if constexpr (LT) {
+        Tb[j * nconv + tw * half + s] = ca[r] + y0 * sc * tap0;
+        ca[r] = y1 * sc * tap1;
+      } else {
+        Tb[j * nconv + tw * half + s] = CA[j * half + s] + y0 * sc * tap0;
+        CA[j * half + s] = y1 * sc * tap1;
+      }
     }
   }
   int const tl = ntw * half + s;
 #pragma unroll
   for (int r = 0; r < NJ; ++r)
-    if (tl < nconv) Tb[(j0 + JS * r) * nconv + tl] = CA[(j0 + JS * r) * half + s];
+    if (tl < nconv)
+      Tb[(j0 + JS * r) * nconv + tl] = LT ? ca[LT ? r : 0] : CA[(j0 + JS * r) * half + s];
   // zero the capacity beyond the data portion (wavelet input padding)
   for (int j = 0; j < TG; ++j)
     for (int t = (ntw + 1) * half + tid; t < nconv; t += NT) Tb[(int64_t)j * nconv + t] = 0.f;
@@ -2112,7 +2172,9 @@ int agdg_init(void) {
     (void)hipFuncSetAttribute((void const *)k_wavelet,
                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 #define ATTR_T(a, b)                                                              \
-    (void)hipFuncSetAttribute((void const *)k_fwd_time_t<a, b>,                   \
+    (void)hipFuncSetAttribute((void const *)k_fwd_time_t<a, b, false>,            \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
+    (void)hipFuncSetAttribute((void const *)k_fwd_time_t<a, b, true>,             \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);   \
     (void)hipFuncSetAttribute((void const *)k_adj_time_t<a, b>,                   \
                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -2244,11 +2306,17 @@ int agdg_fwd_time(AgdDev const *g) {
     int const n = class_range(g, c, &first);
 #define FT_CASE(a, b)                                                                   \
     if (!done && c.p0 == a && c.p1 == b) {                                              \
-      if (n > 0 &&                                                                      \
-          launch("fwd_time", k_fwd_time_t<a, b>, dim3(n),                                 \
-                 dim3(time_threads<a * b>()), fwd_time_lds<a * b>(), g->win, g->tblocks,   \
-                 first, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
-                 (float2 const *)g->U, g->T))                                           \
+      if (n > 0 && c.var == 0 &&                                                        \
+          launch("fwd_time", k_fwd_time_t<a, b, false>, dim3(n),                        \
+                 dim3(time_threads<a * b>()), fwd_time_lds<a * b, false>(), g->win,     \
+                 g->tblocks, first, g->plans, (float2 const *)g->roots, g->tapers,      \
+                 g->tscale, (float2 const *)g->U, g->T))                                \
+        return 1;                                                                       \
+      if (n > 0 && c.var != 0 &&                                                        \
+          launch("fwd_time", k_fwd_time_t<a, b, true>, dim3(n),                         \
+                 dim3(time_threads<a * b>()), fwd_time_lds<a * b, true>(), g->win,      \
+                 g->tblocks, first, g->plans, (float2 const *)g->roots, g->tapers,      \
+                 g->tscale, (float2 const *)g->U, g->T))                                \
         return 1;                                                                       \
       done = true;                                                                      \
     }

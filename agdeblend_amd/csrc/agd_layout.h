@@ -80,7 +80,8 @@ typedef struct {
   int begin, count;
   int tb;       /* time classes: traces per block; specialised spatial
                    classes: 1 when U / V are tiled by 16 traces */
-  int pad;
+  int var;      /* time classes: forward kernel variant (0 register
+                   prefetch, 1 lean tile); spatial classes: unused */
   size_t lds;
   int const *wpos; /* host array [n_win + 1]: first unit of the class that
                       belongs to window w or later (units are in window

@@ -74,6 +74,8 @@ def load():
     fn("agdx_n_time_classes", ci, vp)
     fn("agdx_time_factors", ci, vp, vp)
     fn("agdx_set_time_factors", ci, vp, ci, ci, ci)
+    fn("agdx_time_variants", ci, vp, vp)
+    fn("agdx_set_time_variant", ci, vp, ci, ci)
     fn("agdx_data_dev", vp, vp)
     fn("agdx_data_len", ctypes.c_int64, vp)
     fn("agdx_warm_model_dev", ci, vp, vp)
@@ -126,7 +128,7 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend", "
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
            "agdx_warm_model", "agdx_set_spatial_precision", "agdx_autotune",
            "agdx_n_spatial_classes", "agdx_n_time_classes", "agdx_time_factors",
-           "agdx_set_time_factors", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
+           "agdx_set_time_factors", "agdx_time_variants", "agdx_set_time_variant", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
            "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
            "agdx_get_model", "agdx_trace_len", "agdx_operator", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback", "agdx_last_warm_ratio",
@@ -351,6 +353,18 @@ class Problem:
     def set_time_factors(self, c, m1, m2):
         if self.L.agdx_set_time_factors(self.h, int(c), int(m1), int(m2)):
             raise ValueError("no time kernel %d x %d for class %d" % (m1, m2, c))
+
+    def time_variants(self):
+        """Forward time kernel variant of each time class (0 register
+        prefetch, 1 lean tile)."""
+        n = int(self.L.agdx_n_time_classes(self.h))
+        out = np.zeros(max(n, 1), C_INT)
+        self.L.agdx_time_variants(self.h, _p(out))
+        return [int(v) for v in out[:n]]
+
+    def set_time_variant(self, c, var):
+        if self.L.agdx_set_time_variant(self.h, int(c), int(var)):
+            raise ValueError("no variant %d for time class %d" % (var, c))
 
     def data_dev(self):
         """Device address of the patch data set by set_data (flat, patch

@@ -339,9 +339,11 @@ def main():
         tf = pm.time_factors()
         tune = dict(planes_per_task=nk, spatial_ms=[round(t, 4) for t in ms],
                     time_factors=["%dx%d" % f for f in tf],
+                    fwd_time_variants=pm.time_variants(),
                     tuning_s=round(time.perf_counter() - t0, 3))
-        log("autotune: time factorizations %s; planes per spatial task %s (%s ms), %.2f s" %
-            (tune["time_factors"], nk, ", ".join("%.3f" % t for t in ms), tune["tuning_s"]))
+        log("autotune: time factorizations %s (forward variants %s); planes per spatial task "
+            "%s (%s ms), %.2f s" % (tune["time_factors"], tune["fwd_time_variants"], nk,
+                                    ", ".join("%.3f" % t for t in ms), tune["tuning_s"]))
     n_windows = pm.n_windows
     by = pm.bytes()
     b_fk = 3 * by["M"] + 2 * by["T"]

@@ -142,6 +142,12 @@ int agdx_n_spatial_classes(AgdProblem const *p);
  * kernels). agdx_time_factors returns the number of classes. */
 int agdx_n_time_classes(AgdProblem const *p);
 int agdx_time_factors(AgdProblem const *p, int *out);
+/* Forward time kernel variant per time class (out: one int per class):
+ * 0 the next time window's tile prefetched into registers, 1 the lean tile
+ * (one LDS tile for input and transform, the carry in registers, more
+ * blocks per CU). The autotuner times both with every factorization. */
+int agdx_time_variants(AgdProblem const *p, int *out);
+int agdx_set_time_variant(AgdProblem *p, int c, int var);
 /* Sets time class c to the factorization m1 x m2 of its half length (one
  * of the autotuner's candidates). */
 int agdx_set_time_factors(AgdProblem *p, int c, int m1, int m2);

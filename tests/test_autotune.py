@@ -25,8 +25,8 @@ TIME_CAND = {40: [(5, 8), (8, 5), (4, 10)], 80: [(10, 8), (8, 10), (16, 5)],
 @pytest.mark.parametrize("wt", [64, 128, 256, 512])
 def test_every_time_factorization_matches_fp64_oracle(wt):
     """Forward and adjoint operators with each candidate factorization of
-    the time transform: relative error <= 2e-6 against the FP64 oracle and
-    the FP64 dot test to 1e-6."""
+    the time transform and each variant of the forward kernel: relative
+    error <= 2e-6 against the FP64 oracle and the FP64 dot test to 1e-6."""
     geo = _spec_geometry((64, 2048), (32, wt), 40 + wt)
     kw = _kw(geo)
     om = O.Model(double=True, **kw)
@@ -40,12 +40,13 @@ def test_every_time_factorization_matches_fp64_oracle(wt):
         y = _mute_vec(om, rng.random(om.record_len) - 0.5)
         ref = om.forward(x)
         gref = om.adjoint_update(y, -1.0, om.zeros_model())
-        for m1, m2 in TIME_CAND[M]:
+        for (m1, m2), var in [(f, v) for f in TIME_CAND[M] for v in (0, 1)]:
             pm.set_time_factors(0, m1, m2)
-            assert pm.time_factors() == [(m1, m2)]
+            pm.set_time_variant(0, var)
+            assert pm.time_factors() == [(m1, m2)] and pm.time_variants() == [var]
             got = pm.forward(_to_product_layout(om, x).astype(np.complex64)).astype(np.float64)
             rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
-            assert rel <= 2e-6, (m1, m2, rel)
+            assert rel <= 2e-6, (m1, m2, var, rel)
             gp = _to_oracle_layout(om, pm.adjoint(y.astype(np.float32)).astype(np.complex128))
             rel_g = np.linalg.norm(gp - gref) / np.linalg.norm(gref)
             assert rel_g <= 2e-6, (m1, m2, rel_g)
@@ -71,8 +72,10 @@ def test_autotuned_problem_matches_fp64_oracle(name, geo):
         pm.set_data(blended)
         nk, ms = pm.autotune(reps=2)
         tf = pm.time_factors()
-        print(name, "nk", nk, "ms", ["%.3f" % t for t in ms], "time factors", tf)
+        print(name, "nk", nk, "ms", ["%.3f" % t for t in ms], "time factors", tf,
+              "variants", pm.time_variants())
         assert all(f in TIME_CAND[f[0] * f[1]] for f in tf if f[0])
+        assert all(v in (0, 1) for v in pm.time_variants())
         assert len(nk) >= 1 and all(k in CAND for k in nk if k)
         assert all(t > 0 for k, t in zip(nk, ms) if k)
         pm.run(1.0, n_its)
