@@ -9,8 +9,8 @@
 //
 // 3x3x3 convolutions (stride 1, zero padding 1) are implicit GEMMs on the
 // matrix cores: one wavefront computes 16 consecutive voxels along w by all
-// output channels with v_mfma_f32_16x16x16_bf16 (M = voxels, N = 16 output
-// channels, K = 16 input channels of one tap per instruction). The input
+// output channels with v_mfma_f32_16x16x32_bf16 (M = voxels, N = 16 output
+// channels, K = the 16 input channels of two taps per instruction). The input
 // gradient is the same kernel run on the flipped, transposed weights; the
 // weight gradient is a deterministic two-pass reduction (per-block FP32
 // partial sums over voxel slabs, then a fixed-order sum).
@@ -23,6 +23,15 @@
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][8 (l / 16) + e] and
+// B[8 (l / 16) + e][l % 16], e < 8, as 8 bf16 in a u32x4; D as for K = 16
+__device__ __forceinline__ f32x4 mfma32(u32x4 a, u32x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
 
 #define UT 256
 
@@ -64,9 +73,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // weights ([27][CO][16]) once and walks the DB planes with a ring of three
 // input planes in LDS ((CT_H + 2) rows x (CT_W + 2) voxels x 16 channels
 // each; one new plane per step), so the input is read (DB + 2) / DB times
-// instead of three. Both MFMA operands are contiguous 512-byte LDS reads per
-// wave. Staging loads are branch-free (clamped address, zero selected
-// afterwards) so a thread's loads stay in flight together.
+// instead of three. Each matrix-core step takes two taps: K = 32 is the 16
+// channels of tap 2 s (lanes 0-31) and of tap 2 s + 1 (lanes 32-63), 14
+// steps for the 27 taps (the last one's second tap has zero weights); both
+// operands are 16-byte LDS reads per lane. Staging loads are branch-free
+// (clamped address, zero selected afterwards) so a thread's loads stay in
+// flight together.
 // grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / DB)); block 256.
 // mode bits: 1 = add bias, 2 = ReLU on the output, 4 = multiply the output
 // by (gate > 0) (the ReLU mask of the layer below, for input gradients).
@@ -85,6 +97,19 @@ __device__ __forceinline__ bf16x4 ld_swz(uint16_t const *base, int row, int q) {
 __device__ __forceinline__ void st_swz(uint16_t *base, int row, int half, uint4 v) {
   *(uint2 *)(base + swz_q(row, 2 * half)) = make_uint2(v.x, v.y);
   *(uint2 *)(base + swz_q(row, 2 * half + 1)) = make_uint2(v.z, v.w);
+}
+// The K = 32 operands read 16 bytes (8 channels, one half of a row) per
+// lane: rows of 16 channels with their two halves swapped when (row >> 3) &
+// 1, so the 16 lanes of an operand (16 consecutive rows, one half) cover
+// the 64 banks once
+__device__ __forceinline__ int swz_h(int row, int half) {
+  return row * 16 + ((half ^ ((row >> 3) & 1)) << 3);
+}
+__device__ __forceinline__ u32x4 ld_h(uint16_t const *base, int row, int half) {
+  return *(u32x4 const *)(base + swz_h(row, half));
+}
+__device__ __forceinline__ void st_h(uint16_t *base, int row, int half, uint4 v) {
+  *(uint4 *)(base + swz_h(row, half)) = v;
 }
 
 template <int CO, int CU, bool UPC = false>
@@ -112,8 +137,8 @@ __host__ __device__ constexpr int up_tap(int pd, int ph, int a, int b, int kw) {
 // formed: input channels [0, CU) are xu's ([n][D/2][H/2][W/2][CU], nearest
 // x2 upsampling, D, H, W even) and [CU, CI) are x's ([n][D][H][W][CI - CU]).
 // UPC: the xu channels use the 48 combined taps wcu ([48][CO][CU], built by
-// k_up_weights from wt) instead of wt's 27, 12 matrix-core steps per
-// 16-channel chunk instead of 27.
+// k_up_weights from wt) instead of wt's 27: per 16-channel chunk 12
+// combined taps, 6 matrix-core steps of two, instead of 27 taps in 14.
 template <int CI, int CO, int CU, bool UPC = false>
 __global__ __launch_bounds__(UT) void k_conv3(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ xu,
@@ -148,7 +173,7 @@ __global__ __launch_bounds__(UT) void k_conv3(
   int const h0 = blockIdx.y * CT_H;
   int const w0 = blockIdx.x * CT_W;
   int const i = lane & 15;   // A row: voxel; B / D column: channel in the tile
-  int const kq = lane >> 4;  // k quarter: channels 4 kq .. 4 kq + 3
+  int const kq = lane >> 4;  // k group: channels 8 (kq & 1) .. + 7 of the step's tap kq >> 1
   f32x4 acc[DB][4][NT];
 #pragma unroll
   for (int j = 0; j < DB; ++j)
@@ -201,14 +226,14 @@ __global__ __launch_bounds__(UT) void k_conv3(
     for (int k = 0; k < NPK; ++k) {
       int const e0 = tid + k * UT;
       int const e = e0 < NPI ? e0 : NPI - 1;
-      st_swz(dst, e >> 1, e & 1, v[k]);
+      st_h(dst, e >> 1, e & 1, v[k]);
     }
   };
-  // operands of one row of taps: A for the 4 voxel tiles and 3 kw shifts,
-  // B for the 3 taps and NT channel tiles
+  // operands of one matrix-core step (two taps): A for the 4 voxel tiles,
+  // B for the NT output channel tiles
   struct Ops {
-    bf16x4 a[3][4];
-    bf16x4 b[3][NT];
+    u32x4 a[4];
+    u32x4 b[NT];
   };
   if constexpr (UPC) {
     // The upsampled channels, on the 48 combined taps ([48][CO][CU]) and
@@ -249,13 +274,13 @@ __global__ __launch_bounds__(UT) void k_conv3(
           int const half = e & 1;
           int const tc = e >> 1;  // combined tap * CO + co
           uint4 const v = *(uint4 const *)(wcu + (int64_t)tc * CU + c * 16 + half * 8);
-          st_swz(sw, tc, half, v);
+          st_h(sw, tc, half, v);
         }
 #pragma unroll
         for (int k = 0; k < NCK; ++k) {
           int const e0 = tid + k * UT;
           int const e = e0 < NCI ? e0 : NCI - 1;
-          st_swz(sx, e >> 1, e & 1, cv[k]);  // past the end: the last item again
+          st_h(sx, e >> 1, e & 1, cv[k]);  // past the end: the last item again
         }
       }
       __syncthreads();
@@ -263,40 +288,36 @@ __global__ __launch_bounds__(UT) void k_conv3(
 #pragma unroll
       for (int j = 0; j < DB; ++j) {
         int const pd = (d0 + j) & 1;
-        // rows (a, b): fine plane d - 1 + pd + a and fine row h - 1 + ph + b,
-        // i.e. coarse plane (j + 1) / 2 + a and row (wave + 1) / 2 + b of the
-        // staging; fine column w0 + x (x = 16 m + i + kw - 1 >= -1) is
+        // combined tap u = (a 2 + b) 3 + kw (12 of them, 6 steps of two):
+        // rows (a, b) are fine plane d - 1 + pd + a and fine row h - 1 + ph +
+        // b, i.e. coarse plane (j + 1) / 2 + a and row (wave + 1) / 2 + b of
+        // the staging; fine column w0 + x (x = 16 m + i + kw - 1 >= -1) is
         // coarse column floor(x / 2) + 1
-        auto load_up = [&](int ab, Ops &o) {
+        auto load_up = [&](int st, Ops &o) {
+          int const u = 2 * st + (kq >> 1);
+          int const ab = u / 3, kw = u - 3 * ab;
           int const a = ab >> 1, b = ab & 1;
           int const v0 = (((j + 1) / 2 + a) * CPH + (wave + 1) / 2 + b) * CPW + 1;
 #pragma unroll
-          for (int kw = 0; kw < 3; ++kw) {
+          for (int m = 0; m < 4; ++m)
+            o.a[m] = ld_h(sx, v0 + ((16 * m + i + kw - 1) >> 1), kq & 1);
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
-              o.a[kw][m] = ld_swz(sx, v0 + ((16 * m + i + kw - 1) >> 1), kq);
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-              o.b[kw][t] = ld_swz(sw, up_tap(pd, ph, a, b, kw) * CO + t * 16 + i, kq);
-          }
+          for (int t = 0; t < NT; ++t)
+            o.b[t] = ld_h(sw, up_tap(pd, ph, a, b, kw) * CO + t * 16 + i, kq & 1);
         };
         auto mma_up = [&](Ops const &o) {
 #pragma unroll
-          for (int kw = 0; kw < 3; ++kw)
+          for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
-#pragma unroll
-              for (int m = 0; m < 4; ++m)
-                acc[j][m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                    o.a[kw][m], o.b[kw][t], acc[j][m][t], 0, 0, 0);
+            for (int m = 0; m < 4; ++m) acc[j][m][t] = mfma32(o.a[m], o.b[t], acc[j][m][t]);
         };
         Ops opa, opb;
         load_up(0, opa);
 #pragma unroll 1
-        for (int r = 1; r < 4; r += 2) {
-          load_up(r, opb);
+        for (int st = 1; st < 6; st += 2) {
+          load_up(st, opb);
           mma_up(opa);
-          load_up(r + 1 < 4 ? r + 1 : r, opa);  // the last row again, unused
+          if (st + 1 < 6) load_up(st + 1, opa);
           mma_up(opb);
         }
       }
@@ -317,7 +338,7 @@ __global__ __launch_bounds__(UT) void k_conv3(
         int const half = e & 1;
         int const tc = e >> 1;  // tap * CO + co
         uint4 const v = *(uint4 const *)(wt + (int64_t)tc * CI + c * 16 + half * 8);
-        st_swz(sw, tc, half, v);  // past the end: the last item again
+        st_h(sw, tc, half, v);  // past the end: the last item again
       }
       store_plane(d0 - 1, v0);
       store_plane(d0, v1);
@@ -331,41 +352,36 @@ __global__ __launch_bounds__(UT) void k_conv3(
       // the slot of plane d - 1
       uint4 nxt[NPK];
       if (j + 1 < DB) load_plane(d + 2, c, nxt);
-      // operands of row r = (kd, kh) of taps, double-buffered so the LDS
-      // reads of the next row overlap this row's MFMAs
-      auto load_ops = [&](int r, Ops &o) {
-        int const kd = r / 3, kh = r - 3 * (r / 3);
-        int const wrow = r * 3;
+      // operands of step st: this lane's tap 2 st + (kq >> 1) (the 28th,
+      // past the last, reads tap 26 with zero weights), double-buffered so
+      // the LDS reads of the next step overlap this step's MFMAs
+      auto load_ops = [&](int st, Ops &o) {
+        int const tl = 2 * st + (kq >> 1);
+        int const t = tl < 27 ? tl : 26;
+        int const kd = t / 9, kh = t / 3 - 3 * kd, kw = t - 3 * (t / 3);
         uint16_t const *const pl = sx + ((d + kd - 1 + 3) % 3) * PLANE;
-        int const v0 = (wave + kh) * PW + i;
+        int const v0 = (wave + kh) * PW + i + kw;
+        unsigned const keep = tl < 27 ? ~0u : 0u;
 #pragma unroll
-        for (int kw = 0; kw < 3; ++kw) {
+        for (int m = 0; m < 4; ++m) o.a[m] = ld_h(pl, v0 + 16 * m, kq & 1);
 #pragma unroll
-          for (int m = 0; m < 4; ++m) o.a[kw][m] = ld_swz(pl, v0 + 16 * m + kw, kq);
-#pragma unroll
-          for (int t = 0; t < NT; ++t) o.b[kw][t] = ld_swz(sw, (wrow + kw) * CO + t * 16 + i, kq);
-        }
+        for (int tt = 0; tt < NT; ++tt) o.b[tt] = ld_h(sw, t * CO + tt * 16 + i, kq & 1) & keep;
       };
       auto mma_ops = [&](Ops const &o) {
 #pragma unroll
-        for (int kw = 0; kw < 3; ++kw)
+        for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-          for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-              acc[j][m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(o.a[kw][m], o.b[kw][t],
-                                                                     acc[j][m][t], 0, 0, 0);
+          for (int m = 0; m < 4; ++m) acc[j][m][tt] = mfma32(o.a[m], o.b[tt], acc[j][m][tt]);
       };
       Ops opa, opb;
       load_ops(0, opa);
 #pragma unroll 1
-      for (int r = 1; r < 9; r += 2) {
-        load_ops(r, opb);
+      for (int st = 1; st < 14; st += 2) {
+        load_ops(st, opb);
         mma_ops(opa);
-        load_ops(r + 1, opa);
+        if (st + 1 < 14) load_ops(st + 1, opa);
         mma_ops(opb);
       }
-      mma_ops(opa);
       if (j + 1 < DB) {
         __syncthreads();  // every wave is done with plane d - 1
         store_plane(d + 2, nxt);
@@ -506,7 +522,7 @@ __global__ __launch_bounds__(UT) void k_up_weights(uint16_t const *__restrict__ 
 // ---------------------------------------------------------------------------
 // First layer: one live input channel (the field), so the taps are the GEMM's
 // K: D[voxel][co] = sum_tap bf16(field[voxel + tap] * scale) w[tap][co][0],
-// K = 27 taps padded to 32 (two K = 16 steps) instead of 27 steps of 16
+// K = 27 taps padded to 32 (one K = 32 step) instead of 27 steps of 16
 // channels. Block: CT_H rows x CT_W voxels of FDB consecutive d planes, a
 // ring of three bf16 planes of the field in LDS; output as in k_conv3
 // (bias, ReLU, 16-byte stores through per-wave LDS rows).
@@ -530,26 +546,24 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
   int const d0 = (blockIdx.z - n * ndb) * FDB;
   int const h0 = blockIdx.y * CT_H, w0 = blockIdx.x * CT_W;
   float const *const fn = src + (int64_t)n * D * H * W;
-  // B: taps 16 ks + 4 kq + r of output channel i (zero past tap 26)
-  bf16x4 bw[2];
+  // B: taps 8 kq + e of output channel i (zero past tap 26): K = 32
+  unsigned bh[8];
 #pragma unroll
-  for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int const t = 16 * ks + 4 * kq + r;
-      bw[ks][r] = t < 27 ? (short)wt[((int64_t)t * 16 + i) * wci] : (short)0;
-    }
+  for (int e = 0; e < 8; ++e) {
+    int const t = 8 * kq + e;
+    bh[e] = t < 27 ? (unsigned)wt[((int64_t)t * 16 + i) * wci] : 0u;
+  }
+  u32x4 const bw = {bh[0] | bh[1] << 16, bh[2] | bh[3] << 16, bh[4] | bh[5] << 16,
+                    bh[6] | bh[7] << 16};
   // LDS offsets of this lane's A taps relative to its voxel (slot, row, col)
-  int toff[2][4], tkd[2][4];
+  int toff[8], tkd[8];
 #pragma unroll
-  for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int const t = 16 * ks + 4 * kq + r;
-      int const tt = t < 27 ? t : 13;  // padded taps read the centre (weight 0)
-      tkd[ks][r] = tt / 9;
-      toff[ks][r] = ((tt / 3) % 3) * PW + tt % 3;
-    }
+  for (int e = 0; e < 8; ++e) {
+    int const t = 8 * kq + e;
+    int const tt = t < 27 ? t : 13;  // padded taps read the centre (weight 0)
+    tkd[e] = tt / 9;
+    toff[e] = ((tt / 3) % 3) * PW + tt % 3;
+  }
   auto stage = [&](int dd) {
     uint16_t *const dst = sx + ((dd + 3) % 3) * PLANE;
     bool const okd = dd >= 0 && dd < D;
@@ -575,14 +589,13 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
     for (int m = 0; m < 4; ++m) {
       acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
       int const vb = wave * PW + 16 * m + i;  // the voxel's (row, col) in a plane
+      unsigned ah[8];
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        bf16x4 a;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          a[r] = (short)sx[((d + tkd[ks][r] - 1 + 3) % 3) * PLANE + vb + toff[ks][r]];
-        acc[m] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[ks], acc[m], 0, 0, 0);
-      }
+      for (int e = 0; e < 8; ++e)
+        ah[e] = sx[((d + tkd[e] - 1 + 3) % 3) * PLANE + vb + toff[e]];
+      u32x4 const a = {ah[0] | ah[1] << 16, ah[2] | ah[3] << 16, ah[4] | ah[5] << 16,
+                       ah[6] | ah[7] << 16};
+      acc[m] = mfma32(a, bw, acc[m]);
     }
     // D: lane holds voxels 16 m + 4 kq + r of channel i
 #pragma unroll
