@@ -613,6 +613,18 @@ static int smooth_even(int n) {
   }
 }
 
+/* real length of the wavelet convolution of n >= nconv + nfk - 1 samples: a
+ * length with a compile-time kernel when one is at most 25% longer than the
+ * smallest even 7-smooth length, else that length (the generic kernel) */
+static int wavelet_len(int n) {
+  static int const half[] = AGD_WAVELET_HALF_LENGTHS;
+  int const m = smooth_even(n);
+  size_t i;
+  for (i = 0; i < sizeof half / sizeof half[0]; i++)
+    if (2 * half[i] >= n) return 4 * (2 * half[i]) <= 5 * m ? 2 * half[i] : m;
+  return m;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Problem                                                                   */
 /* ------------------------------------------------------------------------ */
@@ -1479,7 +1491,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
       size_t q;
       for (w = 0; w < p->n_win; w++) {
         AgdWin *W = &p->win[w];
-        int const M = smooth_even(W->nconv + W->nfk - 1);
+        int const M = wavelet_len(W->nconv + W->nfk - 1);
         W->wM = M;
         W->wplan = planset_get(&ps, M / 2);
         W->wplan_full = planset_get(&ps, M);
@@ -1629,6 +1641,9 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
         !p->dev.partials || !p->dev.tile_ptr || !p->dev.tile_ent ||
         !p->dev.mute || !p->dev.wspec)
       goto err;
+    /* the wavelet adjoint writes each trace of T2 only below n_times_fk
+     * (all that the adjoint time transform reads): the rest stays zero */
+    if (!p->host_only && p->use_wavelets && agdg_zero(p->dev.T2, (size_t)to * 4)) goto err;
   }
   /* user-data blend coordinates (blend_input, blend.c:1294-1335) and
    * output assembly lists (set_output, model.c:1425-1515) */
@@ -1868,7 +1883,7 @@ EXPORT int agdx_window_info(AgdProblem const *p, int w, int *info) {
   if (w < 0 || w >= p->n_win) return 1;
   W = &p->win[w];
   s = &p->sw[w];
-  memset(info, 0, 24 * sizeof(int));
+  memset(info, 0, 25 * sizeof(int));
   info[0] = W->patch;
   info[1] = W->nsp;
   info[2] = W->ntr;
@@ -1886,6 +1901,7 @@ EXPORT int agdx_window_info(AgdProblem const *p, int w, int *info) {
     info[16 + d] = W->W[d];
     info[20 + d] = s->coords[d];
   }
+  info[24] = p->use_wavelets ? W->wM : 0;
   return 0;
 }
 

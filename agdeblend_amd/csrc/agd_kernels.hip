@@ -439,8 +439,9 @@ __global__ __launch_bounds__(NTHR) void k_fwd_time(
 // modes: 0 sum, 1 mean (taper weights), 2 overwrite, 3 sum - d (residual),
 // 4 per-entry constant sum (stepsize fold)
 // ---------------------------------------------------------------------------
+template <int MODE>
 __global__ __launch_bounds__(NTHR) void k_blend(
-    int mode, int64_t rec_len, int tile, int const *__restrict__ tile_ptr,
+    int64_t rec_len, int tile, int const *__restrict__ tile_ptr,
     int const *__restrict__ tile_ent, AgdEntry const *__restrict__ ent,
     float const *__restrict__ src, float const *__restrict__ cst,
     float const *__restrict__ taper, int taper_len,
@@ -459,6 +460,9 @@ __global__ __launch_bounds__(NTHR) void k_blend(
     int const o = ob + (int)threadIdx.x;
     int64_t const s = base + o;
     bool const live = o < tile && s < rec_len;
+    // loaded before the entry loop: a barrier does not wait for them
+    bool const muted = live && mute != nullptr && mute[s];
+    float const dv = MODE == 3 && live ? d[s] : 0.f;
     float acc = 0.f;
     float cnt = 0.f;
     for (int c0 = e0; c0 < e1; c0 += NTHR) {
@@ -473,25 +477,26 @@ __global__ __launch_bounds__(NTHR) void k_blend(
         s_id[threadIdx.x] = id;
       }
       __syncthreads();
-      // four entries at a time: their loads are issued together (clamped
+      // BG entries at a time: their loads are issued together (clamped
       // indices, values outside an entry's range unused), then added in the
       // entries' order, so the sums are those of the serial loop
-      for (int q = 0; q < nc; q += 4) {
-        float v[4], wg[4];
-        bool ok[4];
+      constexpr int BG = 16;
+      for (int q = 0; q < nc; q += BG) {
+        float v[BG], wg[BG];
+        bool ok[BG];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < BG; ++u) {
           int const qq = min(q + u, nc - 1);
           int64_t const i = s - s_rec[qq];
           int const n = s_n[qq];
           ok[u] = live && (q + u < nc) && i >= 0 && i < n;
           int const ii = ok[u] ? (int)i : 0;
           wg[u] = 1.f;
-          if (mode == 4) {
+          if (MODE == 4) {
             v[u] = cst[s_id[qq]];
           } else {
             v[u] = src[s_src[qq] + ii];
-            if (mode == 1) {
+            if (MODE == 1) {
               int const to_end = n - 1 - ii;
               int const edge = ii < to_end ? ii : to_end;
               wg[u] = (taper_len > 1 && edge >= 0 && edge < taper_len) ? taper[edge] : 1.f;
@@ -499,12 +504,12 @@ __global__ __launch_bounds__(NTHR) void k_blend(
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < BG; ++u) {
           if (!ok[u]) continue;
-          if (mode == 1) {
+          if (MODE == 1) {
             acc += v[u] * wg[u];
             cnt += wg[u];
-          } else if (mode == 2) {
+          } else if (MODE == 2) {
             acc = v[u];
           } else {
             acc += v[u];
@@ -513,10 +518,9 @@ __global__ __launch_bounds__(NTHR) void k_blend(
       }
     }
     if (live) {
-      if (mode == 1 && cnt > 0.f) acc /= cnt;
-      bool const muted = mute != nullptr && mute[s];
-      if (mode == 3) {
-        out[s] = muted ? 0.f : acc - d[s];
+      if (MODE == 1 && cnt > 0.f) acc /= cnt;
+      if (MODE == 3) {
+        out[s] = muted ? 0.f : acc - dv;
       } else {
         out[s] = muted ? 0.f : acc;
       }
@@ -873,25 +877,141 @@ __global__ __launch_bounds__(NTHR) void k_wavelet(
 
 // ---------------------------------------------------------------------------
 // Wavelet convolution with a compile-time transform length: the real length
-// Mr = 2 MH of the window's convolution, MH = M1 M2. One wave per window
-// trace; the half-length complex FFT (z[n] = x[2n] + i x[2n+1]) is a
-// four-step transform in one padded LDS tile [M1][M2 + 1]: forward over the
-// columns (length M1, register codelets), twiddle, then over the rows
-// (length M2), leaving Z[k1 + M1 k2] at (k1, k2); the real-spectrum unpack,
-// the product with the wavelet spectrum and the pack for the inverse run in
-// place on the pairs (k, MH - k); the inverse runs rows, twiddle, columns,
-// which returns the natural order. The same arithmetic as k_wavelet.
+// Mr = 2 MH of the window's convolution, MH the product of the radices Rs
+// (the host picks Mr among the lengths built here when one is close to the
+// smallest admissible length). One block of NT threads per window trace; the
+// half-length complex FFT (z[n] = x[2n] + i x[2n+1]) runs as the radix
+// passes of a Stockham transform through LDS tiles of MH values: in place
+// (PP false: each pass reads every butterfly's inputs into registers, waits
+// for the block, then writes) or between two tiles (PP true: one barrier
+// per pass). One float2 of padding per 32 spreads the strided writes of the
+// early passes over the banks; every pass length is a multiple of 32, so a
+// butterfly's padded addresses are one base plus constant offsets.
+// The forward's first pass reads its inputs straight from memory and the
+// inverse's last pass writes straight to memory; between the two
+// transforms the real-spectrum unpack, the product with the wavelet
+// spectrum and the pack for the inverse run in place on the pairs
+// (k, MH - k). Pruned passes: when the input has samples only in the first
+// NB1 complex values (the forward reads n_times_fk samples), the first
+// pass's DFT is its first input repeated; when only the first MH / RL
+// complex outputs are needed (the adjoint keeps n_times_fk samples), the
+// last pass computes only its output r = 0. The same arithmetic as
+// k_wavelet otherwise. The adjoint writes only t < nfk: the rest of T2 is
+// zeroed when it is allocated and never written.
 // ---------------------------------------------------------------------------
-template <int M1, int M2>
-__global__ __launch_bounds__(64) void k_wavelet_t(
+__device__ __forceinline__ int wpad(int i) { return i + (i >> 5); }
+
+template <int I, int... Rs>
+constexpr int wav_radix() {
+  constexpr int r[] = {Rs...};
+  return r[I];
+}
+// A thread's twiddles of the passes 1 .. NP - 1 (pass 0 has none), kept in
+// registers for the whole kernel: the inverse uses their conjugates (its
+// pass p has the same Ns and butterflies as the forward's). wav_tw_off(p):
+// the first of pass p in that array.
+template <int NT, int... Rs>
+constexpr int wav_tw_off(int p) {
+  constexpr int r[] = {Rs...};
+  constexpr int L = (Rs * ...);
+  int o = 0;
+  for (int i = 1; i < p; ++i) o += ((L / r[i] + NT - 1) / NT) * (r[i] - 1);
+  return o;
+}
+template <int NT, int P, int Ns, int... Rs>
+__device__ __forceinline__ void wav_tw_load(float2 *tw, float2 const *__restrict__ rt, int tid) {
+  if constexpr (P < (int)sizeof...(Rs)) {
+    constexpr int R = wav_radix<P, Rs...>(), L = (Rs * ...), NB = L / R;
+    constexpr int Q = (NB + NT - 1) / NT, TS = L / (Ns * R), O = wav_tw_off<NT, Rs...>(P);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      int const j = min(tid + q * NT, NB - 1);
+      int const k = j % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) tw[O + q * (R - 1) + r - 1] = rt[r * k * TS];  // < L
+    }
+    wav_tw_load<NT, P + 1, Ns * R, Rs...>(tw, rt, tid);
+  }
+}
+
+// One Stockham pass (decimation in time, autosorting) of radix R over the
+// L values in src, Ns the product of the radices before it: butterfly j
+// takes src[j + r L/R], twiddles by exp(S 2 pi i r k / (Ns R)) (tw, for
+// Ns > 1), k = j mod Ns, and writes dst[(j - k) R + k + r Ns] (dst == src
+// unless PP).
+template <int R, int Ns, int L, int S, int NT, bool PP>
+__device__ __forceinline__ void wav_pass(float2 const *src, float2 *dst, float2 const *tw,
+                                         int tid) {
+  constexpr int NB = L / R, Q = (NB + NT - 1) / NT;
+  static_assert(NB % 32 == 0, "padded offsets need pass lengths that are multiples of 32");
+  float2 v[Q][R];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    int const j = min(tid + q * NT, NB - 1);
+    int const b = wpad(j);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = src[b + r * (NB + NB / 32)];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 t = tw[q * (R - 1) + r - 1];
+        if (S > 0) t.y = -t.y;
+        v[q][r] = cmul(v[q][r], t);
+      }
+    }
+    agdfft::fft<R, S>(v[q]);
+  }
+  if (!PP) __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    int const j = tid + q * NT;
+    if (j < NB) {
+      int const k = j % Ns;
+      int const w0 = (j - k) * R + k;
+      if (Ns % 32 == 0) {
+        int const b = wpad(w0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[b + r * (Ns + Ns / 32)] = v[q][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[wpad(w0 + r * Ns)] = v[q][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// the passes I .. E - 1 of the radix list Rs, from s0 (to s1 and back with PP)
+template <int S, int NT, bool PP, int L, int Ns, int I, int E, int... Rs>
+__device__ __forceinline__ void wav_passes(float2 *s0, float2 *s1, float2 const *tw, int tid) {
+  if constexpr (I < E) {
+    constexpr int R = wav_radix<I, Rs...>();
+    wav_pass<R, Ns, L, S, NT, PP>(s0, PP ? s1 : s0, tw + (I > 0 ? wav_tw_off<NT, Rs...>(I) : 0),
+                                  tid);
+    wav_passes<S, NT, PP, L, Ns * R, I + 1, E, Rs...>(PP ? s1 : s0, PP ? s0 : s1, tw, tid);
+  }
+}
+
+template <int NT, bool PP, int... Rs>
+__global__ __launch_bounds__(NT) void k_wavelet_t(
     AgdWin const *__restrict__ wins, int const *__restrict__ trace_win, int trace_first,
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float2 const *__restrict__ wspec, int const *__restrict__ wsrc_off, int adjoint,
     AgdEntry const *__restrict__ went, float const *__restrict__ rec, float *__restrict__ T,
     float *__restrict__ T2) {
-  constexpr int MH = M1 * M2, LD = M2 + 1, MR = 2 * MH;
-  __shared__ float2 sa[M1 * LD];
-  int const lane = threadIdx.x;
+  constexpr int NP = sizeof...(Rs);
+  constexpr int MH = (Rs * ...), MR = 2 * MH, MP = MH + MH / 32;
+  constexpr int R1 = wav_radix<0, Rs...>(), RL = wav_radix<NP - 1, Rs...>();
+  constexpr int NB1 = MH / R1, NBL = MH / RL;  // NBL: the last pass's Ns
+  static_assert(NB1 % 32 == 0 && NBL % 32 == 0, "pass lengths must be multiples of 32");
+  __shared__ float2 sa[PP ? 2 * MP : MP];
+  // with PP the first pass writes tile 0 and each later pass the other
+  // tile: the forward transform ends in tile (NP - 1) % 2 (the spectrum
+  // step works there in place), the inverse before its last pass in tile 0
+  float2 *const sf = sa + (PP && (NP - 1) % 2 ? MP : 0);
+  float2 *const sx = PP ? sa + (sf == sa ? MP : 0) : sa;  // the other tile
+  float2 *const sg = sa;
+  int const tid = threadIdx.x;
   int const g = trace_first + blockIdx.x;
   AgdWin const w = wins[trace_win[g]];
   int const tr = g - w.trace0;
@@ -906,22 +1026,29 @@ __global__ __launch_bounds__(64) void k_wavelet_t(
     // not multiplied: C2R(R2C(x)) = N x
     float const fN = (float)N;
     if (!adjoint) {
-      for (int t = lane; t < N; t += 64) Tio[t] = t < nfk ? Tio[t] * fN : 0.f;
+      for (int t = tid; t < N; t += NT) Tio[t] = t < nfk ? Tio[t] * fN : 0.f;
     } else {
-      for (int t = lane; t < N; t += 64) {
+      for (int t = tid; t < nfk; t += NT) {
         int const i2 = t - e.trace_start;
         float const x = (i2 >= 0 && i2 < e.n) ? rec[e.rec + i2] : 0.f;
-        Tout[t] = t < nfk ? x * fN : 0.f;
+        Tout[t] = x * fN;
       }
     }
     return;
   }
-  float2 const *const rt = roots + plans[w.wplan].roots_off;      // e^{-2 pi i m / MH}
+  float2 const *const rt = roots + plans[w.wplan].roots_off;          // e^{-2 pi i m / MH}
   float2 const *const rfull = roots + plans[w.wplan_full].roots_off;  // e^{-2 pi i m / MR}
   float2 const *const hs = wspec + so;
-  // the trace's samples, all loads in flight together: one source (the trace
-  // buffer, or the record through the blend entry), clamped index, value
-  // masked outside the range (no branch around a load)
+  // the twiddles of both transforms are loaded here, before the first
+  // barrier (a barrier does not wait for global loads, so their latency
+  // hides behind the first pass; loaded pass by pass they would each wait
+  // behind a barrier)
+  constexpr int NTW = wav_tw_off<NT, Rs...>(NP) > 0 ? wav_tw_off<NT, Rs...>(NP) : 1;
+  float2 tw[NTW];
+  wav_tw_load<NT, 1, R1, Rs...>(tw, rt, tid);
+  // the trace's samples: one source (the trace buffer, or the record through
+  // the blend entry), clamped index, value masked outside the range (no
+  // branch around a load)
   float const *const src = adjoint ? rec : Tio;
   int64_t const sbase = adjoint ? e.rec : 0;
   int const sstart = adjoint ? e.trace_start : 0;
@@ -932,111 +1059,141 @@ __global__ __launch_bounds__(64) void k_wavelet_t(
     float const x = src[sbase + (ok ? i2 : 0)];
     return ok ? x : 0.f;
   };
+  // forward transform, first pass from memory into sa (Ns = 1: no twiddles)
   {
-    constexpr int NL = (MH + 63) / 64;
-    float2 vin[NL];
+    constexpr int Q = (NB1 + NT - 1) / NT;
+    if (2 * NB1 >= nin) {
+      // samples only in z[0, NB1): every output of butterfly j is z[j]
 #pragma unroll
-    for (int q = 0; q < NL; ++q) {
-      int const n = lane + 64 * q;
-      vin[q] = make_float2(sample(2 * n), sample(2 * n + 1));
+      for (int q = 0; q < Q; ++q) {
+        int const j = tid + q * NT;
+        if (j < NB1) {
+          float2 const x = make_float2(sample(2 * j), sample(2 * j + 1));
+#pragma unroll
+          for (int r = 0; r < R1; ++r) sa[wpad(j * R1 + r)] = x;
+        }
+      }
+    } else {
+      float2 v[Q][R1];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        int const j = min(tid + q * NT, NB1 - 1);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          int const n = j + r * NB1;
+          v[q][r] = make_float2(sample(2 * n), sample(2 * n + 1));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        int const j = tid + q * NT;
+        agdfft::fft<R1, -1>(v[q]);
+        if (j < NB1) {
+#pragma unroll
+          for (int r = 0; r < R1; ++r) sa[wpad(j * R1 + r)] = v[q][r];
+        }
+      }
     }
-#pragma unroll
-    for (int q = 0; q < NL; ++q) {
-      int const n = lane + 64 * q;
-      if (n < MH) sa[(n / M2) * LD + n % M2] = vin[q];
-    }
   }
   __syncthreads();
-  // forward: columns, twiddle, rows
-  if (lane < M2) {
-    float2 v[M1];
-#pragma unroll
-    for (int a = 0; a < M1; ++a) v[a] = sa[a * LD + lane];
-    agdfft::fft<M1, -1>(v);
-#pragma unroll
-    for (int a = 0; a < M1; ++a) sa[a * LD + lane] = cmul(v[a], rt[(lane * a) % MH]);
-  }
-  __syncthreads();
-  if (lane < M1) {
-    float2 v[M2];
-#pragma unroll
-    for (int b = 0; b < M2; ++b) v[b] = sa[lane * LD + b];
-    agdfft::fft<M2, -1>(v);
-#pragma unroll
-    for (int b = 0; b < M2; ++b) sa[lane * LD + b] = v[b];
-  }
-  __syncthreads();
+  wav_passes<-1, NT, PP, MH, R1, 1, NP, Rs...>(sa, sa + (PP ? MP : 0), tw, tid);
   // spectrum: real unpack, product with the wavelet, pack for the inverse,
-  // in place on the pairs (k, MH - k)
-  auto zpos = [](int k) { return (k % M1) * LD + k / M1; };
-  auto unpack = [&](float2 zk, float2 zo, int k) {
+  // in place on the pairs (k, MH - k) of sf; wk = e^{-2 pi i k / MR}
+  auto unpack = [&](float2 zk, float2 zo, float2 wk, float2 h, int k) {
     float2 const zc = make_float2(zo.x, -zo.y);
     float2 const ev = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
     float2 const o = make_float2(0.5f * (zk.y - zc.y), -0.5f * (zk.x - zc.x));
-    float2 X = cadd(ev, cmul(o, rootf(rfull, MR, k, -1)));
+    float2 X = cadd(ev, cmul(o, wk));
     if (k == 0 || k == MH) X.y = 0.f;
-    float2 h = hs[k];
     if (adjoint) h.y = -h.y;
     return cmul(X, h);
   };
-  auto pack = [&](float2 xk, float2 xo, int k) {
+  auto pack = [&](float2 xk, float2 xo, float2 wk, int k) {
     if (k == 0) {
       xk.y = 0.f;
       xo.y = 0.f;
     }
     float2 const xc = make_float2(xo.x, -xo.y);
     float2 const ev = cadd(xk, xc);
-    float2 const o = cmul(csub(xk, xc), rootf(rfull, MR, k, +1));
+    float2 const o = cmul(csub(xk, xc), make_float2(wk.x, -wk.y));
     return make_float2(ev.x - o.y, ev.y + o.x);
   };
-  // unrolled over the compile-time trip count with the index clamped (a
-  // clamped lane repeats the last pair: the same values to the same places),
-  // so the spectrum and twiddle loads need not wait behind the LDS traffic
-  constexpr int KQ = (MH / 2 + 1 + 63) / 64;
-#pragma unroll
-  for (int q = 0; q < KQ; ++q) {
-    int const k = min(lane + 64 * q, MH / 2);
+  auto pair = [&](int k, float2 hk, float2 hc, float2 wk, bool write) {
     int const kc = MH - k;
-    float2 const za = sa[zpos(k)];
-    float2 const zb = sa[zpos(kc % MH)];
-    float2 const hk = unpack(za, zb, k);
-    float2 const hc = unpack(zb, za, kc);
-    sa[zpos(k)] = pack(hk, hc, k);
-    if (k != 0 && k != kc) sa[zpos(kc)] = pack(hc, hk, kc);
-  }
-  __syncthreads();
-  // inverse: rows, twiddle, columns
-  if (lane < M1) {
-    float2 v[M2];
+    float2 const wc = make_float2(-wk.x, wk.y);  // e^{-2 pi i kc / MR}
+    float2 const za = sf[wpad(k)];
+    float2 const zb = sf[wpad(kc % MH)];
+    float2 const xk = unpack(za, zb, wk, hk, k);
+    float2 const xc = unpack(zb, za, wc, hc, kc);
+    if (write) {
+      sf[wpad(k)] = pack(xk, xc, wk, k);
+      if (k != 0 && k != kc) sf[wpad(kc)] = pack(xc, xk, wc, kc);
+    }
+  };
+  {
+    // unrolled over the compile-time trip count with the index clamped for
+    // the reads; a clamped thread writes nothing (its pair belongs to a
+    // thread of another wave, which may already have written it)
+    constexpr int KQ = (MH / 2 + 1 + NT - 1) / NT;
 #pragma unroll
-    for (int b = 0; b < M2; ++b) v[b] = sa[lane * LD + b];
-    agdfft::fft<M2, +1>(v);
-#pragma unroll
-    for (int b = 0; b < M2; ++b) {
-      float2 t = rt[(lane * b) % MH];
-      t.y = -t.y;
-      sa[lane * LD + b] = cmul(v[b], t);
+    for (int q = 0; q < KQ; ++q) {
+      int const k0 = tid + q * NT;
+      int const k = min(k0, MH / 2);
+      pair(k, hs[k], hs[MH - k], rfull[k], k0 <= MH / 2);
     }
   }
   __syncthreads();
-  if (lane < M2) {
-    float2 v[M1];
+  wav_passes<+1, NT, PP, MH, 1, 0, NP - 1, Rs...>(sf, sx, tw, tid);
+  // inverse transform, last pass from sg to memory: butterfly j < NBL (its
+  // Ns) writes z[j + r NBL], i.e. the samples 2 (j + r NBL) and
+  // 2 (j + r NBL) + 1
+  {
+    constexpr int Q = (NBL + NT - 1) / NT, TL = wav_tw_off<NT, Rs...>(NP - 1);
+    int const nout = adjoint ? nfk : N;
+    float *const dst = adjoint ? Tout : Tio;
+    if (nout <= 2 * NBL) {
+      // only the outputs r = 0: z[j] = sum_m twiddled inputs
 #pragma unroll
-    for (int a = 0; a < M1; ++a) v[a] = sa[a * LD + lane];
-    agdfft::fft<M1, +1>(v);
+      for (int q = 0; q < Q; ++q) {
+        int const j = tid + q * NT;
+        if (2 * j < nout) {
+          int const b = wpad(j);
+          float2 acc = sg[b];
 #pragma unroll
-    for (int a = 0; a < M1; ++a) sa[a * LD + lane] = v[a];
-  }
-  __syncthreads();
-  int const nout = adjoint ? N : N;
-  for (int t = lane; t < nout; t += 64) {
-    int const n = t >> 1;
-    float2 const z = sa[(n / M2) * LD + n % M2];
-    float const y = (t & 1) ? z.y : z.x;
-    if (!adjoint)
-      Tio[t] = y;
-    else
-      Tout[t] = t < nfk ? y : 0.f;
+          for (int m = 1; m < RL; ++m) {
+            float2 t = tw[TL + q * (RL - 1) + m - 1];  // e^{-2 pi i m j / MH}
+            t.y = -t.y;
+            acc = cadd(acc, cmul(sg[b + m * (NBL + NBL / 32)], t));
+          }
+          dst[2 * j] = acc.x;
+          if (2 * j + 1 < nout) dst[2 * j + 1] = acc.y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        int const j = tid + q * NT;
+        if (j < NBL) {
+          int const b = wpad(j);
+          float2 v[RL];
+#pragma unroll
+          for (int r = 0; r < RL; ++r) v[r] = sg[b + r * (NBL + NBL / 32)];
+#pragma unroll
+          for (int r = 1; r < RL; ++r) {
+            float2 t = tw[TL + q * (RL - 1) + r - 1];  // e^{-2 pi i r j / MH}
+            t.y = -t.y;
+            v[r] = cmul(v[r], t);
+          }
+          agdfft::fft<RL, +1>(v);
+#pragma unroll
+          for (int r = 0; r < RL; ++r) {
+            int const t = 2 * (j + r * NBL);
+            if (t < nout) dst[t] = v[r].x;
+            if (t + 1 < nout) dst[t + 1] = v[r].y;
+          }
+        }
+      }
+    }
   }
 }
 
@@ -2332,14 +2489,14 @@ int agdg_fwd_time(AgdDev const *g) {
 }
 
 int agdg_blend_traces(AgdDev const *g, int resid) {
-  return launch("blend", k_blend, dim3(g->n_tiles), dim3(NTHR), 0, resid ? 3 : 0,
+  return launch("blend", resid ? k_blend<3> : k_blend<0>, dim3(g->n_tiles), dim3(NTHR), 0,
                 g->rec_len, g->tile, g->tile_ptr, g->tile_ent, g->went,
                 (float const *)g->T, (float const *)nullptr, (float const *)nullptr, 0,
                 g->mute, (float const *)g->d, g->r);
 }
 
 int agdg_stepsize_fold(AgdDev const *g) {
-  if (launch("fold", k_blend, dim3(g->n_tiles), dim3(NTHR), 0, 4, g->rec_len, g->tile,
+  if (launch("fold", k_blend<4>, dim3(g->n_tiles), dim3(NTHR), 0, g->rec_len, g->tile,
              g->tile_ptr, g->tile_ent, g->went, (float const *)nullptr, g->tconst,
              (float const *)nullptr, 0, (unsigned char const *)nullptr,
              (float const *)nullptr, g->r))
@@ -2455,11 +2612,26 @@ int agdg_wavelet(AgdDev const *g, int adjoint) {
   int const first = g->wtrace0[g->w0];
   int const n = g->wtrace0[g->w1] - first;
   if (n <= 0) return 0;
-  if (g->wavelet_mh == 2800)
-    return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet_t<50, 56>, dim3(n),
-                  dim3(64), 0, g->win, g->trace_win, first, g->plans,
-                  (float2 const *)g->roots, (float2 const *)g->wspec, g->wsrc_off, adjoint,
+#define WAV_CASE(MH, NT, PP, ...)                                                           \
+  case MH:                                                                                  \
+    return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet_t<NT, PP, __VA_ARGS__>, \
+                  dim3(n), dim3(NT), 0, g->win, g->trace_win, first, g->plans,               \
+                  (float2 const *)g->roots, (float2 const *)g->wspec, g->wsrc_off, adjoint, \
                   g->went, (float const *)g->r, g->T, g->T2);
+  // one case per half length of AGD_WAVELET_HALF_LENGTHS
+  switch (g->wavelet_mh) {
+    WAV_CASE(512, 256, true, 8, 8, 8)
+    WAV_CASE(768, 256, true, 8, 12, 8)
+    WAV_CASE(1024, 256, true, 16, 8, 8)
+    WAV_CASE(1536, 512, true, 8, 8, 3, 8)
+    WAV_CASE(2048, 512, true, 8, 8, 4, 8)
+    WAV_CASE(3072, 512, true, 8, 8, 6, 8)
+    WAV_CASE(4096, 512, true, 8, 8, 8, 8)
+    WAV_CASE(6144, 768, false, 8, 8, 12, 8)
+    default:
+      break;
+  }
+#undef WAV_CASE
   return launch(adjoint ? "wavelet_adj" : "wavelet_fwd", k_wavelet, dim3(n),
                 dim3(NTHR), g->lds_wavelet, g->win, g->trace_win, first,
                 g->plans, (float2 const *)g->roots, (float2 const *)g->wspec,
@@ -2488,12 +2660,14 @@ int agdg_output(float const *T, int n_out, int64_t const *o_off, int const *o_nt
 }
 
 int agdg_gather_resid(AgdGather const *gt, float const *src, float const *d, float *rec) {
-  return launch("gather", k_blend, dim3(gt->n_tiles), dim3(NTHR), 0, 3, gt->rec_len, gt->tile,
+  return launch("gather", k_blend<3>, dim3(gt->n_tiles), dim3(NTHR), 0, gt->rec_len, gt->tile,
                 gt->tile_ptr, gt->tile_ent, gt->ent, src, (float const *)nullptr, gt->taper,
                 gt->taper_len, gt->mute, d, rec);
 }
 int agdg_gather(AgdGather const *gt, float const *src, int mode, float *rec) {
-  return launch("gather", k_blend, dim3(gt->n_tiles), dim3(NTHR), 0, mode, gt->rec_len,
+  if (mode < 0 || mode > 2) return 1;
+  return launch("gather", mode == 0 ? k_blend<0> : mode == 1 ? k_blend<1> : k_blend<2>,
+                dim3(gt->n_tiles), dim3(NTHR), 0, gt->rec_len,
                 gt->tile, gt->tile_ptr, gt->tile_ent, gt->ent, src,
                 (float const *)nullptr, gt->taper, gt->taper_len, gt->mute,
                 (float const *)nullptr, rec);

@@ -26,6 +26,10 @@
 
 #define AGD_MAXSP 4      /* AGD_MAX_N_COORDS */
 #define AGD_MAXSTAGE 24  /* FFT stages per plan */
+/* half lengths Mr / 2 of the wavelet convolution with a compile-time kernel
+ * (k_wavelet_t); the host prefers them to the smallest 7-smooth length when
+ * one is at most 25% longer */
+#define AGD_WAVELET_HALF_LENGTHS {512, 768, 1024, 1536, 2048, 3072, 4096, 6144}
 
 /* One FFT length with its radix schedule and roots table
  * roots[m] = exp(-2 pi i m / n), m < n. */

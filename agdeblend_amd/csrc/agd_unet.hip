@@ -202,8 +202,9 @@ __global__ __launch_bounds__(UT) void k_conv3(
     int const pw = r % PW, ph = r / PW;
     int const hh = h0 + ph - 1, ww = w0 + pw - 1;
     iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
-    ioff[k] = iok[k] ? (uint32_t)((hh * W + ww) * CS + half * 8) : (uint32_t)(half * 8);
-    iofu[k] = iok[k] ? (uint32_t)(((hh >> 1) * Wu + (ww >> 1)) * CU + half * 8)
+    ioff[k] = iok[k] ? (uint32_t)(hh * W + ww) * (uint32_t)CS + (uint32_t)(half * 8)
+                     : (uint32_t)(half * 8);
+    iofu[k] = iok[k] ? (uint32_t)((hh >> 1) * Wu + (ww >> 1)) * (uint32_t)CU + (uint32_t)(half * 8)
                      : (uint32_t)(half * 8);
   }
   // global loads of plane dd of chunk c (zeros outside the volume) ...
@@ -674,7 +675,7 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
     int const pw = r % PW, ph = r / PW;
     int const hh = h0 + ph - 1, ww = w0 + pw - 1;
     iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
-    ioff[k] = iok[k] ? (uint32_t)((hh * W + ww) * 16 + half * 8) : (uint32_t)(half * 8);
+    ioff[k] = iok[k] ? (uint32_t)(hh * W + ww) * 16u + (uint32_t)(half * 8) : (uint32_t)(half * 8);
   }
   auto load_plane = [&](int dd, uint4 (&v)[NPK]) {
     bool const okd = dd >= 0 && dd < D;
@@ -1215,6 +1216,9 @@ static inline int64_t blocks_capped(int64_t n, int t) {
 
 extern "C" {
 
+// The kernels index within one d plane in 32-bit unsigned arithmetic (voxel
+// index h * W + w in int, then times the channel count in uint32), so a
+// plane's elements H * W * channels must stay below 2^32.
 static int conv_shape_ok(int N, int D, int H, int W, int CI, int CO) {
   if (N < 1 || D < 1 || H < 1 || W < 1 || H > 65535 || (int64_t)N * D > 65535 ||
       (int64_t)H * W * (CI > CO ? CI : CO) >= ((int64_t)1 << 32)) {

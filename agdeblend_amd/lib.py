@@ -204,7 +204,7 @@ class Problem:
         self.close()
 
     def window(self, w):
-        info = np.zeros(24, C_INT)
+        info = np.zeros(25, C_INT)
         if self.L.agdx_window_info(self.h, w, _p(info)):
             raise IndexError(w)
         nsp = int(info[1])
@@ -214,7 +214,7 @@ class Problem:
                     x_off=int(info[10]) + (int(info[11]) << 31),
                     P=[int(v) for v in info[12:12 + nsp]],
                     W=[int(v) for v in info[16:16 + nsp]],
-                    coords=[int(v) for v in info[20:20 + nsp]])
+                    coords=[int(v) for v in info[20:20 + nsp]], wM=int(info[24]))
 
     def window_taper(self, w):
         """(left, right) taper flags per space dim of window w."""

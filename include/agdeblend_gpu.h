@@ -60,8 +60,9 @@ int agdx_unique_channels(AgdProblem const *p, int *out);
 long agdx_model_len(AgdProblem const *p);  /* complex elements */
 long agdx_record_len(AgdProblem const *p); /* floats */
 int agdx_n_windows(AgdProblem const *p);
-/* info[24]: patch, nsp, ntr, ntw, nfk, nconv, Pt, Ft, Wt, S, x_off(lo),
- * x_off(hi), P[4], W[4], coords[4] */
+/* info[25]: patch, nsp, ntr, ntw, nfk, nconv, Pt, Ft, Wt, S, x_off(lo),
+ * x_off(hi), P[4], W[4], coords[4], wavelet convolution length (0 without
+ * wavelets) */
 int agdx_window_info(AgdProblem const *p, int w, int *info);
 /* per window trace: channel, interval, interval_start, trace_start, n */
 int agdx_window_blend(AgdProblem const *p, int w, long long *out);

@@ -4,9 +4,10 @@ own inputs (regenerated with glibc rand(), see reference_cases.py).
 CPU: the oracle (float32, the reference's AGD_TYPE) meets the test's
 criterion ||out - true||^2 < ||true||^2 / 10 after 800 iterations on all five
 problems. GPU: agd_deblend of the HIP build meets the same criterion, and its
-output stays within 1e-3 relative L2 of the float32 oracle after the full
-800 iterations (ISTA amplifies float32 rounding over this many iterations;
-the tight 1e-4 bar at a few iterations is in test_gpu_parity).
+output stays within 1e-4 relative L2 of the float64 oracle after the full
+800 iterations (the float32 oracle itself ends 1e-7 to 4e-7 from the float64
+one on these problems, so ISTA does not amplify the rounding here; 1e-4 is
+the bar of the shorter runs in test_gpu_parity).
 """
 import numpy as np
 import pytest
@@ -54,8 +55,10 @@ def test_hip_meets_reference_criterion_and_oracle(i):
                       c["channels"], c["trace_types"], 1.0, R.N_ITS, 0, blended)
     ok, ratio = R.passes(out, c["truth"])
     assert ok, ratio
-    ref, _ = O.deblend(data=ref_blend, initial_factor=1.0, n_its=R.N_ITS, double=False,
+    ref, _ = O.deblend(data=ref_blend, initial_factor=1.0, n_its=R.N_ITS, double=True,
                        **_geo(c))
     num = sum(float(np.sum((np.asarray(a, np.float64) - b) ** 2)) for a, b in zip(out, ref))
     den = sum(float(np.sum(np.asarray(b, np.float64) ** 2)) for b in ref)
-    assert (num / den) ** 0.5 <= 1e-3, (num / den) ** 0.5
+    err = (num / den) ** 0.5
+    print("problem %d: %d iterations, relative L2 to the float64 oracle %.3e" % (i, R.N_ITS, err))
+    assert err <= 1e-4, err

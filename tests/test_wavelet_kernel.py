@@ -1,7 +1,7 @@
 """The wavelet convolution of the full vibroseis workload (2 fleets of
-8 x 128 sources x 4096 samples, a 4000-sample sweep: transform length 5600,
-served by the four-step kernel with the length fixed at compile time)
-against the FP64 oracle, and against the generic LDS kernel (forced with
+8 x 128 sources x 4096 samples, a 4000-sample sweep: transform length 6144,
+served by the kernel with that length fixed at compile time) against the
+FP64 oracle, and against the generic LDS kernel (forced with
 AGD_GENERIC_WAVELET). Bars: operators <= 2e-6 relative to the oracle, the
 FP64-accumulated dot test <= 1e-6, the two GPU kernels within 2e-6."""
 import numpy as np
@@ -43,7 +43,7 @@ def test_vibroseis_wavelet_kernel_matches_oracle_and_generic(monkeypatch):
     finally:
         pm.close()
     rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)  # noqa: E731
-    print("four-step: forward %.2e adjoint %.2e | generic: forward %.2e adjoint %.2e" %
+    print("compile-time length: forward %.2e adjoint %.2e | generic: forward %.2e adjoint %.2e" %
           (rel(f1, ref_f), rel(a1, ref_a), rel(f2, ref_f), rel(a2, ref_a)))
     assert rel(f1, ref_f) <= 2e-6 and rel(a1, ref_a) <= 2e-6
     assert rel(f1, f2) <= 2e-6 and rel(a1, a2) <= 2e-6
