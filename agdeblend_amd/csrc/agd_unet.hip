@@ -99,11 +99,13 @@ __device__ __forceinline__ void st_swz(uint16_t *base, int row, int half, uint4 
   *(uint2 *)(base + swz_q(row, 2 * half + 1)) = make_uint2(v.z, v.w);
 }
 // The K = 32 operands read 16 bytes (8 channels, one half of a row) per
-// lane: rows of 16 channels with their two halves swapped when (row >> 3) &
-// 1, so the 16 lanes of an operand (16 consecutive rows, one half) cover
-// the 64 banks once
+// lane. The LDS serves 128 bytes (32 banks) per cycle, i.e. 8 lanes of a
+// 16-byte read: 8 consecutive rows of 32 bytes, one half each, span 256
+// bytes, and rows r and r + 4 fall on the same banks; swapping the two
+// halves of a row when (row >> 2) & 1 sends them to different banks, so
+// every 8 lanes cover the 32 banks once
 __device__ __forceinline__ int swz_h(int row, int half) {
-  return row * 16 + ((half ^ ((row >> 3) & 1)) << 3);
+  return row * 16 + ((half ^ ((row >> 2) & 1)) << 3);
 }
 __device__ __forceinline__ u32x4 ld_h(uint16_t const *base, int row, int half) {
   return *(u32x4 const *)(base + swz_h(row, half));
