@@ -40,6 +40,9 @@ typedef struct {
   /* time blocks: (window, first trace) pairs */
   int const *tblocks;         /* 2 ints per block */
   int n_tblocks;
+  int const *tadj;            /* [n_tblocks]: per class, its blocks sorted by
+                                 their first trace (the adjoint time kernel's
+                                 launch order), or NULL */
   int tb;                     /* traces per time block */
   size_t lds_time;            /* bytes (generic kernel) */
   AgdClass const *tclass;     /* host array */

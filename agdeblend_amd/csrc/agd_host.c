@@ -1163,6 +1163,44 @@ static int spatial_tasks(AgdProblem *p, AgdClass *cls, int ncls, int const *nk_r
   return 0;
 }
 
+/* The adjoint time kernel's launch order: per time class, its blocks
+ * sorted by (patch, patch trace index of the block's first trace, window),
+ * so the blocks of overlapping windows that read the same traces' record
+ * samples are neighbours (the kernel gives each XCD a contiguous run). */
+typedef struct {
+  int64_t key;
+  int w, id;
+} TbKey;
+static int cmp_tbkey(void const *a, void const *b) {
+  TbKey const *x = (TbKey const *)a, *y = (TbKey const *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  if (x->w != y->w) return x->w < y->w ? -1 : 1;
+  return (x->id > y->id) - (x->id < y->id);
+}
+static int *adj_time_order(AgdProblem const *p, int const *tb, int n) {
+  int *ord = (int *)malloc(((size_t)n + 1) * sizeof(int));
+  TbKey *k = (TbKey *)malloc(((size_t)n + 1) * sizeof(TbKey));
+  int c, j;
+  if (ord == NULL || k == NULL) {
+    free(ord);
+    free(k);
+    return NULL;
+  }
+  for (c = 0; c < p->n_tclass; c++) {
+    AgdClass const *cl = &p->tclass[c];
+    for (j = 0; j < cl->count; j++) {
+      int const id = cl->begin + j, w = tb[2 * id], t0 = tb[2 * id + 1];
+      k[j].key = ((int64_t)p->win[w].patch << 32) | (uint32_t)p->wpatch_trace[p->win[w].trace0 + t0];
+      k[j].w = w;
+      k[j].id = id;
+    }
+    qsort(k, (size_t)cl->count, sizeof(TbKey), cmp_tbkey);
+    for (j = 0; j < cl->count; j++) ord[cl->begin + j] = k[j].id;
+  }
+  free(k);
+  return ord;
+}
+
 static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp_out,
                          size_t *lds_t_out) {
   int w, c;
@@ -1587,6 +1625,11 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     }
     p->dev.tblocks = (int const *)dev_upload(p, tblocks.p, tblocks.n * 2 * sizeof(int));
     p->dev.n_tblocks = (int)tblocks.n;
+    {
+      int *ord = adj_time_order(p, (int const *)tblocks.p, (int)tblocks.n);
+      p->dev.tadj = ord ? (int const *)dev_upload(p, ord, tblocks.n * sizeof(int)) : NULL;
+      free(ord);
+    }
     p->dev.lds_time = lds_t;
     p->dev.rec_len = p->cfg.rec_len;
     p->dev.use_wavelets = p->use_wavelets;
@@ -1635,7 +1678,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     vec_free(&tblocks);
     if (!p->dev.win || !p->dev.plans || !p->dev.roots || !p->dev.tapers ||
         !p->dev.tscale || !p->dev.tconst || !p->dev.went || !p->dev.trace_win ||
-        !p->dev.tasks || !p->dev.tblocks || !p->dev.wsrc_off || !p->dev.X ||
+        !p->dev.tasks || !p->dev.tblocks || !p->dev.tadj || !p->dev.wsrc_off || !p->dev.X ||
         !p->dev.U || !p->dev.V || !p->dev.T || !p->dev.T2 || !p->dev.d || !p->dev.r ||
         !p->dev.scalar || !p->dev.lamb_sched || !p->dev.lamb_idx || !p->lamb_cur ||
         !p->dev.partials || !p->dev.tile_ptr || !p->dev.tile_ent ||

@@ -115,6 +115,14 @@ __device__ __forceinline__ float2 rootf(float2 const *roots, int n, int e, int s
   return w;
 }
 
+// Blocks go to the eight XCDs in turn (block b runs on XCD b mod 8, each
+// XCD with its own L2). This bijection of [0, n) gives XCD x a contiguous
+// run of the n work items instead.
+__device__ __forceinline__ int xcd_chunk(int b, int n) {
+  int const q = n >> 3, r = n & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
 // ---------------------------------------------------------------------------
 // Line geometries for LDS-resident batched FFTs
 // ---------------------------------------------------------------------------
@@ -1000,7 +1008,7 @@ __global__ __launch_bounds__(NT) void k_wavelet_t(
     AgdEntry const *__restrict__ went, float const *__restrict__ rec, float *__restrict__ T,
     float *__restrict__ T2) {
   constexpr int NP = sizeof...(Rs);
-  constexpr int MH = (Rs * ...), MR = 2 * MH, MP = MH + MH / 32;
+  constexpr int MH = (Rs * ...), MP = MH + MH / 32;
   constexpr int R1 = wav_radix<0, Rs...>(), RL = wav_radix<NP - 1, Rs...>();
   constexpr int NB1 = MH / R1, NBL = MH / RL;  // NBL: the last pass's Ns
   static_assert(NB1 % 32 == 0 && NBL % 32 == 0, "pass lengths must be multiples of 32");
@@ -1037,7 +1045,7 @@ __global__ __launch_bounds__(NT) void k_wavelet_t(
     return;
   }
   float2 const *const rt = roots + plans[w.wplan].roots_off;          // e^{-2 pi i m / MH}
-  float2 const *const rfull = roots + plans[w.wplan_full].roots_off;  // e^{-2 pi i m / MR}
+  float2 const *const rfull = roots + plans[w.wplan_full].roots_off;  // e^{-2 pi i m / (2 MH)}
   float2 const *const hs = wspec + so;
   // the twiddles of both transforms are loaded here, before the first
   // barrier (a barrier does not wait for global loads, so their latency
@@ -1098,7 +1106,7 @@ __global__ __launch_bounds__(NT) void k_wavelet_t(
   __syncthreads();
   wav_passes<-1, NT, PP, MH, R1, 1, NP, Rs...>(sa, sa + (PP ? MP : 0), tw, tid);
   // spectrum: real unpack, product with the wavelet, pack for the inverse,
-  // in place on the pairs (k, MH - k) of sf; wk = e^{-2 pi i k / MR}
+  // in place on the pairs (k, MH - k) of sf; wk = e^{-2 pi i k / (2 MH)}
   auto unpack = [&](float2 zk, float2 zo, float2 wk, float2 h, int k) {
     float2 const zc = make_float2(zo.x, -zo.y);
     float2 const ev = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y + zc.y));
@@ -1120,7 +1128,7 @@ __global__ __launch_bounds__(NT) void k_wavelet_t(
   };
   auto pair = [&](int k, float2 hk, float2 hc, float2 wk, bool write) {
     int const kc = MH - k;
-    float2 const wc = make_float2(-wk.x, wk.y);  // e^{-2 pi i kc / MR}
+    float2 const wc = make_float2(-wk.x, wk.y);  // e^{-2 pi i kc / (2 MH)}
     float2 const za = sf[wpad(k)];
     float2 const zb = sf[wpad(kc % MH)];
     float2 const xk = unpack(za, zb, wk, hk, k);
@@ -1659,7 +1667,8 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
     AgdFftPlan const *__restrict__ plans, float2 const *__restrict__ roots,
     float const *__restrict__ tapers, float const *__restrict__ tscale,
     AgdEntry const *__restrict__ went, float const *__restrict__ rec,
-    float const *__restrict__ Tin, float2 *__restrict__ V, int plain) {
+    float const *__restrict__ Tin, float2 *__restrict__ V, int plain,
+    int const *__restrict__ order) {
   constexpr int M = M1 * M2;
   constexpr int NT = time_threads<M>();
   constexpr int Ft = M + 1;
@@ -1680,7 +1689,10 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   int64_t *erec = (int64_t *)(RP + Ft);
   int *ets = (int *)(erec + TG);
   int *en = ets + TG;
-  int const bid = block0 + blockIdx.x;
+  // with an order: the class's blocks sorted by first trace, each XCD given
+  // a contiguous run of them, so the (up to four) windows' blocks that read
+  // the same traces' record samples meet in one L2
+  int const bid = order ? order[block0 + xcd_chunk(blockIdx.x, gridDim.x)] : block0 + blockIdx.x;
   AgdWin const &w = wins[blocks[2 * bid]];
   int const t0 = blocks[2 * bid + 1];
   int const nconv = w.nconv, ntw = w.ntw;
@@ -2520,7 +2532,8 @@ int agdg_adj_time(AgdDev const *g, float const *Tin, int plain) {
           launch("adj_time", k_adj_time_t<a, b>, dim3(n),                                 \
                  dim3(time_threads<a * b>()), adj_time_lds<a * b>(), g->win, g->tblocks,   \
                  first, g->plans, (float2 const *)g->roots, g->tapers, g->tscale,        \
-                 g->went, (float const *)g->r, Tin, (float2 *)g->V, plain))             \
+                 g->went, (float const *)g->r, Tin, (float2 *)g->V, plain,              \
+                 first == c.begin && n == c.count ? g->tadj : (int const *)nullptr))    \
         return 1;                                                                       \
       done = true;                                                                      \
     }
