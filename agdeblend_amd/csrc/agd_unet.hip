@@ -495,6 +495,217 @@ __global__ __launch_bounds__(UT) void k_conv3(
   }
 }
 
+// ---------------------------------------------------------------------------
+// 16 -> 16 channels, streaming over the input planes: each input plane is
+// staged into LDS once (double-buffered) and each of its A fragments is read
+// once and used for the three depth taps, i.e. for the three output planes
+// the plane touches (kd = 0, 1, 2 into outputs p + 1, p, p - 1), with the
+// weights of all 27 taps in registers. K = 32 pairs two (kh, kw) positions
+// of one input plane: step s takes positions 2 s and 2 s + 1 (the tenth
+// position, past the ninth, reads the ninth with zero weights). Per 16
+// voxels and input plane this reads 5 A fragments from LDS and no B, where
+// k_conv3 reads 14 A and 14 B fragments per 16 voxels and output plane.
+// Block: CT_H rows x CT_W voxels (one row per wave) of DB output planes.
+// Epilogue per output plane: bias and ReLU (mode bits 1, 2), the wave's
+// rows through its own LDS rows as 16-byte coalesced stores; with mode bit 8
+// the 2x2x2 max pooling (k_maxpool's scan order and strict >) is formed one
+// plane later from the four waves' rows (D, H, W even, DB even).
+// grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / DB)); block 256.
+// ---------------------------------------------------------------------------
+template <int DB>
+__global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
+                                               uint16_t const *__restrict__ wt,
+                                               float const *__restrict__ bias,
+                                               uint16_t *__restrict__ y,
+                                               uint16_t *__restrict__ pool, int D, int H, int W,
+                                               int mode) {
+  static_assert(DB % 2 == 0, "pooling pairs the planes of a block");
+  constexpr int PW = CT_W + 2, PH = CT_H + 2;
+  constexpr int PLANE = PH * PW * 16;  // bf16 elements of one staged plane
+  constexpr int NPI = PH * PW * 2;     // staging items (8 channels) per plane
+  constexpr int NPK = (NPI + UT - 1) / UT;
+  constexpr int OBS = 16 + 4;               // output row: 16 channels + 4 (bank spread)
+  constexpr int ROWS = CT_H * 64 * OBS;     // one output plane's rows
+  static_assert(CT_H == 4 && UT == 256, "pooling: one thread per (pooled row, column, 4 channels)");
+  __shared__ __align__(16) uint16_t sx[2 * PLANE];
+  __shared__ __align__(16) uint16_t so[2 * ROWS + 2 * 32 * 16];
+  uint16_t *const pm = so + 2 * ROWS;  // pooling partials of the even plane [2][32][16]
+  int const tid = threadIdx.x;
+  int const lane = tid & 63;
+  int const wave = tid >> 6;
+  int const ndb = (D + DB - 1) / DB;
+  int const n = blockIdx.z / ndb;
+  int const d0 = (blockIdx.z - n * ndb) * DB;
+  int const h0 = blockIdx.y * CT_H;
+  int const w0 = blockIdx.x * CT_W;
+  int const i = lane & 15;   // A row: voxel; B / D column: output channel
+  int const kq = lane >> 4;  // channels 8 (kq & 1) .. + 7 of position 2 s + (kq >> 1)
+  uint16_t const *const xn = x + (int64_t)n * D * H * W * 16;
+  int64_t const plane = (int64_t)H * W * 16;
+  // the weights of every tap: bw[kd][s] = w[kd 9 + 2 s + (kq >> 1)][i][8 (kq & 1) ..]
+  u32x4 bw[3][5];
+#pragma unroll
+  for (int kd = 0; kd < 3; ++kd)
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+      int const q = 2 * s + (kq >> 1);
+      u32x4 const v = *(u32x4 const *)(wt + ((int64_t)(kd * 9 + (q < 9 ? q : 8)) * 16 + i) * 16 +
+                                       8 * (kq & 1));
+      unsigned const keep = q < 9 ? ~0u : 0u;
+      bw[kd][s] = v & keep;
+    }
+  // staging items: 32-bit offset within a plane and whether (h, w) lies
+  // inside the volume; items past the last repeat it (no branch)
+  uint32_t ioff[NPK];
+  bool iok[NPK];
+#pragma unroll
+  for (int k = 0; k < NPK; ++k) {
+    int const e0 = tid + k * UT;
+    int const e = e0 < NPI ? e0 : NPI - 1;
+    int const half = e & 1, r = e >> 1;
+    int const pw = r % PW, ph = r / PW;
+    int const hh = h0 + ph - 1, ww = w0 + pw - 1;
+    iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
+    ioff[k] = iok[k] ? (uint32_t)(hh * W + ww) * 16u + (uint32_t)(half * 8) : (uint32_t)(half * 8);
+  }
+  auto load_plane = [&](int dd, uint4 (&v)[NPK]) {
+    bool const okd = dd >= 0 && dd < D;
+    uint16_t const *const src = xn + (okd ? dd : 0) * plane;
+#pragma unroll
+    for (int k = 0; k < NPK; ++k) {
+      uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;
+      uint4 const t = *(uint4 const *)(src + ioff[k]);
+      v[k] = make_uint4(t.x & msk, t.y & msk, t.z & msk, t.w & msk);
+    }
+  };
+  auto store_plane = [&](uint16_t *dst, uint4 const (&v)[NPK]) {
+#pragma unroll
+    for (int k = 0; k < NPK; ++k) {
+      int const e0 = tid + k * UT;
+      int const e = e0 < NPI ? e0 : NPI - 1;
+      st_h(dst, e >> 1, e & 1, v[k]);  // past the end: the last item again
+    }
+  };
+  f32x4 acc[DB][4];
+#pragma unroll
+  for (int j = 0; j < DB; ++j)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[j][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  int const h = h0 + wave;
+  // output plane oi (d0 + oi): bias, ReLU, into the wave's rows of buffer
+  // oi & 1, then stored by the wave itself
+  auto epilogue = [&](int oi) {
+    int const d = d0 + oi;
+    uint16_t *const ob = so + (oi & 1) * ROWS + wave * 64 * OBS;
+    float const b = (mode & 1) ? bias[i] : 0.f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v = acc[oi][m][r] + b;
+        if (mode & 2) v = fmaxf(v, 0.f);
+        ob[(16 * m + 4 * kq + r) * OBS + i] = f2bf(v);
+      }
+    __builtin_amdgcn_wave_barrier();
+    if (h < H && d < D) {
+      int64_t const orow = (((int64_t)n * D + d) * H + h) * W;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        int const e = lane + 64 * k;  // (voxel, 8-channel half)
+        int const v = e >> 1, c8 = e & 1;
+        if (w0 + v < W) {
+          uint2 const lo = *(uint2 const *)(ob + v * OBS + c8 * 8);
+          uint2 const hi = *(uint2 const *)(ob + v * OBS + c8 * 8 + 4);
+          *(uint4 *)(y + (orow + w0 + v) * 16 + c8 * 8) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
+      }
+    }
+  };
+  // 2x2x2 max pooling of output plane oi from all four waves' rows (written
+  // before the last barrier): item (pooled row pr, column pc, 4 channels),
+  // one per thread, scans its voxels in (d, h, w) order with a strict >; the
+  // even plane's partial (one of the bf16 values, so exact) waits in pm for
+  // the odd one, handled by the same thread
+  auto pool_plane = [&](int oi) {
+    int const d = d0 + oi;
+    uint16_t const *const rows = so + (oi & 1) * ROWS;
+    int const c4 = tid & 3, pc = (tid >> 2) & 31, pr = tid >> 7;
+    uint16_t *const pp = pm + (pr * 32 + pc) * 16 + c4 * 4;
+    float best[4];
+    if (oi % 2 == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) best[k] = -INFINITY;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) best[k] = bf2f(pp[k]);
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      uint16_t const *const r =
+          rows + (2 * pr + (o >> 1)) * 64 * OBS + (2 * pc + (o & 1)) * OBS + c4 * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float const f = bf2f(r[k]);
+        if (f > best[k]) best[k] = f;
+      }
+    }
+    if (oi % 2 == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pp[k] = f2bf(best[k]);
+    } else {
+      int const dp = d / 2, hp = h0 / 2 + pr, wp = w0 / 2 + pc;
+      if (d < D && 2 * hp < H && 2 * wp < W) {
+        uint32_t const q0 = (uint32_t)f2bf(best[0]) | ((uint32_t)f2bf(best[1]) << 16);
+        uint32_t const q1 = (uint32_t)f2bf(best[2]) | ((uint32_t)f2bf(best[3]) << 16);
+        int64_t const po =
+            ((((int64_t)n * (D / 2) + dp) * (H / 2) + hp) * (W / 2) + wp) * 16 + c4 * 4;
+        *(uint2 *)(pool + po) = make_uint2(q0, q1);
+      }
+    }
+  };
+  uint4 nxt[NPK];
+  {
+    uint4 v0[NPK];
+    load_plane(d0 - 1, v0);
+    load_plane(d0, nxt);
+    store_plane(sx, v0);
+  }
+  __syncthreads();
+  // input plane p = d0 - 1 + j is in buffer j & 1; nxt holds plane p + 1,
+  // stored into the other buffer (free: every wave passed the barrier after
+  // reading it) before plane p + 2 is loaded into nxt
+#pragma unroll
+  for (int j = 0; j < DB + 2; ++j) {
+    uint16_t const *const cur = sx + (j & 1) * PLANE;
+    if (j + 1 < DB + 2) {
+      store_plane(sx + ((j + 1) & 1) * PLANE, nxt);
+      if (j + 2 < DB + 2) load_plane(d0 + 1 + j, nxt);
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+      int const q = 2 * s + (kq >> 1);
+      int const qq = q < 9 ? q : 8;
+      int const kh = qq / 3, kw = qq - 3 * (qq / 3);
+      int const v0 = (wave + kh) * PW + i + kw;
+      u32x4 a[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a[m] = ld_h(cur, v0 + 16 * m, kq & 1);
+#pragma unroll
+      for (int kd = 0; kd < 3; ++kd) {
+        int const oi = j - kd;  // output plane d0 + oi = p + 1 - kd
+        if (oi >= 0 && oi < DB) {
+#pragma unroll
+          for (int m = 0; m < 4; ++m) acc[oi][m] = mfma32(a[m], bw[kd][s], acc[oi][m]);
+        }
+      }
+    }
+    if (j >= 2) epilogue(j - 2);
+    if ((mode & 8) && j >= 3) pool_plane(j - 3);
+    __syncthreads();
+  }
+  if (mode & 8) pool_plane(DB - 1);
+}
+
 // Combined taps of the first CU input channels of wt ([27][CO][CI]):
 // wcu[up_tap(pd, ph, a, b, kw)][co][cu] = bf16(sum over the d taps kd of
 // (pd, a) and the h taps kh of (ph, b) of wt[(kd * 3 + kh) * 3 + kw][co][cu]),
@@ -1241,6 +1452,15 @@ static int conv_shape_ok(int N, int D, int H, int W, int CI, int CO) {
                    (uint16_t *)pool, D, H, W, mode);                                        \
   }
 
+#define CONVS_DB 16
+static int conv3s_launch(void const *x, void const *wt, float const *bias, void *y, void *pool,
+                         int N, int D, int H, int W, int mode, void *stream) {
+  dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H),
+                  (unsigned)(N * ((D + CONVS_DB - 1) / CONVS_DB)));
+  return ulaunch("conv3s", k_conv3s<CONVS_DB>, grid, dim3(UT), stream, (uint16_t const *)x,
+                 (uint16_t const *)wt, bias, (uint16_t *)y, (uint16_t *)pool, D, H, W, mode);
+}
+
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
                int N, int D, int H, int W, int CI, int CO, int mode, void *stream) {
   void const *const xu = NULL;
@@ -1248,6 +1468,9 @@ int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gat
   void *const pool = NULL;
   mode &= 7;
   if (!conv_shape_ok(N, D, H, W, CI, CO)) return 1;
+  // 16 -> 16 without the gradient's gate: the streaming form
+  if (CI == 16 && CO == 16 && !(mode & 4))
+    return conv3s_launch(x, wt, bias, y, NULL, N, D, H, W, mode, stream);
 #define CC(a, b) \
   if (CI == a && CO == b) CONV_LAUNCH(a, b, 0, false)
   CONV_CASES(CC)
@@ -1273,6 +1496,7 @@ int agdu_conv3_pool(void const *x, void const *wt, float const *bias, void *y, v
     return 1;
   }
   mode = (mode & 3) | 8;
+  if (CI == 16 && CO == 16) return conv3s_launch(x, wt, bias, y, pool, N, D, H, W, mode, stream);
 #define CP(a, b) \
   if (CI == a && CO == b && ConvDepth<b, 0>::DB % 2 == 0) CONV_LAUNCH(a, b, 0, false)
   CONV_CASES(CP)

@@ -72,6 +72,42 @@ def test_conv3_forward(ci, co):
     _close(_np(y), R.conv3(x, w, b.astype(np.float64), relu=True))
 
 
+def test_conv3_streaming_16_channels_across_depth_blocks():
+    """The 16 -> 16 convolution streams over the input planes in blocks of 8
+    output planes: several depth blocks, a partial last one, partial row and
+    column tiles, against the FP64 restatement; its pooling form against the
+    pooling kernel over several depth blocks."""
+    rng = np.random.default_rng(1616)
+    L = _lib()
+    shape = (1, 19, 5, 130, 16)
+    x = R.bf16_round(rng.standard_normal(shape))
+    w = R.bf16_round(rng.standard_normal((27, 16, 16)) / np.sqrt(27 * 16))
+    b = rng.standard_normal(16).astype(np.float32)
+    y = torch.empty(shape, dtype=torch.bfloat16, device="cuda")
+    bt = torch.tensor(b, device="cuda")
+    assert L.agdu_conv3(_p(_bf(x)), _p(_bf(w)), _p(bt), None, _p(y), 1, 19, 5, 130, 16, 16, 3,
+                        None) == 0
+    torch.cuda.synchronize()
+    _close(_np(y), R.conv3(x, w, b.astype(np.float64), relu=True))
+    N, D, H, W = 1, 18, 6, 68
+    xp = _bf(rng.standard_normal((N, D, H, W, 16)))
+    y1 = torch.empty((N, D, H, W, 16), dtype=torch.bfloat16, device="cuda")
+    y2 = torch.empty_like(y1)
+    p1 = torch.empty((N, D // 2, H // 2, W // 2, 16), dtype=torch.bfloat16, device="cuda")
+    p2 = torch.empty_like(p1)
+    arg = torch.empty((N, D // 2, H // 2, W // 2, 16), dtype=torch.uint8, device="cuda")
+    wt = _bf(w)
+    assert L.agdu_conv3(_p(xp), _p(wt), _p(bt), None, _p(y1), N, D, H, W, 16, 16, 3, None) == 0
+    assert L.agdu_maxpool(_p(y1), _p(p1), _p(arg), N, D, H, W, 16, None) == 0
+    assert L.agdu_conv3_pool(_p(xp), _p(wt), _p(bt), _p(y2), _p(p2), N, D, H, W, 16, 16, 3,
+                             None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    assert torch.equal(p1.view(torch.int16), p2.view(torch.int16))
+    ref = R.conv3(_np(xp).astype(np.float64), w, b.astype(np.float64), relu=True)
+    _close(_np(y1), ref)
+
+
 @pytest.mark.parametrize("ci,co", [(16, 32), (32, 64), (96, 32)])
 def test_conv3_input_gradient_with_relu_mask(ci, co):
     rng = np.random.default_rng(ci + 3 * co)
