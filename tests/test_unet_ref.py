@@ -115,3 +115,20 @@ def test_combined_taps_equal_the_upsampled_convolution():
     assert np.array_equal(wr, R.bf16_round(wr))
     assert np.abs(wr - R.up_weights(wb, cu, rounded=False)).max() <= 2 ** -8 * np.abs(wr).max()
 
+
+
+def test_convolution_shape_limits_are_refused_on_the_host():
+    """The kernels index a d plane in 32-bit unsigned arithmetic (H * W *
+    channels < 2^32) and launch one grid row per (n, d block): larger shapes
+    are refused before any device work (no GPU needed for the refusal)."""
+    from agdeblend_amd import lib as L
+    lib = L.load()
+    # H * W * 16 = 2^36: over the plane limit
+    assert lib.agdu_conv3(None, None, None, None, None, 1, 1, 65535, 65536, 16, 16, 3, None) == 1
+    # H over 65535 grid rows
+    assert lib.agdu_conv3(None, None, None, None, None, 1, 1, 65536, 64, 16, 16, 3, None) == 1
+    # N * D over 65535
+    assert lib.agdu_conv3(None, None, None, None, None, 2, 40000, 8, 64, 16, 16, 3, None) == 1
+    assert lib.agdu_conv_in(None, None, 16, None, None, 1, 1, 65535, 65536, 1.0, None) == 1
+    assert lib.agdu_conv_out(None, None, 16, None, None, None, 1, 1, 65535, 65536, 1.0,
+                             None) == 1
