@@ -288,40 +288,40 @@ __global__ __launch_bounds__(UT) void k_conv3(
       }
       __syncthreads();
       int const ph = (h0 + wave) & 1;
+      // Streamed over the staged coarse planes: fine plane j (parity j & 1,
+      // d0 even) reads coarse plane (j + 1) / 2 + a with its d-combination
+      // a, so coarse plane cp feeds fine planes 2 (cp - a) - 1 and
+      // 2 (cp - a) for a = 0, 1. Each A fragment (coarse plane cp, h
+      // combination b, tap kw) is read once and used for those (up to
+      // four) fine planes; K = 32 pairs two of the six (b, kw) positions.
+      // Per fine plane the contributions keep their order (a = 0, then
+      // a = 1; positions in order), so the sums are those of one plane at
+      // a time.
 #pragma unroll
-      for (int j = 0; j < DB; ++j) {
-        int const pd = (d0 + j) & 1;
-        // combined tap u = (a 2 + b) 3 + kw (12 of them, 6 steps of two):
-        // rows (a, b) are fine plane d - 1 + pd + a and fine row h - 1 + ph +
-        // b, i.e. coarse plane (j + 1) / 2 + a and row (wave + 1) / 2 + b of
-        // the staging; fine column w0 + x (x = 16 m + i + kw - 1 >= -1) is
-        // coarse column floor(x / 2) + 1
-        auto load_up = [&](int st, Ops &o) {
-          int const u = 2 * st + (kq >> 1);
-          int const ab = u / 3, kw = u - 3 * ab;
-          int const a = ab >> 1, b = ab & 1;
-          int const v0 = (((j + 1) / 2 + a) * CPH + (wave + 1) / 2 + b) * CPW + 1;
-#pragma unroll
-          for (int m = 0; m < 4; ++m)
-            o.a[m] = ld_h(sx, v0 + ((16 * m + i + kw - 1) >> 1), kq & 1);
-#pragma unroll
-          for (int t = 0; t < NT; ++t)
-            o.b[t] = ld_h(sw, up_tap(pd, ph, a, b, kw) * CO + t * 16 + i, kq & 1);
-        };
-        auto mma_up = [&](Ops const &o) {
-#pragma unroll
-          for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int m = 0; m < 4; ++m) acc[j][m][t] = mfma32(o.a[m], o.b[t], acc[j][m][t]);
-        };
-        Ops opa, opb;
-        load_up(0, opa);
+      for (int cp = 0; cp < CPD; ++cp) {
 #pragma unroll 1
-        for (int st = 1; st < 6; st += 2) {
-          load_up(st, opb);
-          mma_up(opa);
-          if (st + 1 < 6) load_up(st + 1, opa);
-          mma_up(opb);
+        for (int sp = 0; sp < 3; ++sp) {
+          int const u = 2 * sp + (kq >> 1);  // (b, kw) position of this lane
+          int const b = u / 3, kw = u - 3 * b;
+          int const v0 = (cp * CPH + (wave + 1) / 2 + b) * CPW + 1;
+          u32x4 a4[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) a4[m] = ld_h(sx, v0 + ((16 * m + i + kw - 1) >> 1), kq & 1);
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int jp = 0; jp < 2; ++jp) {
+              int const j = 2 * (cp - a) - 1 + jp;
+              if (j >= 0 && j < DB) {
+                int const pd = j & 1;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                  u32x4 const bt = ld_h(sw, up_tap(pd, ph, a, b, kw) * CO + t * 16 + i, kq & 1);
+#pragma unroll
+                  for (int m = 0; m < 4; ++m) acc[j][m][t] = mfma32(a4[m], bt, acc[j][m][t]);
+                }
+              }
+            }
         }
       }
     }
