@@ -348,6 +348,51 @@ __global__ __launch_bounds__(UT) void k_conv3(
       store_plane(d0 + 1, v2);
     }
     __syncthreads();
+    if constexpr (UPC) {
+      // The x channels of d1a streamed over the input planes p = d0 - 1 ..
+      // d0 + DB (in ring slot (p + 3) % 3): each A fragment (plane p, two
+      // (kh, kw) positions of step s) is read once and used for the output
+      // planes p + 1 - kd it feeds, with the weights of tap kd 9 + position
+      // from LDS; plane p + 3 is loaded while plane p is computed and stored
+      // into its slot.
+#pragma unroll
+      for (int jp = 0; jp < DB + 2; ++jp) {
+        int const p = d0 - 1 + jp;
+        uint16_t const *const pl = sx + ((p + 3) % 3) * PLANE;
+        uint4 nxt[NPK];
+        bool const more = jp + 3 < DB + 2;  // plane p + 3 still an input
+        if (more) load_plane(p + 3, c, nxt);
+#pragma unroll 1
+        for (int st = 0; st < 5; ++st) {
+          int const q = 2 * st + (kq >> 1);
+          int const qq = q < 9 ? q : 8;
+          int const kh = qq / 3, kw = qq - 3 * kh;
+          unsigned const keep = q < 9 ? ~0u : 0u;
+          int const v0 = (wave + kh) * PW + i + kw;
+          u32x4 a4[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) a4[m] = ld_h(pl, v0 + 16 * m, kq & 1);
+#pragma unroll
+          for (int kd = 0; kd < 3; ++kd) {
+            int const oi = jp - kd;  // output plane d0 + oi = p + 1 - kd
+            if (oi >= 0 && oi < DB) {
+#pragma unroll
+              for (int tt = 0; tt < NT; ++tt) {
+                u32x4 const bt = ld_h(sw, (kd * 9 + qq) * CO + tt * 16 + i, kq & 1) & keep;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) acc[oi][m][tt] = mfma32(a4[m], bt, acc[oi][m][tt]);
+              }
+            }
+          }
+        }
+        if (more) {
+          __syncthreads();  // every wave is done with plane p
+          store_plane(p + 3, nxt);
+          __syncthreads();
+        }
+      }
+      continue;
+    }
 #pragma unroll
     for (int j = 0; j < DB; ++j) {
       int const d = d0 + j;
