@@ -348,7 +348,7 @@ __global__ __launch_bounds__(UT) void k_conv3(
       store_plane(d0 + 1, v2);
     }
     __syncthreads();
-    if constexpr (UPC) {
+    if constexpr (UPC && DB >= 4) {
       // The x channels of d1a streamed over the input planes p = d0 - 1 ..
       // d0 + DB (in ring slot (p + 3) % 3): each A fragment (plane p, two
       // (kh, kw) positions of step s) is read once and used for the output
