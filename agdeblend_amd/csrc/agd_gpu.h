@@ -13,6 +13,9 @@
 
 #include "agd_layout.h"
 
+/* ints per time block: window, first trace, first and end time window */
+#define TB_INTS 4
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -37,8 +40,8 @@ typedef struct {
   size_t lds_spatial;         /* bytes (generic kernel) */
   AgdClass const *sclass;     /* host array */
   int n_sclass;
-  /* time blocks: (window, first trace) pairs */
-  int const *tblocks;         /* 2 ints per block */
+  /* time blocks: (window, first trace, time windows [tw0, tw1)) */
+  int const *tblocks;         /* TB_INTS ints per block */
   int n_tblocks;
   int const *tadj;            /* [n_tblocks]: per class, its blocks sorted by
                                  their first trace (the adjoint time kernel's

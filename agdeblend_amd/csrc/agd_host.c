@@ -684,6 +684,8 @@ struct AgdProblem {
   int *wtrace0;  /* per window: first global window trace; n_wtr at n_win */
   int *chunk;    /* window chunk boundaries [n_chunk + 1] */
   int n_chunk;
+  int *tblk;     /* host copy of the time blocks [n_tblk][TB_INTS] */
+  int n_tblk;
   float *lamb_cur;  /* device threshold read by captured iterations */
   int sched_cap;    /* capacity of dev.lamb_sched (floats) */
   int u_valid;   /* U holds the forward spatial transform of X */
@@ -743,6 +745,7 @@ EXPORT void agdx_destroy(AgdProblem *p) {
   free(p->tclass);
   free(p->wtrace0);
   free(p->chunk);
+  free(p->tblk);
   blendcfg_free(&p->cfg);
   free(p);
 }
@@ -1164,16 +1167,18 @@ static int spatial_tasks(AgdProblem *p, AgdClass *cls, int ncls, int const *nk_r
 }
 
 /* The adjoint time kernel's launch order: per time class, its blocks
- * sorted by (patch, patch trace index of the block's first trace, window),
- * so the blocks of overlapping windows that read the same traces' record
- * samples are neighbours (the kernel gives each XCD a contiguous run). */
+ * sorted by (patch, patch trace index of the block's first trace, first
+ * time window, window), so the blocks of overlapping windows that read the
+ * same traces' record samples are neighbours (the kernel gives each XCD a
+ * contiguous run). */
 typedef struct {
   int64_t key;
-  int w, id;
+  int tw, w, id;
 } TbKey;
 static int cmp_tbkey(void const *a, void const *b) {
   TbKey const *x = (TbKey const *)a, *y = (TbKey const *)b;
   if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  if (x->tw != y->tw) return x->tw < y->tw ? -1 : 1;
   if (x->w != y->w) return x->w < y->w ? -1 : 1;
   return (x->id > y->id) - (x->id < y->id);
 }
@@ -1189,8 +1194,9 @@ static int *adj_time_order(AgdProblem const *p, int const *tb, int n) {
   for (c = 0; c < p->n_tclass; c++) {
     AgdClass const *cl = &p->tclass[c];
     for (j = 0; j < cl->count; j++) {
-      int const id = cl->begin + j, w = tb[2 * id], t0 = tb[2 * id + 1];
+      int const id = cl->begin + j, w = tb[TB_INTS * id], t0 = tb[TB_INTS * id + 1];
       k[j].key = ((int64_t)p->win[w].patch << 32) | (uint32_t)p->wpatch_trace[p->win[w].trace0 + t0];
+      k[j].tw = tb[TB_INTS * id + 2];
       k[j].w = w;
       k[j].id = id;
     }
@@ -1201,9 +1207,22 @@ static int *adj_time_order(AgdProblem const *p, int const *tb, int n) {
   return ord;
 }
 
+/* time windows per block of the specialised time kernels: all of a
+ * window's (0) for the whole-volume order, where each block streams a
+ * trace tile's time windows one after the other; 4 with the window-chunk
+ * order (AGD_CHUNK_MB), so that a chunk of a few windows still makes
+ * thousands of blocks; AGD_TIME_TW overrides */
+static int time_tw_run(void) {
+  char const *e = getenv("AGD_TIME_TW");
+  if (e != NULL) return atoi(e) > 0 ? atoi(e) : 0;
+  e = getenv("AGD_CHUNK_MB");
+  return (e != NULL && atof(e) > 0.0) ? 4 : 0;
+}
+
 static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp_out,
                          size_t *lds_t_out) {
   int w, c;
+  int const tw_run = time_tw_run();
   size_t lds_sp = 0, lds_t = 0;
   vec sc, tc;
   vec_init(&sc, sizeof(AgdClass));
@@ -1275,15 +1294,23 @@ static int build_classes(AgdProblem *p, vec *tasks, vec *tblocks, size_t *lds_sp
     cl->begin = (int)tblocks->n;
     for (w = 0; w < p->n_win; w++) {
       AgdWin const *W = &p->win[w];
-      int a, b, t0;
+      int a, b, t0, q, parts;
       wpos[w] = (int)tblocks->n - cl->begin;
       time_key(W, &a, &b);
       if (a != cl->p0 || b != cl->p1) continue;
+      /* the specialised kernels' blocks split the window's time windows
+       * into nearly equal runs of at most tw_run (0: one run) */
+      parts = (a && tw_run > 0) ? (W->ntw + tw_run - 1) / tw_run : 1;
+      if (parts < 1) parts = 1;
       for (t0 = 0; t0 < W->ntr; t0 += tb) {
-        int *b2 = (int *)vec_push(tblocks);
-        if (b2 == NULL) return 1;
-        b2[0] = w;
-        b2[1] = t0;
+        for (q = 0; q < parts; q++) {
+          int *b4 = (int *)vec_push(tblocks);
+          if (b4 == NULL) return 1;
+          b4[0] = w;
+          b4[1] = t0;
+          b4[2] = (int)((int64_t)W->ntw * q / parts);
+          b4[3] = (int)((int64_t)W->ntw * (q + 1) / parts);
+        }
       }
     }
     cl->count = (int)tblocks->n - cl->begin;
@@ -1404,7 +1431,7 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
     p->wpatch_trace = (int *)malloc(((size_t)p->n_wtr + 1) * sizeof(int));
     vec_init(&tapers, sizeof(float));
     vec_init(&tasks, sizeof(AgdTask));
-    vec_init(&tblocks, 2 * sizeof(int));
+    vec_init(&tblocks, TB_INTS * sizeof(int));
     if (!tscale || !tconst || !wsrc_off || !trace_win || !went || !p->wcoord ||
         !p->wpatch_trace)
       goto err;
@@ -1623,7 +1650,11 @@ static AgdProblem *create_impl(int np, int const *volumes, int const *n_dims,
       }
       p->chunk[++p->n_chunk] = p->n_win;
     }
-    p->dev.tblocks = (int const *)dev_upload(p, tblocks.p, tblocks.n * 2 * sizeof(int));
+    p->dev.tblocks = (int const *)dev_upload(p, tblocks.p, tblocks.n * TB_INTS * sizeof(int));
+    p->tblk = (int *)malloc((tblocks.n + 1) * TB_INTS * sizeof(int));
+    if (p->tblk == NULL) goto err;
+    memcpy(p->tblk, tblocks.p, tblocks.n * TB_INTS * sizeof(int));
+    p->n_tblk = (int)tblocks.n;
     p->dev.n_tblocks = (int)tblocks.n;
     {
       int *ord = adj_time_order(p, (int const *)tblocks.p, (int)tblocks.n);
@@ -2644,6 +2675,13 @@ EXPORT int agdx_time_variants(AgdProblem const *p, int *out) {
   if (out)
     for (c = 0; c < p->n_tclass; c++) out[c] = p->tclass[c].var;
   return p->n_tclass;
+}
+
+/* the time blocks (window, first trace, time windows [tw0, tw1)), TB_INTS
+ * ints each, into out when given; returns their number */
+EXPORT int agdx_time_blocks(AgdProblem const *p, int *out) {
+  if (out) memcpy(out, p->tblk, (size_t)p->n_tblk * TB_INTS * sizeof(int));
+  return p->n_tblk;
 }
 
 EXPORT int agdx_time_factors(AgdProblem const *p, int *out) {

@@ -333,8 +333,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_time(
     float2 const *__restrict__ U, float *__restrict__ T) {
   extern __shared__ float2 sm[];
   int const bid = block0 + blockIdx.x;
-  AgdWin const w = wins[blocks[2 * bid]];
-  int const t0 = blocks[2 * bid + 1];
+  AgdWin const w = wins[blocks[TB_INTS * bid]];
+  int const t0 = blocks[TB_INTS * bid + 1];
   int const nt = min(tb, w.ntr - t0);
   int const Pt = w.Pt, Ft = w.Ft, Wt = w.Wt, half = w.half;
   bool const even = (Pt % 2 == 0) && Pt >= 4;
@@ -547,8 +547,8 @@ __global__ __launch_bounds__(NTHR) void k_adj_time(
     float const *__restrict__ Tin, float2 *__restrict__ V, int plain) {
   extern __shared__ float2 sm[];
   int const bid = block0 + blockIdx.x;
-  AgdWin const w = wins[blocks[2 * bid]];
-  int const t0 = blocks[2 * bid + 1];
+  AgdWin const w = wins[blocks[TB_INTS * bid]];
+  int const t0 = blocks[TB_INTS * bid + 1];
   int const nt = min(tb, w.ntr - t0);
   int const Pt = w.Pt, Ft = w.Ft, Wt = w.Wt, half = w.half;
   bool const even = (Pt % 2 == 0) && Pt >= 4;
@@ -1450,7 +1450,11 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
 // (forward) or a last pass (adjoint). The overlap with the neighbouring time
 // window stays in registers (each thread owns fixed samples of fixed
 // traces), and the next time window's input is loaded into registers while
-// the current one is transformed.
+// the current one is transformed. A block serves a run [tw0, tw1) of its
+// tile's time windows (all of them unless the host splits them, as it does
+// for the window-chunk order); a forward block with tw0 > 0 transforms time
+// window tw0 - 1 again for its carry, so every sample is the same sum as in
+// one run over all time windows.
 // ---------------------------------------------------------------------------
 template <int M>
 __host__ __device__ constexpr int time_threads() {
@@ -1512,8 +1516,12 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
   float *CA = (float *)(RP + Ft);          // [TG][half] overlap carry (LT: unused)
   float *SC = LT ? CA : CA + TG * half;    // [TG] per-trace scale
   int const bid = block0 + blockIdx.x;
-  AgdWin const &w = wins[blocks[2 * bid]];
-  int const t0 = blocks[2 * bid + 1];
+  AgdWin const &w = wins[blocks[TB_INTS * bid]];
+  int const t0 = blocks[TB_INTS * bid + 1];
+  // the block's time windows [tw0, tw1); one that starts inside the window
+  // first transforms time window tw0 - 1 again, for its overlap carry only
+  int const tw0 = blocks[TB_INTS * bid + 2], tw1 = blocks[TB_INTS * bid + 3];
+  int const twa = tw0 > 0 ? tw0 - 1 : 0;
   int const nconv = w.nconv, ntw = w.ntw;
   int const tid = threadIdx.x;
   {
@@ -1545,11 +1553,11 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
   f4 pre[LT ? 1 : R];
   if constexpr (!LT) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) pre[r] = tile[min(tid + NT * r, NCH - 1)];
+    for (int r = 0; r < R; ++r) pre[r] = tile[(int64_t)twa * tw_step + min(tid + NT * r, NCH - 1)];
   }
-  for (int tw = 0; tw < ntw; ++tw) {
+  for (int tw = twa; tw < tw1; ++tw) {
     if constexpr (LT) {
-      if (tw > 0) __syncthreads();  // the previous window's output pass has read Z
+      if (tw > twa) __syncthreads();  // the previous window's output pass has read Z
       f4 const *nt = tile + (int64_t)tw * tw_step;
       f4 ld[R];
 #pragma unroll
@@ -1561,7 +1569,7 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (r + 1 < R || tid + NT * r < NCH) ((f4 *)IN)[tid + NT * r] = pre[r];
-      if (tw + 1 < ntw) {
+      if (tw + 1 < tw1) {
         f4 const *nt = tile + (int64_t)(tw + 1) * tw_step;
 #pragma unroll
         for (int r = 0; r < R; ++r) pre[r] = nt[min(tid + NT * r, NCH - 1)];
@@ -1641,14 +1649,15 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_fwd_time_t(
       float const y1 = (s & 1) ? z1.y : z1.x;
       float const sc = SC[j];
       if constexpr (LT) {
-        Tb[j * nconv + tw * half + s] = ca[r] + y0 * sc * tap0;
+        if (tw >= tw0) Tb[j * nconv + tw * half + s] = ca[r] + y0 * sc * tap0;
         ca[r] = y1 * sc * tap1;
       } else {
-        Tb[j * nconv + tw * half + s] = CA[j * half + s] + y0 * sc * tap0;
+        if (tw >= tw0) Tb[j * nconv + tw * half + s] = CA[j * half + s] + y0 * sc * tap0;
         CA[j * half + s] = y1 * sc * tap1;
       }
     }
   }
+  if (tw1 < ntw) return;
   int const tl = ntw * half + s;
 #pragma unroll
   for (int r = 0; r < NJ; ++r)
@@ -1693,9 +1702,10 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   // a contiguous run of them, so the (up to four) windows' blocks that read
   // the same traces' record samples meet in one L2
   int const bid = order ? order[block0 + xcd_chunk(blockIdx.x, gridDim.x)] : block0 + blockIdx.x;
-  AgdWin const &w = wins[blocks[2 * bid]];
-  int const t0 = blocks[2 * bid + 1];
-  int const nconv = w.nconv, ntw = w.ntw;
+  AgdWin const &w = wins[blocks[TB_INTS * bid]];
+  int const t0 = blocks[TB_INTS * bid + 1];
+  int const tw0 = blocks[TB_INTS * bid + 2], tw1 = blocks[TB_INTS * bid + 3];
+  int const nconv = w.nconv;
   int const tid = threadIdx.x;
   {
     float2 const *rM = roots + plans[w.tplan].roots_off;
@@ -1737,12 +1747,12 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
   for (int r = 0; r < NJ; ++r) {
     int const j = j0 + JS * r;
     scl[r] = plain ? w.scaler : tscale[w.trace0 + t0 + j];
-    cur[r] = sample(j, s);
-    nxt[r] = sample(j, half + s);
+    cur[r] = sample(j, tw0 * half + s);
+    nxt[r] = sample(j, (tw0 + 1) * half + s);
   }
   float2 *vt = V + w.u_off + (int64_t)(t0 / TG) * Ft * TG;
   int64_t const tw_step = (int64_t)(w.ntr / TG) * Ft * TG;
-  for (int tw = 0; tw < ntw; ++tw) {
+  for (int tw = tw0; tw < tw1; ++tw) {
 #pragma unroll
     for (int r = 0; r < NJ; ++r) {
       float *zr = (float *)(Z + (j0 + JS * r) * LDL);
@@ -1754,7 +1764,7 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
       int const j = i / (Pt - Wt);
       ((float *)(Z + j * LDL))[Wt + (i - j * (Pt - Wt))] = 0.f;
     }
-    if (tw + 1 < ntw) {
+    if (tw + 1 < tw1) {
 #pragma unroll
       for (int r = 0; r < NJ; ++r) nxt[r] = sample(j0 + JS * r, (tw + 2) * half + s);
     }

@@ -73,6 +73,7 @@ def load():
     fn("agdx_n_spatial_classes", ci, vp)
     fn("agdx_n_time_classes", ci, vp)
     fn("agdx_time_factors", ci, vp, vp)
+    fn("agdx_time_blocks", ci, vp, vp)
     fn("agdx_set_time_factors", ci, vp, ci, ci, ci)
     fn("agdx_time_variants", ci, vp, vp)
     fn("agdx_set_time_variant", ci, vp, ci, ci)
@@ -353,6 +354,14 @@ class Problem:
     def set_time_factors(self, c, m1, m2):
         if self.L.agdx_set_time_factors(self.h, int(c), int(m1), int(m2)):
             raise ValueError("no time kernel %d x %d for class %d" % (m1, m2, c))
+
+    def time_blocks(self):
+        """Time blocks as rows (window, first trace, tw0, tw1): the trace
+        tile and the run of its time windows one block transforms."""
+        n = int(self.L.agdx_time_blocks(self.h, None))
+        out = np.zeros(4 * max(n, 1), C_INT)
+        self.L.agdx_time_blocks(self.h, _p(out))
+        return out[:4 * n].reshape(n, 4)
 
     def time_variants(self):
         """Forward time kernel variant of each time class (0 register
