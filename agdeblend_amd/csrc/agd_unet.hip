@@ -935,23 +935,25 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
     iok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
     ioff[k] = iok[k] ? (uint32_t)(hh * W + ww) * 16u + (uint32_t)(half * 8) : (uint32_t)(half * 8);
   }
+  // the loads are raw (clamped addresses); the halo and out-of-range planes
+  // are zeroed when the plane is stored, so that a prefetch stays in flight
+  // until then
   auto load_plane = [&](int dd, uint4 (&v)[NPK]) {
     bool const okd = dd >= 0 && dd < D;
     uint16_t const *const src = xn + (okd ? dd : 0) * plane;
 #pragma unroll
-    for (int k = 0; k < NPK; ++k) {
-      uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;
-      uint4 const t = *(uint4 const *)(src + ioff[k]);
-      v[k] = make_uint4(t.x & msk, t.y & msk, t.z & msk, t.w & msk);
-    }
+    for (int k = 0; k < NPK; ++k) v[k] = *(uint4 const *)(src + ioff[k]);
   };
   auto store_plane = [&](int dd, uint4 const (&v)[NPK]) {
+    bool const okd = dd >= 0 && dd < D;
     uint16_t *const dstp = sx + ((dd + 3) % 3) * PLANE;
 #pragma unroll
     for (int k = 0; k < NPK; ++k) {
       int const e0 = tid + k * UT;
       int const e = e0 < NPI ? e0 : NPI - 1;
-      st_swz(dstp, e >> 1, e & 1, v[k]);
+      uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;
+      st_swz(dstp, e >> 1, e & 1,
+             make_uint4(v[k].x & msk, v[k].y & msk, v[k].z & msk, v[k].w & msk));
     }
   };
   {
@@ -965,8 +967,15 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
   }
   int const row = tid >> 6, col = tid & 63;
   float const b0 = bias[0];
+  constexpr int TPW = (NVT + UT / 64 - 1) / (UT / 64);  // tiles per wave
   for (int j = 0; j < HDB; ++j) {
     int const d = d0 + j;
+    int const h = h0 + row, w = w0 + col;
+    bool const live = h < H && w < W && d < D;
+    int64_t const v = (((int64_t)n * D + (live ? d : 0)) * H + (live ? h : 0)) * W + (live ? w : 0);
+    // the field value is loaded before the prefetch, so that waiting for
+    // it does not wait for the prefetch
+    float const bv = base ? base[v] : 0.f;
     uint4 nxt[NPK];
     if (j + 1 < HDB) load_plane(d + 2, nxt);
     float acc = 0.f;
@@ -974,15 +983,24 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
     for (int kd = 0; kd < 3; ++kd) {
       __syncthreads();  // the ring is staged; the E buffer is free
       uint16_t const *const pl = sx + ((d + kd - 1 + 3) % 3) * PLANE;
-      for (int t = wave; t < NVT; t += UT / 64) {
-        int const u = t * 16 + i;
-        bf16x4 const a = ld_swz(pl, u < NV ? u : NV - 1, kq);
-        f32x4 e = {0.f, 0.f, 0.f, 0.f};
-        e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[kd], e, 0, 0, 0);
-        // D: lane holds voxels 16 t + 4 kq + r of tap i
-        if (i < 9) {
+      // all of the wave's A fragments first, then its MFMAs
+      bf16x4 a[TPW];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) se[(t * 16 + 4 * kq + r) * EP + i] = e[r];
+      for (int q = 0; q < TPW; ++q) {
+        int const u = (wave + q * (UT / 64)) * 16 + i;
+        a[q] = ld_swz(pl, u < NV ? u : NV - 1, kq);
+      }
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) {
+        int const t = wave + q * (UT / 64);
+        if (t < NVT) {
+          f32x4 e = {0.f, 0.f, 0.f, 0.f};
+          e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], bw[kd], e, 0, 0, 0);
+          // D: lane holds voxels 16 t + 4 kq + r of tap i
+          if (i < 9) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) se[(t * 16 + 4 * kq + r) * EP + i] = e[r];
+          }
         }
       }
       __syncthreads();
@@ -992,11 +1010,9 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
         for (int kw = 0; kw < 3; ++kw)
           acc += se[((row + kh) * PW + col + kw) * EP + kh * 3 + kw];
     }
-    int const h = h0 + row, w = w0 + col;
-    if (h < H && w < W && d < D) {
-      int64_t const v = (((int64_t)n * D + d) * H + h) * W + w;
+    if (live) {
       float const o = (acc + b0) * unscale;
-      dst[v] = base ? base[v] + o : o;
+      dst[v] = base ? bv + o : o;
     }
     if (j + 1 < HDB) {
       __syncthreads();  // every wave is done with plane d - 1

@@ -358,3 +358,29 @@ def test_first_layer_and_head_kernels():
     torch.cuda.synchronize()
     ref = field + 0.25 * R.conv3(xin, w, b.astype(np.float64))[..., 0]
     assert np.abs(out.cpu().numpy() - ref).max() <= 1e-4 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("with_base", [True, False])
+def test_head_across_depth_blocks_and_tiles(with_base):
+    """The head over 37 planes (three depth blocks of 16, the last partial),
+    three row tiles and three w tiles (the last partial), with and without
+    the field it adds to: every output against the fp64 convolution."""
+    rng = np.random.default_rng(23)
+    N, D, H, W = 1, 37, 9, 136
+    w = R.bf16_round(rng.standard_normal((27, 16, 16)) / 6.0)
+    b = rng.standard_normal(16).astype(np.float32) * 0.1
+    L = _lib()
+    xin = R.bf16_round(rng.standard_normal((N, D, H, W, 16)))
+    field = rng.standard_normal((N, D, H, W)).astype(np.float32)
+    out = torch.full((N, D, H, W), np.nan, dtype=torch.float32, device="cuda")
+    f_d = torch.tensor(field, device="cuda")
+    x_d, w_d, b_d = _bf(xin), _bf(w), torch.tensor(b, device="cuda")
+    assert L.agdu_conv_out(_p(x_d), _p(w_d), 16, _p(b_d), _p(f_d) if with_base else None,
+                           _p(out), N, D, H, W, 0.5, None) == 0
+    torch.cuda.synchronize()
+    ref = 0.5 * R.conv3(xin, w, b.astype(np.float64))[..., 0]
+    if with_base:
+        ref = ref + field
+    got = out.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
