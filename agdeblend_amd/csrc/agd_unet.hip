@@ -637,12 +637,14 @@ __global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[j][m] = f32x4{0.f, 0.f, 0.f, 0.f};
   int const h = h0 + wave;
+  // loaded once: a load in the loop would be issued after the next plane's
+  // prefetch, and waiting for it would wait for the prefetch too
+  float const b = (mode & 1) ? bias[i] : 0.f;
   // output plane oi (d0 + oi): bias, ReLU, into the wave's rows of buffer
   // oi & 1, then stored by the wave itself
   auto epilogue = [&](int oi) {
     int const d = d0 + oi;
     uint16_t *const ob = so + (oi & 1) * ROWS + wave * 64 * OBS;
-    float const b = (mode & 1) ? bias[i] : 0.f;
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
@@ -823,25 +825,47 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
     tkd[e] = tt / 9;
     toff[e] = ((tt / 3) % 3) * PW + tt % 3;
   }
-  auto stage = [&](int dd) {
+  // a plane's field values: raw loads at clamped offsets, converted and
+  // zeroed outside the volume when stored, so that the next plane's loads
+  // stay in flight while the current one is computed
+  constexpr int NPK = (PLANE + UT - 1) / UT;
+  int foff[NPK];
+  bool fok[NPK];
+#pragma unroll
+  for (int k = 0; k < NPK; ++k) {
+    int const e = min(tid + k * UT, PLANE - 1);
+    int const ph = e / PW, pw = e - ph * PW;
+    int const hh = h0 + ph - 1, ww = w0 + pw - 1;
+    fok[k] = hh >= 0 && hh < H && ww >= 0 && ww < W;
+    foff[k] = fok[k] ? hh * W + ww : 0;
+  }
+  auto load_plane = [&](int dd, float (&v)[NPK]) {
+    float const *const src = fn + (int64_t)(dd >= 0 && dd < D ? dd : 0) * H * W;
+#pragma unroll
+    for (int k = 0; k < NPK; ++k) v[k] = src[foff[k]];
+  };
+  auto store_plane = [&](int dd, float const (&v)[NPK]) {
     uint16_t *const dst = sx + ((dd + 3) % 3) * PLANE;
     bool const okd = dd >= 0 && dd < D;
-    for (int e = tid; e < PLANE; e += UT) {
-      int const ph = e / PW, pw = e - ph * PW;
-      int const hh = h0 + ph - 1, ww = w0 + pw - 1;
-      bool const ok = okd && hh >= 0 && hh < H && ww >= 0 && ww < W;
-      float const v = fn[ok ? ((int64_t)dd * H + hh) * W + ww : 0];
-      dst[e] = ok ? f2bf(v * scale) : (uint16_t)0;
+#pragma unroll
+    for (int k = 0; k < NPK; ++k) {
+      int const e = tid + k * UT;
+      if (e < PLANE) dst[e] = (okd && fok[k]) ? f2bf(v[k] * scale) : (uint16_t)0;
     }
   };
-  stage(d0 - 1);
-  stage(d0);
   float const bsel = bias[i];
+  float nv[NPK];
+  load_plane(d0 - 1, nv);
+  store_plane(d0 - 1, nv);
+  load_plane(d0, nv);
+  store_plane(d0, nv);
+  load_plane(d0 + 1, nv);
   int const h = h0 + wave;
   uint16_t *const obw = ob + wave * 64 * OBS;
   for (int j = 0; j < FDB; ++j) {
     int const d = d0 + j;
-    stage(d + 1);
+    store_plane(d + 1, nv);
+    if (j + 1 < FDB) load_plane(d + 2, nv);
     __syncthreads();
     f32x4 acc[4];
 #pragma unroll
