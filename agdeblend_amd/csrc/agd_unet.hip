@@ -209,28 +209,55 @@ __global__ __launch_bounds__(UT) void k_conv3(
     iofu[k] = iok[k] ? (uint32_t)((hh >> 1) * Wu + (ww >> 1)) * (uint32_t)CU + (uint32_t)(half * 8)
                      : (uint32_t)(half * 8);
   }
-  // global loads of plane dd of chunk c (zeros outside the volume) ...
+  // global loads of plane dd of chunk c, raw (clamped addresses) ...
   auto load_plane = [&](int dd, int c, uint4 (&v)[NPK]) {
     bool const okd = dd >= 0 && dd < D;
     bool const up = CU > 0 && c < CU / 16;  // the same for the whole block
     uint16_t const *const src = up ? xun + (okd ? dd >> 1 : 0) * planeu + c * 16
                                    : xn + (okd ? dd : 0) * plane + (c - CU / 16) * 16;
 #pragma unroll
-    for (int k = 0; k < NPK; ++k) {
-      uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;  // a mask, not a branch
-      uint4 const t = *(uint4 const *)(src + (up ? iofu[k] : ioff[k]));
-      v[k] = make_uint4(t.x & msk, t.y & msk, t.z & msk, t.w & msk);
-    }
+    for (int k = 0; k < NPK; ++k) v[k] = *(uint4 const *)(src + (up ? iofu[k] : ioff[k]));
   };
-  // ... and their store into ring slot (dd + 3) % 3
+  // ... and their store into ring slot (dd + 3) % 3, zero outside the
+  // volume (a mask, not a branch; applied here so that the loads stay in
+  // flight until the store)
   auto store_plane = [&](int dd, uint4 const (&v)[NPK]) {
+    bool const okd = dd >= 0 && dd < D;
     uint16_t *const dst = sx + ((dd + 3) % 3) * PLANE;
 #pragma unroll
     for (int k = 0; k < NPK; ++k) {
       int const e0 = tid + k * UT;
       int const e = e0 < NPI ? e0 : NPI - 1;
-      st_h(dst, e >> 1, e & 1, v[k]);
+      uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;
+      st_h(dst, e >> 1, e & 1, make_uint4(v[k].x & msk, v[k].y & msk, v[k].z & msk, v[k].w & msk));
     }
+  };
+  // a chunk's first three planes and its weights ([27][CO][16]), loaded
+  // into registers (PRE: while the previous chunk is computed; for the
+  // 64-channel layers, whose registers and LDS allow one block per CU
+  // either way), then stored
+  constexpr bool PRE = CO >= 64;
+  constexpr int NXW = (NWI + UT - 1) / UT;
+  uint4 xv[3][NPK], xw[NXW];
+  auto load_chunk = [&](int c) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) load_plane(d0 - 1 + q, c, xv[q]);
+#pragma unroll
+    for (int k = 0; k < NXW; ++k) {
+      int const e0 = tid + k * UT;
+      int const e = e0 < NWI ? e0 : NWI - 1;
+      xw[k] = *(uint4 const *)(wt + (int64_t)(e >> 1) * CI + c * 16 + (e & 1) * 8);  // tap * CO + co
+    }
+  };
+  auto store_chunk = [&]() {
+#pragma unroll
+    for (int k = 0; k < NXW; ++k) {
+      int const e0 = tid + k * UT;
+      int const e = e0 < NWI ? e0 : NWI - 1;
+      st_h(sw, e >> 1, e & 1, xw[k]);  // past the end: the last item again
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) store_plane(d0 - 1 + q, xv[q]);
   };
   // operands of one matrix-core step (two taps): A for the 4 voxel tiles,
   // B for the NT output channel tiles
@@ -249,44 +276,53 @@ __global__ __launch_bounds__(UT) void k_conv3(
     constexpr int NCK = (NCI + UT - 1) / UT;
     static_assert(DB % 2 == 0 && CT_H % 2 == 0 && NCI * 8 <= 3 * PLANE,
                   "coarse staging: whole coarse planes and rows, within the ring");
+    static_assert(NCK <= 32, "inside-the-volume bits of the coarse items");
+    constexpr int NCW = (NWU + UT - 1) / UT;
+    // per coarse item: its offset (the same for every chunk; 0 outside the
+    // volume) and, in cok, whether it lies inside the volume
+    int64_t coff[NCK];
+    uint32_t cok = 0;
+#pragma unroll
+    for (int k = 0; k < NCK; ++k) {
+      int const e0 = tid + k * UT;
+      int const e = e0 < NCI ? e0 : NCI - 1;
+      int const half = e & 1, r = e >> 1;
+      int const cw = r % CPW, t = r / CPW;
+      int const gd = d0 / 2 - 1 + t / CPH, gh = h0 / 2 - 1 + t % CPH;
+      int const gw = w0 / 2 - 1 + cw;
+      bool const ok = gd >= 0 && gd < D / 2 && gh >= 0 && gh < Hu && gw >= 0 && gw < Wu;
+      coff[k] = (ok ? (int64_t)gd * planeu + (int64_t)(gh * Wu + gw) * CU : 0) + half * 8;
+      cok |= (ok ? 1u : 0u) << k;
+    }
+    // a coarse chunk's region, loaded into registers while the previous
+    // chunk is computed (its combined weights, the same for every block,
+    // come from L2 when stored)
+    uint4 cv[NCK];
+    auto load_coarse = [&](int c) {
+#pragma unroll
+      for (int k = 0; k < NCK; ++k) cv[k] = *(uint4 const *)(xun + coff[k] + c * 16);
+    };
+    load_coarse(0);
 #pragma unroll 1
     for (int c = 0; c < CU / 16; ++c) {
       __syncthreads();
-      {
-        uint4 cv[NCK];
 #pragma unroll
-        for (int k = 0; k < NCK; ++k) {
-          int const e0 = tid + k * UT;
-          int const e = e0 < NCI ? e0 : NCI - 1;
-          int const half = e & 1, r = e >> 1;
-          int const cw = r % CPW, t = r / CPW;
-          int const gd = d0 / 2 - 1 + t / CPH, gh = h0 / 2 - 1 + t % CPH;
-          int const gw = w0 / 2 - 1 + cw;
-          bool const ok = gd >= 0 && gd < D / 2 && gh >= 0 && gh < Hu && gw >= 0 && gw < Wu;
-          uint16_t const *const src =
-              xun + (ok ? (int64_t)gd * planeu + (int64_t)(gh * Wu + gw) * CU : 0) + c * 16 +
-              half * 8;
-          uint32_t const msk = ok ? ~0u : 0u;  // a mask, not a branch
-          uint4 const v = *(uint4 const *)src;
-          cv[k] = make_uint4(v.x & msk, v.y & msk, v.z & msk, v.w & msk);
-        }
+      for (int k = 0; k < NCW; ++k) {
+        int const e0 = tid + k * UT;
+        int const e = e0 < NWU ? e0 : NWU - 1;
+        uint4 const v = *(uint4 const *)(wcu + (int64_t)(e >> 1) * CU + c * 16 + (e & 1) * 8);
+        st_h(sw, e >> 1, e & 1, v);  // combined tap * CO + co
+      }
 #pragma unroll
-        for (int k = 0; k < (NWU + UT - 1) / UT; ++k) {
-          int const e0 = tid + k * UT;
-          int const e = e0 < NWU ? e0 : NWU - 1;
-          int const half = e & 1;
-          int const tc = e >> 1;  // combined tap * CO + co
-          uint4 const v = *(uint4 const *)(wcu + (int64_t)tc * CU + c * 16 + half * 8);
-          st_h(sw, tc, half, v);
-        }
-#pragma unroll
-        for (int k = 0; k < NCK; ++k) {
-          int const e0 = tid + k * UT;
-          int const e = e0 < NCI ? e0 : NCI - 1;
-          st_h(sx, e >> 1, e & 1, cv[k]);  // past the end: the last item again
-        }
+      for (int k = 0; k < NCK; ++k) {
+        int const e0 = tid + k * UT;
+        int const e = e0 < NCI ? e0 : NCI - 1;
+        uint32_t const msk = ((cok >> k) & 1u) ? ~0u : 0u;
+        st_h(sx, e >> 1, e & 1,
+             make_uint4(cv[k].x & msk, cv[k].y & msk, cv[k].z & msk, cv[k].w & msk));
       }
       __syncthreads();
+      if (c + 1 < CU / 16) load_coarse(c + 1);
       int const ph = (h0 + wave) & 1;
       // Streamed over the staged coarse planes: fine plane j (parity j & 1,
       // d0 even) reads coarse plane (j + 1) / 2 + a with its d-combination
@@ -326,28 +362,17 @@ __global__ __launch_bounds__(UT) void k_conv3(
       }
     }
   }
+  if constexpr (PRE) load_chunk(UPC ? CU / 16 : 0);
 #pragma unroll 1
   for (int c = UPC ? CU / 16 : 0; c < CI / 16; ++c) {
     __syncthreads();
-    {
-      uint4 v0[NPK], v1[NPK], v2[NPK];
-      load_plane(d0 - 1, c, v0);
-      load_plane(d0, c, v1);
-      load_plane(d0 + 1, c, v2);
-#pragma unroll
-      for (int k = 0; k < (NWI + UT - 1) / UT; ++k) {
-        int const e0 = tid + k * UT;
-        int const e = e0 < NWI ? e0 : NWI - 1;
-        int const half = e & 1;
-        int const tc = e >> 1;  // tap * CO + co
-        uint4 const v = *(uint4 const *)(wt + (int64_t)tc * CI + c * 16 + half * 8);
-        st_h(sw, tc, half, v);  // past the end: the last item again
-      }
-      store_plane(d0 - 1, v0);
-      store_plane(d0, v1);
-      store_plane(d0 + 1, v2);
-    }
+    if constexpr (!PRE) load_chunk(c);
+    store_chunk();
     __syncthreads();
+    // PRE: the next chunk's loads go out once this chunk's plane loads are
+    // done (in the last plane steps), so that waiting for a plane never
+    // waits for them
+    bool const pre = PRE && c + 1 < CI / 16;
     if constexpr (UPC && DB >= 4) {
       // The x channels of d1a streamed over the input planes p = d0 - 1 ..
       // d0 + DB (in ring slot (p + 3) % 3): each A fragment (plane p, two
@@ -361,7 +386,10 @@ __global__ __launch_bounds__(UT) void k_conv3(
         uint16_t const *const pl = sx + ((p + 3) % 3) * PLANE;
         uint4 nxt[NPK];
         bool const more = jp + 3 < DB + 2;  // plane p + 3 still an input
-        if (more) load_plane(p + 3, c, nxt);
+        if (more)
+          load_plane(p + 3, c, nxt);
+        else if (jp == DB - 1 && pre)
+          load_chunk(c + 1);
 #pragma unroll 1
         for (int st = 0; st < 5; ++st) {
           int const q = 2 * st + (kq >> 1);
@@ -399,7 +427,10 @@ __global__ __launch_bounds__(UT) void k_conv3(
       // plane d + 2 is loaded while plane d is computed, then stored into
       // the slot of plane d - 1
       uint4 nxt[NPK];
-      if (j + 1 < DB) load_plane(d + 2, c, nxt);
+      if (j + 1 < DB)
+        load_plane(d + 2, c, nxt);
+      else if (pre)
+        load_chunk(c + 1);
       // operands of step st: this lane's tap 2 st + (kq >> 1) (the 28th,
       // past the last, reads tap 26 with zero weights), double-buffered so
       // the LDS reads of the next step overlap this step's MFMAs
