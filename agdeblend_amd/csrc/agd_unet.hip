@@ -142,7 +142,7 @@ __host__ __device__ constexpr int up_tap(int pd, int ph, int a, int b, int kw) {
 // k_up_weights from wt) instead of wt's 27: per 16-channel chunk 12
 // combined taps, 6 matrix-core steps of two, instead of 27 taps in 14.
 template <int CI, int CO, int CU, bool UPC = false>
-__global__ __launch_bounds__(UT) void k_conv3(
+__global__ __launch_bounds__(UT, 2) void k_conv3(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ xu,
     uint16_t const *__restrict__ wt, uint16_t const *__restrict__ wcu,
     float const *__restrict__ bias, uint16_t const *__restrict__ gate,
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(UT) void k_conv3(
       // a time.
 #pragma unroll
       for (int cp = 0; cp < CPD; ++cp) {
-#pragma unroll 1
+#pragma unroll
         for (int sp = 0; sp < 3; ++sp) {
           int const u = 2 * sp + (kq >> 1);  // (b, kw) position of this lane
           int const b = u / 3, kw = u - 3 * b;
