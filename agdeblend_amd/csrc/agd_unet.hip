@@ -390,7 +390,7 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
           load_plane(p + 3, c, nxt);
         else if (jp == DB - 1 && pre)
           load_chunk(c + 1);
-#pragma unroll 1
+#pragma unroll
         for (int st = 0; st < 5; ++st) {
           int const q = 2 * st + (kq >> 1);
           int const qq = q < 9 ? q : 8;
