@@ -323,6 +323,7 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
       }
       __syncthreads();
       if (c + 1 < CU / 16) load_coarse(c + 1);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
       int const ph = (h0 + wave) & 1;
       // Streamed over the staged coarse planes: fine plane j (parity j & 1,
       // d0 even) reads coarse plane (j + 1) / 2 + a with its d-combination
@@ -390,6 +391,7 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
           load_plane(p + 3, c, nxt);
         else if (jp == DB - 1 && pre)
           load_chunk(c + 1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
 #pragma unroll
         for (int st = 0; st < 5; ++st) {
           int const q = 2 * st + (kq >> 1);
@@ -431,6 +433,7 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
         load_plane(d + 2, c, nxt);
       else if (pre)
         load_chunk(c + 1);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
       // operands of step st: this lane's tap 2 st + (kq >> 1) (the 28th,
       // past the last, reads tap 26 with zero weights), double-buffered so
       // the LDS reads of the next step overlap this step's MFMAs
@@ -758,6 +761,7 @@ __global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
     if (j + 1 < DB + 2) {
       store_plane(sx + ((j + 1) & 1) * PLANE, nxt);
       if (j + 2 < DB + 2) load_plane(d0 + 1 + j, nxt);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
     }
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
@@ -897,6 +901,7 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
     int const d = d0 + j;
     store_plane(d + 1, nv);
     if (j + 1 < FDB) load_plane(d + 2, nv);
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
     __syncthreads();
     f32x4 acc[4];
 #pragma unroll
@@ -959,9 +964,13 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
   constexpr int PLANE = NV * 16;        // bf16 elements per staged plane
   constexpr int NPI = NV * 2;           // staging items of 8 channels
   constexpr int NPK = (NPI + UT - 1) / UT;
-  constexpr int EP = 9;                 // floats per E row (9 taps)
+  // E tap-major, one row of NVP >= 16 NVT floats per tap: a lane's four
+  // consecutive voxels of one tap are one 16-byte write, and NVP = 20 mod 32
+  // spreads the eight lanes of a write cycle over distinct bank groups
+  constexpr int NVP = 404;
+  static_assert(NVP >= NVT * 16 && NVP % 32 == 20, "E rows");
   __shared__ __align__(16) uint16_t sx[3 * PLANE];
-  __shared__ __align__(16) float se[NVT * 16 * EP];
+  __shared__ __align__(16) float se[9 * NVP];
   int const tid = threadIdx.x;
   int const lane = tid & 63, wave = tid >> 6;
   int const i = lane & 15, kq = lane >> 4;
@@ -1033,6 +1042,7 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
     float const bv = base ? base[v] : 0.f;
     uint4 nxt[NPK];
     if (j + 1 < HDB) load_plane(d + 2, nxt);
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
     float acc = 0.f;
 #pragma unroll
     for (int kd = 0; kd < 3; ++kd) {
@@ -1052,10 +1062,7 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
           f32x4 e = {0.f, 0.f, 0.f, 0.f};
           e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], bw[kd], e, 0, 0, 0);
           // D: lane holds voxels 16 t + 4 kq + r of tap i
-          if (i < 9) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) se[(t * 16 + 4 * kq + r) * EP + i] = e[r];
-          }
+          if (i < 9) *(f32x4 *)(se + i * NVP + t * 16 + 4 * kq) = e;
         }
       }
       __syncthreads();
@@ -1063,7 +1070,7 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
       for (int kh = 0; kh < 3; ++kh)
 #pragma unroll
         for (int kw = 0; kw < 3; ++kw)
-          acc += se[((row + kh) * PW + col + kw) * EP + kh * 3 + kw];
+          acc += se[(kh * 3 + kw) * NVP + (row + kh) * PW + col + kw];
     }
     if (live) {
       float const o = (acc + b0) * unscale;
