@@ -584,15 +584,20 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
 // position, past the ninth, reads the ninth with zero weights). Per 16
 // voxels and input plane this reads 5 A fragments from LDS and no B, where
 // k_conv3 reads 14 A and 14 B fragments per 16 voxels and output plane.
-// Block: CT_H rows x CT_W voxels (one row per wave) of DB output planes.
+// Block: CT_H rows x CT_W voxels of DB output planes; without pooling one
+// row per wave, with it (POOL) two rows x 32 voxels per wave (wave w: rows
+// 2 (w >> 1) + (m >> 1), voxels 32 (w & 1) + 16 (m & 1) + 0..15 of its
+// 16-voxel tiles m), so that a lane holds every voxel of its 2 x 2 pooling
+// windows (the w pair in one tile, the h pair in tiles m and m + 2).
 // Epilogue per output plane: bias and ReLU (mode bits 1, 2), the wave's
-// rows through its own LDS rows as 16-byte coalesced stores; with mode bit 8
-// the 2x2x2 max pooling (k_maxpool's scan order and strict >) is formed one
-// plane later from the four waves' rows (D, H, W even, DB even).
+// voxels through its own LDS rows as 16-byte coalesced stores; POOL (mode
+// bit 8): the 2x2x2 max pooling (k_maxpool's scan order and strict >) in
+// registers, the even plane's partial kept for the odd one (D, H, W even,
+// DB even).
 // grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / DB)); block 256.
 // ---------------------------------------------------------------------------
-template <int DB>
-__global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
+template <int DB, bool POOL>
+__global__ __launch_bounds__(UT, 2) void k_conv3s(uint16_t const *__restrict__ x,
                                                uint16_t const *__restrict__ wt,
                                                float const *__restrict__ bias,
                                                uint16_t *__restrict__ y,
@@ -605,10 +610,9 @@ __global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
   constexpr int NPK = (NPI + UT - 1) / UT;
   constexpr int OBS = 16 + 4;               // output row: 16 channels + 4 (bank spread)
   constexpr int ROWS = CT_H * 64 * OBS;     // one output plane's rows
-  static_assert(CT_H == 4 && UT == 256, "pooling: one thread per (pooled row, column, 4 channels)");
+  static_assert(CT_H == 4 && CT_W == 64 && UT == 256, "four waves of 64 voxels");
   __shared__ __align__(16) uint16_t sx[2 * PLANE];
-  __shared__ __align__(16) uint16_t so[2 * ROWS + 2 * 32 * 16];
-  uint16_t *const pm = so + 2 * ROWS;  // pooling partials of the even plane [2][32][16]
+  __shared__ __align__(16) uint16_t so[2 * ROWS];
   int const tid = threadIdx.x;
   int const lane = tid & 63;
   int const wave = tid >> 6;
@@ -670,78 +674,70 @@ __global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
   for (int j = 0; j < DB; ++j)
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[j][m] = f32x4{0.f, 0.f, 0.f, 0.f};
-  int const h = h0 + wave;
+  // row and first voxel (within the block) of the wave's tile m
+  auto trow = [&](int m) { return POOL ? 2 * (wave >> 1) + (m >> 1) : wave; };
+  auto tcol = [&](int m) { return POOL ? 32 * (wave & 1) + 16 * (m & 1) : 16 * m; };
   // loaded once: a load in the loop would be issued after the next plane's
   // prefetch, and waiting for it would wait for the prefetch too
   float const b = (mode & 1) ? bias[i] : 0.f;
+  // POOL: the even plane's pooling partials of this lane, per (tile column
+  // mc, voxel pair rp) (one of the bf16 values, so exact)
+  float pp[2][2] = {{-INFINITY, -INFINITY}, {-INFINITY, -INFINITY}};
   // output plane oi (d0 + oi): bias, ReLU, into the wave's rows of buffer
-  // oi & 1, then stored by the wave itself
+  // oi & 1, then stored by the wave itself; POOL: its pooling
   auto epilogue = [&](int oi) {
     int const d = d0 + oi;
     uint16_t *const ob = so + (oi & 1) * ROWS + wave * 64 * OBS;
+    float fv[4][4];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         float v = acc[oi][m][r] + b;
         if (mode & 2) v = fmaxf(v, 0.f);
-        ob[(16 * m + 4 * kq + r) * OBS + i] = f2bf(v);
+        uint16_t const q = f2bf(v);
+        fv[m][r] = bf2f(q);
+        ob[(16 * m + 4 * kq + r) * OBS + i] = q;
       }
     __builtin_amdgcn_wave_barrier();
-    if (h < H && d < D) {
-      int64_t const orow = (((int64_t)n * D + d) * H + h) * W;
+    if (d < D) {
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        int const e = lane + 64 * k;  // (voxel, 8-channel half)
+        int const e = lane + 64 * k;  // (voxel of the wave, 8-channel half)
         int const v = e >> 1, c8 = e & 1;
-        if (w0 + v < W) {
+        int const hh = h0 + trow(v >> 4), ww = w0 + tcol(v >> 4) + (v & 15);
+        if (hh < H && ww < W) {
           uint2 const lo = *(uint2 const *)(ob + v * OBS + c8 * 8);
           uint2 const hi = *(uint2 const *)(ob + v * OBS + c8 * 8 + 4);
-          *(uint4 *)(y + (orow + w0 + v) * 16 + c8 * 8) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+          *(uint4 *)(y + ((((int64_t)n * D + d) * H + hh) * W + ww) * 16 + c8 * 8) =
+              make_uint4(lo.x, lo.y, hi.x, hi.y);
         }
       }
     }
-  };
-  // 2x2x2 max pooling of output plane oi from all four waves' rows (written
-  // before the last barrier): item (pooled row pr, column pc, 4 channels),
-  // one per thread, scans its voxels in (d, h, w) order with a strict >; the
-  // even plane's partial (one of the bf16 values, so exact) waits in pm for
-  // the odd one, handled by the same thread
-  auto pool_plane = [&](int oi) {
-    int const d = d0 + oi;
-    uint16_t const *const rows = so + (oi & 1) * ROWS;
-    int const c4 = tid & 3, pc = (tid >> 2) & 31, pr = tid >> 7;
-    uint16_t *const pp = pm + (pr * 32 + pc) * 16 + c4 * 4;
-    float best[4];
-    if (oi % 2 == 0) {
+    if constexpr (POOL) {
+      // pooling window (tile column mc, voxel pair rp): rows m = mc, mc + 2
+      // (h order), voxels 2 rp, 2 rp + 1 (w order), after the even plane's
+      bool const odd = oi & 1;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) best[k] = -INFINITY;
-    } else {
+      for (int mc = 0; mc < 2; ++mc)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) best[k] = bf2f(pp[k]);
-    }
+        for (int rp = 0; rp < 2; ++rp) {
+          float best = odd ? pp[mc][rp] : -INFINITY;
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      uint16_t const *const r =
-          rows + (2 * pr + (o >> 1)) * 64 * OBS + (2 * pc + (o & 1)) * OBS + c4 * 4;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float const f = bf2f(r[k]);
-        if (f > best[k]) best[k] = f;
-      }
-    }
-    if (oi % 2 == 0) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) pp[k] = f2bf(best[k]);
-    } else {
-      int const dp = d / 2, hp = h0 / 2 + pr, wp = w0 / 2 + pc;
-      if (d < D && 2 * hp < H && 2 * wp < W) {
-        uint32_t const q0 = (uint32_t)f2bf(best[0]) | ((uint32_t)f2bf(best[1]) << 16);
-        uint32_t const q1 = (uint32_t)f2bf(best[2]) | ((uint32_t)f2bf(best[3]) << 16);
-        int64_t const po =
-            ((((int64_t)n * (D / 2) + dp) * (H / 2) + hp) * (W / 2) + wp) * 16 + c4 * 4;
-        *(uint2 *)(pool + po) = make_uint2(q0, q1);
-      }
+          for (int o = 0; o < 4; ++o) {
+            float const f = fv[2 * (o >> 1) + mc][2 * rp + (o & 1)];
+            if (f > best) best = f;
+          }
+          if (!odd) {
+            pp[mc][rp] = best;
+          } else {
+            int const dp = d / 2, hp = h0 / 2 + (wave >> 1);
+            int const wp = w0 / 2 + 16 * (wave & 1) + 8 * mc + 2 * kq + rp;
+            if (d < D && 2 * hp < H && 2 * wp < W)
+              pool[((((int64_t)n * (D / 2) + dp) * (H / 2) + hp) * (W / 2) + wp) * 16 + i] =
+                  f2bf(best);
+          }
+        }
     }
   };
   uint4 nxt[NPK];
@@ -768,10 +764,10 @@ __global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
       int const q = 2 * s + (kq >> 1);
       int const qq = q < 9 ? q : 8;
       int const kh = qq / 3, kw = qq - 3 * (qq / 3);
-      int const v0 = (wave + kh) * PW + i + kw;
       u32x4 a[4];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) a[m] = ld_h(cur, v0 + 16 * m, kq & 1);
+      for (int m = 0; m < 4; ++m)
+        a[m] = ld_h(cur, (trow(m) + kh) * PW + tcol(m) + i + kw, kq & 1);
 #pragma unroll
       for (int kd = 0; kd < 3; ++kd) {
         int const oi = j - kd;  // output plane d0 + oi = p + 1 - kd
@@ -782,10 +778,8 @@ __global__ __launch_bounds__(UT) void k_conv3s(uint16_t const *__restrict__ x,
       }
     }
     if (j >= 2) epilogue(j - 2);
-    if ((mode & 8) && j >= 3) pool_plane(j - 3);
     __syncthreads();
   }
-  if (mode & 8) pool_plane(DB - 1);
 }
 
 // Combined taps of the first CU input channels of wt ([27][CO][CI]):
@@ -1580,8 +1574,13 @@ static int conv3s_launch(void const *x, void const *wt, float const *bias, void 
                          int N, int D, int H, int W, int mode, void *stream) {
   dim3 const grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H),
                   (unsigned)(N * ((D + CONVS_DB - 1) / CONVS_DB)));
-  return ulaunch("conv3s", k_conv3s<CONVS_DB>, grid, dim3(UT), stream, (uint16_t const *)x,
-                 (uint16_t const *)wt, bias, (uint16_t *)y, (uint16_t *)pool, D, H, W, mode);
+  if (mode & 8)
+    return ulaunch("conv3s", k_conv3s<CONVS_DB, true>, grid, dim3(UT), stream,
+                   (uint16_t const *)x, (uint16_t const *)wt, bias, (uint16_t *)y,
+                   (uint16_t *)pool, D, H, W, mode);
+  return ulaunch("conv3s", k_conv3s<CONVS_DB, false>, grid, dim3(UT), stream,
+                 (uint16_t const *)x, (uint16_t const *)wt, bias, (uint16_t *)y,
+                 (uint16_t *)pool, D, H, W, mode);
 }
 
 int agdu_conv3(void const *x, void const *wt, float const *bias, void const *gate, void *y,
