@@ -265,104 +265,108 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
     u32x4 a[4];
     u32x4 b[NT];
   };
-  if constexpr (UPC) {
-    // The upsampled channels, on the 48 combined taps ([48][CO][CU]) and
-    // staged at their own (coarse) resolution, the whole block's region at
-    // once: coarse planes d0/2 - 1 .. (d0 + DB)/2, rows h0/2 - 1 ..
-    // (h0 + CT_H)/2, columns w0/2 - 1 .. (w0 + CT_W)/2 (d0, h0, w0 even),
-    // row (cd * CPH + ch) * CPW + cw of the ring's LDS.
-    constexpr int CPW = CT_W / 2 + 2, CPH = CT_H / 2 + 2, CPD = DB / 2 + 2;
-    constexpr int NCI = CPD * CPH * CPW * 2;  // staging items (8 channels)
-    constexpr int NCK = (NCI + UT - 1) / UT;
-    static_assert(DB % 2 == 0 && CT_H % 2 == 0 && NCI * 8 <= 3 * PLANE,
-                  "coarse staging: whole coarse planes and rows, within the ring");
-    static_assert(NCK <= 32, "inside-the-volume bits of the coarse items");
-    constexpr int NCW = (NWU + UT - 1) / UT;
-    // per coarse item: its offset (the same for every chunk; 0 outside the
-    // volume) and, in cok, whether it lies inside the volume
-    int64_t coff[NCK];
-    uint32_t cok = 0;
+  // UPC: the upsampled channels, on the 48 combined taps ([48][CO][CU]) and
+  // staged at their own (coarse) resolution, the whole block's region at
+  // once: coarse planes d0/2 - 1 .. (d0 + DB)/2, rows h0/2 - 1 ..
+  // (h0 + CT_H)/2, columns w0/2 - 1 .. (w0 + CT_W)/2 (d0, h0, w0 even),
+  // row (cd * CPH + ch) * CPW + cw of the ring's LDS. They are computed
+  // after the x channels, whose last planes' compute overlaps the loads of
+  // the first coarse chunk.
+  constexpr int CPW = CT_W / 2 + 2, CPH = CT_H / 2 + 2, CPD = DB / 2 + 2;
+  constexpr int NCI = CPD * CPH * CPW * 2;  // staging items (8 channels)
+  constexpr int NCK = (NCI + UT - 1) / UT;
+  static_assert(!UPC || (DB % 2 == 0 && CT_H % 2 == 0 && NCI * 8 <= 3 * PLANE),
+                "coarse staging: whole coarse planes and rows, within the ring");
+  static_assert(!UPC || NCK <= 32, "inside-the-volume bits of the coarse items");
+  constexpr int NCW = (NWU + UT - 1) / UT;
+  // per coarse item: its offset (the same for every chunk; 0 outside the
+  // volume) and, in cok, whether it lies inside the volume
+  int64_t coff[UPC ? NCK : 1];
+  uint32_t cok = 0;
 #pragma unroll
-    for (int k = 0; k < NCK; ++k) {
-      int const e0 = tid + k * UT;
-      int const e = e0 < NCI ? e0 : NCI - 1;
-      int const half = e & 1, r = e >> 1;
-      int const cw = r % CPW, t = r / CPW;
-      int const gd = d0 / 2 - 1 + t / CPH, gh = h0 / 2 - 1 + t % CPH;
-      int const gw = w0 / 2 - 1 + cw;
-      bool const ok = gd >= 0 && gd < D / 2 && gh >= 0 && gh < Hu && gw >= 0 && gw < Wu;
-      coff[k] = (ok ? (int64_t)gd * planeu + (int64_t)(gh * Wu + gw) * CU : 0) + half * 8;
-      cok |= (ok ? 1u : 0u) << k;
-    }
-    // a coarse chunk's region, loaded into registers while the previous
-    // chunk is computed (its combined weights, the same for every block,
-    // come from L2 when stored)
-    uint4 cv[NCK];
-    auto load_coarse = [&](int c) {
+  for (int k = 0; k < (UPC ? NCK : 0); ++k) {
+    int const e0 = tid + k * UT;
+    int const e = e0 < NCI ? e0 : NCI - 1;
+    int const half = e & 1, r = e >> 1;
+    int const cw = r % CPW, t = r / CPW;
+    int const gd = d0 / 2 - 1 + t / CPH, gh = h0 / 2 - 1 + t % CPH;
+    int const gw = w0 / 2 - 1 + cw;
+    bool const ok = gd >= 0 && gd < D / 2 && gh >= 0 && gh < Hu && gw >= 0 && gw < Wu;
+    coff[k] = (ok ? (int64_t)gd * planeu + (int64_t)(gh * Wu + gw) * CU : 0) + half * 8;
+    cok |= (ok ? 1u : 0u) << k;
+  }
+  // a coarse chunk's region, loaded into registers while the previous
+  // chunk is computed (its combined weights, the same for every block,
+  // come from L2 when stored)
+  uint4 cv[UPC ? NCK : 1];
+  auto load_coarse = [&](int c) {
 #pragma unroll
-      for (int k = 0; k < NCK; ++k) cv[k] = *(uint4 const *)(xun + coff[k] + c * 16);
-    };
-    load_coarse(0);
+    for (int k = 0; k < (UPC ? NCK : 0); ++k) cv[k] = *(uint4 const *)(xun + coff[k] + c * 16);
+  };
+  // the coarse chunks (UPC), chunk 0's region already in cv
+  auto coarse_part = [&]() {
+    if constexpr (UPC) {
 #pragma unroll 1
-    for (int c = 0; c < CU / 16; ++c) {
-      __syncthreads();
+      for (int c = 0; c < CU / 16; ++c) {
+        __syncthreads();
 #pragma unroll
-      for (int k = 0; k < NCW; ++k) {
-        int const e0 = tid + k * UT;
-        int const e = e0 < NWU ? e0 : NWU - 1;
-        uint4 const v = *(uint4 const *)(wcu + (int64_t)(e >> 1) * CU + c * 16 + (e & 1) * 8);
-        st_h(sw, e >> 1, e & 1, v);  // combined tap * CO + co
-      }
+        for (int k = 0; k < NCW; ++k) {
+          int const e0 = tid + k * UT;
+          int const e = e0 < NWU ? e0 : NWU - 1;
+          uint4 const v = *(uint4 const *)(wcu + (int64_t)(e >> 1) * CU + c * 16 + (e & 1) * 8);
+          st_h(sw, e >> 1, e & 1, v);  // combined tap * CO + co
+        }
 #pragma unroll
-      for (int k = 0; k < NCK; ++k) {
-        int const e0 = tid + k * UT;
-        int const e = e0 < NCI ? e0 : NCI - 1;
-        uint32_t const msk = ((cok >> k) & 1u) ? ~0u : 0u;
-        st_h(sx, e >> 1, e & 1,
-             make_uint4(cv[k].x & msk, cv[k].y & msk, cv[k].z & msk, cv[k].w & msk));
-      }
-      __syncthreads();
-      if (c + 1 < CU / 16) load_coarse(c + 1);
-      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
-      int const ph = (h0 + wave) & 1;
-      // Streamed over the staged coarse planes: fine plane j (parity j & 1,
-      // d0 even) reads coarse plane (j + 1) / 2 + a with its d-combination
-      // a, so coarse plane cp feeds fine planes 2 (cp - a) - 1 and
-      // 2 (cp - a) for a = 0, 1. Each A fragment (coarse plane cp, h
-      // combination b, tap kw) is read once and used for those (up to
-      // four) fine planes; K = 32 pairs two of the six (b, kw) positions.
-      // Per fine plane the contributions keep their order (a = 0, then
-      // a = 1; positions in order), so the sums are those of one plane at
-      // a time.
+        for (int k = 0; k < NCK; ++k) {
+          int const e0 = tid + k * UT;
+          int const e = e0 < NCI ? e0 : NCI - 1;
+          uint32_t const msk = ((cok >> k) & 1u) ? ~0u : 0u;
+          st_h(sx, e >> 1, e & 1,
+               make_uint4(cv[k].x & msk, cv[k].y & msk, cv[k].z & msk, cv[k].w & msk));
+        }
+        __syncthreads();
+        if (c + 1 < CU / 16) load_coarse(c + 1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
+        int const ph = (h0 + wave) & 1;
+        // Streamed over the staged coarse planes: fine plane j (parity j & 1,
+        // d0 even) reads coarse plane (j + 1) / 2 + a with its d-combination
+        // a, so coarse plane cp feeds fine planes 2 (cp - a) - 1 and
+        // 2 (cp - a) for a = 0, 1. Each A fragment (coarse plane cp, h
+        // combination b, tap kw) is read once and used for those (up to
+        // four) fine planes; K = 32 pairs two of the six (b, kw) positions.
+        // Per fine plane the contributions keep their order (a = 0, then
+        // a = 1; positions in order), so the sums are those of one plane at
+        // a time.
 #pragma unroll
-      for (int cp = 0; cp < CPD; ++cp) {
+        for (int cp = 0; cp < CPD; ++cp) {
 #pragma unroll
-        for (int sp = 0; sp < 3; ++sp) {
-          int const u = 2 * sp + (kq >> 1);  // (b, kw) position of this lane
-          int const b = u / 3, kw = u - 3 * b;
-          int const v0 = (cp * CPH + (wave + 1) / 2 + b) * CPW + 1;
-          u32x4 a4[4];
+          for (int sp = 0; sp < 3; ++sp) {
+            int const u = 2 * sp + (kq >> 1);  // (b, kw) position of this lane
+            int const b = u / 3, kw = u - 3 * b;
+            int const v0 = (cp * CPH + (wave + 1) / 2 + b) * CPW + 1;
+            u32x4 a4[4];
 #pragma unroll
-          for (int m = 0; m < 4; ++m) a4[m] = ld_h(sx, v0 + ((16 * m + i + kw - 1) >> 1), kq & 1);
+            for (int m = 0; m < 4; ++m) a4[m] = ld_h(sx, v0 + ((16 * m + i + kw - 1) >> 1), kq & 1);
 #pragma unroll
-          for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int jp = 0; jp < 2; ++jp) {
-              int const j = 2 * (cp - a) - 1 + jp;
-              if (j >= 0 && j < DB) {
-                int const pd = j & 1;
+              for (int jp = 0; jp < 2; ++jp) {
+                int const j = 2 * (cp - a) - 1 + jp;
+                if (j >= 0 && j < DB) {
+                  int const pd = j & 1;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                  u32x4 const bt = ld_h(sw, up_tap(pd, ph, a, b, kw) * CO + t * 16 + i, kq & 1);
+                  for (int t = 0; t < NT; ++t) {
+                    u32x4 const bt = ld_h(sw, up_tap(pd, ph, a, b, kw) * CO + t * 16 + i, kq & 1);
 #pragma unroll
-                  for (int m = 0; m < 4; ++m) acc[j][m][t] = mfma32(a4[m], bt, acc[j][m][t]);
+                    for (int m = 0; m < 4; ++m) acc[j][m][t] = mfma32(a4[m], bt, acc[j][m][t]);
+                  }
                 }
               }
-            }
+          }
         }
       }
     }
-  }
+  };
   if constexpr (PRE) load_chunk(UPC ? CU / 16 : 0);
 #pragma unroll 1
   for (int c = UPC ? CU / 16 : 0; c < CI / 16; ++c) {
@@ -391,6 +395,8 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
           load_plane(p + 3, c, nxt);
         else if (jp == DB - 1 && pre)
           load_chunk(c + 1);
+        else if (jp == DB - 1 && c + 1 == CI / 16)
+          load_coarse(0);
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
 #pragma unroll
         for (int st = 0; st < 5; ++st) {
@@ -433,6 +439,8 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
         load_plane(d + 2, c, nxt);
       else if (pre)
         load_chunk(c + 1);
+      else if (UPC && c + 1 == CI / 16)
+        load_coarse(0);
       __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
       // operands of step st: this lane's tap 2 st + (kq >> 1) (the 28th,
       // past the last, reads tap 26 with zero weights), double-buffered so
@@ -471,6 +479,7 @@ __global__ __launch_bounds__(UT, 2) void k_conv3(
       }
     }
   }
+  coarse_part();
   // Epilogue per plane: each wave writes its row (bias, ReLU applied) into
   // its own LDS rows [voxel][OBS] (lane (i, kq) holds voxels 4 kq + r of
   // channel t * 16 + i; the row padding puts the four kq groups in distinct
