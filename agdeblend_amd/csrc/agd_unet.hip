@@ -945,13 +945,14 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
 // ---------------------------------------------------------------------------
 // Head: output channel 0 of a 16 -> 16 convolution, in fp32, plus the field:
 // out[v] = base[v] + s (b0 + sum_tap sum_ci x[v + tap][ci] w[tap][0][ci]).
-// For an input plane p and depth offset kd, E[u][(kh, kw)] =
-// sum_ci x_p[u][ci] w[(kd, kh, kw)][0][ci] over every staged voxel u (halo
-// included) is a GEMM with M = voxels, K = 16 channels, N = 9 taps (padded
-// to 16): one MFMA per 16 voxels. Then out[v] gathers E[v + (kh - 1,
-// kw - 1)][(kh, kw)] over the three planes, each with its own kd. Block:
-// CT_H rows x CT_W voxels of HDB consecutive d planes, the input planes in
-// a ring of three (swizzled rows, as in k_conv3), one E buffer reused per kd.
+// For an output plane d, F[u][(kh, kw)] = sum_kd sum_ci x_{d+kd-1}[u][ci]
+// w[(kd, kh, kw)][0][ci] over every staged voxel u (halo included) is a GEMM
+// with M = voxels, K = (kd, ci) = 48 and N = 9 taps (padded to 16): two
+// K = 32 MFMAs per 16 voxels (planes d - 1 and d, then plane d + 1 with the
+// second half's weights zero). Then out[v] gathers F[v + (kh - 1, kw - 1)]
+// [(kh, kw)], 9 terms. Block: CT_H rows x CT_W voxels of HDB consecutive d
+// planes, the input planes in a ring of three (16-byte halves swizzled as in
+// k_conv3), the next plane loaded while the current one is computed.
 // grid: (ceil(W / CT_W), ceil(H / CT_H), N * ceil(D / HDB)); block 256.
 // ---------------------------------------------------------------------------
 #define HDB 16
@@ -967,11 +968,11 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
   constexpr int PLANE = NV * 16;        // bf16 elements per staged plane
   constexpr int NPI = NV * 2;           // staging items of 8 channels
   constexpr int NPK = (NPI + UT - 1) / UT;
-  // E tap-major, one row of NVP >= 16 NVT floats per tap: a lane's four
+  // F tap-major, one row of NVP >= 16 NVT floats per tap: a lane's four
   // consecutive voxels of one tap are one 16-byte write, and NVP = 20 mod 32
   // spreads the eight lanes of a write cycle over distinct bank groups
   constexpr int NVP = 404;
-  static_assert(NVP >= NVT * 16 && NVP % 32 == 20, "E rows");
+  static_assert(NVP >= NVT * 16 && NVP % 32 == 20, "F rows");
   __shared__ __align__(16) uint16_t sx[3 * PLANE];
   __shared__ __align__(16) float se[9 * NVP];
   int const tid = threadIdx.x;
@@ -983,13 +984,23 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
   int const h0 = blockIdx.y * CT_H, w0 = blockIdx.x * CT_W;
   uint16_t const *const xn = x + (int64_t)n * D * H * W * 16;
   int64_t const plane = (int64_t)H * W * 16;
-  // B per kd: taps (kd, t9 = i) of output channel 0, input channels 4 kq..
-  bf16x4 bw[3];
+  // B: lane (tap i, k group kq) holds input channels 8 (kq & 1) .. + 7 of
+  // depth tap kd = kq >> 1 (bw0) and kd = 2 for kq < 2, zero for kq >= 2
+  // (bw1), of output channel 0; taps i >= 9 zero
+  u32x4 bw0, bw1;
+  {
+    unsigned h0w[8], h1w[8];
 #pragma unroll
-  for (int kd = 0; kd < 3; ++kd)
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      bw[kd][r] = i < 9 ? (short)wt[((int64_t)(kd * 9 + i) * wco) * 16 + 4 * kq + r] : (short)0;
+    for (int e = 0; e < 8; ++e) {
+      int const ci = 8 * (kq & 1) + e;
+      h0w[e] = i < 9 ? (unsigned)wt[((int64_t)((kq >> 1) * 9 + i) * wco) * 16 + ci] : 0u;
+      h1w[e] = (i < 9 && kq < 2) ? (unsigned)wt[((int64_t)(18 + i) * wco) * 16 + ci] : 0u;
+    }
+    bw0 = u32x4{h0w[0] | h0w[1] << 16, h0w[2] | h0w[3] << 16, h0w[4] | h0w[5] << 16,
+                h0w[6] | h0w[7] << 16};
+    bw1 = u32x4{h1w[0] | h1w[1] << 16, h1w[2] | h1w[3] << 16, h1w[4] | h1w[5] << 16,
+                h1w[6] | h1w[7] << 16};
+  }
   uint32_t ioff[NPK];
   bool iok[NPK];
 #pragma unroll
@@ -1019,8 +1030,7 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
       int const e0 = tid + k * UT;
       int const e = e0 < NPI ? e0 : NPI - 1;
       uint32_t const msk = (okd && iok[k]) ? ~0u : 0u;
-      st_swz(dstp, e >> 1, e & 1,
-             make_uint4(v[k].x & msk, v[k].y & msk, v[k].z & msk, v[k].w & msk));
+      st_h(dstp, e >> 1, e & 1, make_uint4(v[k].x & msk, v[k].y & msk, v[k].z & msk, v[k].w & msk));
     }
   };
   {
@@ -1046,42 +1056,42 @@ __global__ __launch_bounds__(UT) void k_conv_head(uint16_t const *__restrict__ x
     uint4 nxt[NPK];
     if (j + 1 < HDB) load_plane(d + 2, nxt);
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
-    float acc = 0.f;
-#pragma unroll
-    for (int kd = 0; kd < 3; ++kd) {
-      __syncthreads();  // the ring is staged; the E buffer is free
-      uint16_t const *const pl = sx + ((d + kd - 1 + 3) % 3) * PLANE;
-      // all of the wave's A fragments first, then its MFMAs
-      bf16x4 a[TPW];
+    __syncthreads();  // the ring holds planes d - 1, d, d + 1; F is free
+    {
+      // lanes kq < 2 read plane d - 1 (first MFMA) and d + 1 (second), lanes
+      // kq >= 2 plane d (first) and d + 1 again (second: zero weights)
+      uint16_t const *const p1 = sx + ((kq < 2 ? d + 2 : d) % 3) * PLANE;
+      uint16_t const *const p2 = sx + ((d + 1) % 3) * PLANE;
+      u32x4 a1[TPW], a2[TPW];
 #pragma unroll
       for (int q = 0; q < TPW; ++q) {
         int const u = (wave + q * (UT / 64)) * 16 + i;
-        a[q] = ld_swz(pl, u < NV ? u : NV - 1, kq);
+        a1[q] = ld_h(p1, u < NV ? u : NV - 1, kq & 1);
+        a2[q] = ld_h(p2, u < NV ? u : NV - 1, kq & 1);
       }
 #pragma unroll
       for (int q = 0; q < TPW; ++q) {
         int const t = wave + q * (UT / 64);
         if (t < NVT) {
           f32x4 e = {0.f, 0.f, 0.f, 0.f};
-          e = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], bw[kd], e, 0, 0, 0);
+          e = mfma32(a1[q], bw0, e);
+          e = mfma32(a2[q], bw1, e);
           // D: lane holds voxels 16 t + 4 kq + r of tap i
           if (i < 9) *(f32x4 *)(se + i * NVP + t * 16 + 4 * kq) = e;
         }
       }
-      __syncthreads();
-#pragma unroll
-      for (int kh = 0; kh < 3; ++kh)
-#pragma unroll
-        for (int kw = 0; kw < 3; ++kw)
-          acc += se[(kh * 3 + kw) * NVP + (row + kh) * PW + col + kw];
     }
+    __syncthreads();  // F is complete; every wave is done with the ring
+    if (j + 1 < HDB) store_plane(d + 2, nxt);  // into the slot of plane d - 1
+    float acc = 0.f;
+#pragma unroll
+    for (int kh = 0; kh < 3; ++kh)
+#pragma unroll
+      for (int kw = 0; kw < 3; ++kw)
+        acc += se[(kh * 3 + kw) * NVP + (row + kh) * PW + col + kw];
     if (live) {
       float const o = (acc + b0) * unscale;
       dst[v] = base ? bv + o : o;
-    }
-    if (j + 1 < HDB) {
-      __syncthreads();  // every wave is done with plane d - 1
-      store_plane(d + 2, nxt);
     }
   }
 }
