@@ -1845,8 +1845,13 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
 // tiled U / V; one plane at a time per block, its operands in LDS.
 // ---------------------------------------------------------------------------
 namespace x3 {
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][8 (l / 16) + e] and
+// B[8 (l / 16) + e][l % 16], e < 8; D[4 (l / 16) + r][l % 16], r < 4
+__device__ __forceinline__ f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
   unsigned int const u = __float_as_uint(f);
@@ -1855,14 +1860,16 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 __device__ __forceinline__ float bf2f(unsigned short h) {
   return __uint_as_float((unsigned int)h << 16);
 }
-__device__ __forceinline__ void split4(float const (&v)[4], bf16x4 &hi, bf16x4 &lo) {
+template <int N, typename V>
+__device__ __forceinline__ void split(float const (&v)[N], V &hi, V &lo) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < N; ++r) {
     unsigned short const h = f2bf(v[r]);
     hi[r] = (short)h;
     lo[r] = (short)f2bf(v[r] - bf2f(h));
   }
 }
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
 // coefficient of output part po (0 re, 1 im) from input part pi of the
 // complex factor c: [[Cr, -Ci], [Ci, Cr]]
 __device__ __forceinline__ float part(float2 c, int po, int pi) {
@@ -1890,8 +1897,10 @@ struct Shape {
 
 // The split data live in LDS as bf16 hi / lo arrays per part (real,
 // imaginary): s[(part * 2 + lo) * BL + row * BP + col]. A split array is
-// read 4 consecutive elements (8 bytes) at a time along its rows.
-__device__ __forceinline__ bf16x4 ld4(uint16_t const *p) { return *(bf16x4 const *)p; }
+// read 8 consecutive elements (16 bytes) at a time along its rows (the row
+// pitch BP = KP + 8 puts the eight rows of a read cycle in distinct bank
+// groups).
+__device__ __forceinline__ bf16x8 ld8(uint16_t const *p) { return *(bf16x8 const *)p; }
 
 // Y = Z C^T, C[o][q] = e^{SIGN 2 pi i o q / P} for o < NC, q < KC (else 0):
 // Z split row-major ([row][q]), rows r16(MZ); Y written split and
@@ -1901,9 +1910,9 @@ template <int P, int MZ, int KC, int NC, int SIGN>
 struct Right {
   static constexpr int BP = Shape<P>::BP, BL = Shape<P>::BL;
   static constexpr int KMZ = r16(MZ), KKC = r16(KC), KNC = r16(NC);
-  static constexpr int NNT = 2 * KNC / 16, NKS = 2 * KKC / 16, NMT = KMZ / 16;
+  static constexpr int NNT = 2 * KNC / 16, NKS = 2 * KKC / 32, NMT = KMZ / 16;
   static constexpr int TPW = (NNT + NTHR / 64 - 1) / (NTHR / 64);
-  bf16x4 h[TPW][NKS], l[TPW][NKS];
+  bf16x8 h[TPW][NKS], l[TPW][NKS];
 
   __device__ __forceinline__ void init() {
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1915,16 +1924,16 @@ struct Right {
       int const po = nc / KNC, o = nc - po * KNC;
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks) {
-        float v[4];
+        float v[8];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int const kk = ks * 16 + 4 * kq + r;
+        for (int r = 0; r < 8; ++r) {
+          int const kk = ks * 32 + 8 * kq + r;
           int const pi = kk / KKC, q = kk - pi * KKC;
           float2 const c = (nt < NNT && o < NC && q < KC) ? coef<P, SIGN>(o * q)
                                                           : make_float2(0.f, 0.f);
           v[r] = part(c, po, pi);
         }
-        split4(v, h[t][ks], l[t][ks]);
+        split<8>(v, h[t][ks], l[t][ks]);
       }
     }
   }
@@ -1944,19 +1953,19 @@ struct Right {
         int const row = mt * 16 + i;
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
-          int const kk = ks * 16 + 4 * kq;
+          int const kk = ks * 32 + 8 * kq;  // 8 elements of one part (KKC % 16 == 0)
           int const pi = kk / KKC, q = kk - pi * KKC;
-          bf16x4 const ah = ld4(z + (pi * 2) * BL + row * BP + q);
-          bf16x4 const al = ld4(z + (pi * 2 + 1) * BL + row * BP + q);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, h[t][ks], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, l[t][ks], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, h[t][ks], acc, 0, 0, 0);
+          bf16x8 const ah = ld8(z + (pi * 2) * BL + row * BP + q);
+          bf16x8 const al = ld8(z + (pi * 2 + 1) * BL + row * BP + q);
+          acc = mma(ah, h[t][ks], acc);
+          acc = mma(ah, l[t][ks], acc);
+          acc = mma(al, h[t][ks], acc);
         }
         // D: rows mt 16 + 4 kq + r of column nc: 4 consecutive rows of the
         // column-major output
         float const v[4] = {acc[0], acc[1], acc[2], acc[3]};
         bf16x4 vh, vl;
-        split4(v, vh, vl);
+        split<4>(v, vh, vl);
         int const off = o * BP + mt * 16 + 4 * kq;
         *(bf16x4 *)(y + (po * 2) * BL + off) = vh;
         *(bf16x4 *)(y + (po * 2 + 1) * BL + off) = vl;
@@ -1974,9 +1983,9 @@ struct Left {
   static constexpr int BP = Shape<P>::BP, BL = Shape<P>::BL;
   static constexpr int FP = Shape<P>::FP, FL = Shape<P>::FL;
   static constexpr int KMC = r16(MC), KKC = r16(KC), KNZ = r16(NZ);
-  static constexpr int NMT = 2 * KMC / 16, NKS = 2 * KKC / 16, NNT = KNZ / 16;
+  static constexpr int NMT = 2 * KMC / 16, NKS = 2 * KKC / 32, NNT = KNZ / 16;
   static constexpr int TPW = (NMT + NTHR / 64 - 1) / (NTHR / 64);
-  bf16x4 h[TPW][NKS], l[TPW][NKS];
+  bf16x8 h[TPW][NKS], l[TPW][NKS];
 
   __device__ __forceinline__ void init() {
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1988,16 +1997,16 @@ struct Left {
       int const po = mr / KMC, pr = mr - po * KMC;
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks) {
-        float v[4];
+        float v[8];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int const kk = ks * 16 + 4 * kq + r;
+        for (int r = 0; r < 8; ++r) {
+          int const kk = ks * 32 + 8 * kq + r;
           int const pi = kk / KKC, q = kk - pi * KKC;
           float2 const c = (mt < NMT && pr < MC && q < KC) ? coef<P, SIGN>(pr * q)
                                                            : make_float2(0.f, 0.f);
           v[r] = part(c, po, pi);
         }
-        split4(v, h[t][ks], l[t][ks]);
+        split<8>(v, h[t][ks], l[t][ks]);
       }
     }
   }
@@ -2015,13 +2024,13 @@ struct Left {
         int const col = nt * 16 + i;
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
-          int const kk = ks * 16 + 4 * kq;
+          int const kk = ks * 32 + 8 * kq;  // 8 elements of one part (KKC % 16 == 0)
           int const pi = kk / KKC, q = kk - pi * KKC;
-          bf16x4 const bh = ld4(z + (pi * 2) * BL + col * BP + q);
-          bf16x4 const bl = ld4(z + (pi * 2 + 1) * BL + col * BP + q);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bh, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h[t][ks], bl, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(l[t][ks], bh, acc, 0, 0, 0);
+          bf16x8 const bh = ld8(z + (pi * 2) * BL + col * BP + q);
+          bf16x8 const bl = ld8(z + (pi * 2 + 1) * BL + col * BP + q);
+          acc = mma(h[t][ks], bh, acc);
+          acc = mma(h[t][ks], bl, acc);
+          acc = mma(l[t][ks], bh, acc);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
