@@ -749,23 +749,35 @@ __global__ __launch_bounds__(UT, 2) void k_conv3s(uint16_t const *__restrict__ x
         }
     }
   };
-  uint4 nxt[NPK];
+  // AH2: two planes in flight per block (nxt, nx2), since the loads a CU
+  // keeps in flight set this layer's HBM rate (with pooling the registers
+  // would spill: one plane)
+  constexpr bool AH2 = !POOL;
+  uint4 nxt[NPK], nx2[AH2 ? NPK : 1];
   {
     uint4 v0[NPK];
     load_plane(d0 - 1, v0);
     load_plane(d0, nxt);
+    if constexpr (AH2) load_plane(d0 + 1, nx2);
     store_plane(sx, v0);
   }
   __syncthreads();
   // input plane p = d0 - 1 + j is in buffer j & 1; nxt holds plane p + 1,
   // stored into the other buffer (free: every wave passed the barrier after
-  // reading it) before plane p + 2 is loaded into nxt
+  // reading it); then (AH2) nx2's plane p + 2 moves to nxt while plane p + 3
+  // is loaded into nx2, or plane p + 2 is loaded into nxt
 #pragma unroll
   for (int j = 0; j < DB + 2; ++j) {
     uint16_t const *const cur = sx + (j & 1) * PLANE;
     if (j + 1 < DB + 2) {
       store_plane(sx + ((j + 1) & 1) * PLANE, nxt);
-      if (j + 2 < DB + 2) load_plane(d0 + 1 + j, nxt);
+      if constexpr (AH2) {
+#pragma unroll
+        for (int k = 0; k < NPK; ++k) nxt[k] = nx2[k];
+        if (j + 3 < DB + 2) load_plane(d0 + 2 + j, nx2);
+      } else if (j + 2 < DB + 2) {
+        load_plane(d0 + 1 + j, nxt);
+      }
       __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
     }
 #pragma unroll
