@@ -904,18 +904,23 @@ __global__ __launch_bounds__(UT) void k_conv_first(float const *__restrict__ src
     }
   };
   float const bsel = bias[i];
-  float nv[NPK];
+  // two planes in flight: nv holds plane d + 1 (stored at the top of the
+  // iteration), nv2 plane d + 2 (moved to nv while plane d + 3 is loaded)
+  float nv[NPK], nv2[NPK];
   load_plane(d0 - 1, nv);
   store_plane(d0 - 1, nv);
   load_plane(d0, nv);
   store_plane(d0, nv);
   load_plane(d0 + 1, nv);
+  load_plane(d0 + 2, nv2);
   int const h = h0 + wave;
   uint16_t *const obw = ob + wave * 64 * OBS;
   for (int j = 0; j < FDB; ++j) {
     int const d = d0 + j;
     store_plane(d + 1, nv);
-    if (j + 1 < FDB) load_plane(d + 2, nv);
+#pragma unroll
+    for (int k = 0; k < NPK; ++k) nv[k] = nv2[k];
+    if (j + 2 < FDB) load_plane(d + 3, nv2);
     __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the compute
     __syncthreads();
     f32x4 acc[4];
