@@ -132,7 +132,8 @@ int agdg_blend_traces(AgdDev const *g, int resid);
 int agdg_adj_time(AgdDev const *g, float const *Tin, int plain);
 /* mode 0: X = X - alpha*G then shrink by lamb; mode 1: X = -alpha*G and
  * record max |X|^2 bits in scalar[0]; mode 2: X = X - alpha*G (no shrink);
- * mode 3: X = G. fuse: also write the forward spatial transform of the new X
+ * mode 3: X = G; mode 4: as mode 1 without writing X where the kernels
+ * allow it (X then holds the previous model or -alpha*G). fuse: also write the forward spatial transform of the new X
  * into U (the next iteration's first forward stage). */
 int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fuse);
 /* 1 when every spatial class has a specialised kernel (fusion available) */

@@ -2406,7 +2406,10 @@ EXPORT int agdx_run_warm_dev(AgdProblem *p, float f, int n_its, int print_freq,
 
 /* The zero start's first update x = 0 - alpha A^H d into dev.X and its
  * threshold lambda0 = max |x| (agdeblend.c:584-590, model.c:1202-1227). */
-static int lambda0_update(AgdProblem *p, float alpha, float *lamb) {
+/* lambda0 = max |alpha A^H d| (the zero start's first update); keep_x: also
+ * leave that update -alpha A^H d in dev.X (else dev.X is left to be
+ * overwritten: a warm start or a given initial model) */
+static int lambda0_update(AgdProblem *p, float alpha, float *lamb, int keep_x) {
   unsigned int bits = 0;
   float l2;
   float *tmp;
@@ -2415,7 +2418,7 @@ static int lambda0_update(AgdProblem *p, float alpha, float *lamb) {
   /* the adjoint reads its record through dev.r: point it at d */
   tmp = p->dev.r;
   p->dev.r = p->dev.d;
-  rc = adjoint_update(p, alpha, 0.0f, 1);
+  rc = adjoint_update(p, alpha, 0.0f, keep_x ? 1 : 4);
   p->dev.r = tmp;
   if (rc || agdg_d2h(&bits, p->dev.scalar, sizeof(bits))) return 1;
   memcpy(&l2, &bits, sizeof(l2));
@@ -2450,7 +2453,7 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
   } else {
     /* x = 0 - alpha * A^H d; lambda0 = max |x| (for a warm start or x0 the
      * schedule keeps the scale of the zero start) */
-    if (lambda0_update(p, alpha, &lamb)) return 1;
+    if (lambda0_update(p, alpha, &lamb, zero_start)) return 1;
     if (x0 != NULL && agdg_h2d(p->dev.X, x0, (size_t)agdx_model_len(p) * 8)) return 1;
   }
   if (warm_any) {
@@ -2474,7 +2477,7 @@ static int run_impl(AgdProblem *p, float f, int n_its, int print_freq, float con
       g_last_warm_fallback = 1;
       r2_start = -1.0;
       zero_start = 1;
-      if (lambda0_update(p, alpha, &lamb)) return 1;
+      if (lambda0_update(p, alpha, &lamb, 1)) return 1;
     }
   }
   g_last_lambda0 = lamb;

@@ -1357,6 +1357,8 @@ __global__ __launch_bounds__(NTHR) void k_fwd_spatial_t(
 //   1  x <- -alpha g, max |x|^2 into maxbits (first iteration)
 //   2  x <- x - alpha g
 //   3  x <- g
+//   4  max |alpha g|^2 into maxbits, x untouched (the threshold scale of a
+//      start whose model is written afterwards)
 // With FUSE (modes 0 and 3) the new model's forward spatial transform, the
 // next forward's first stage, is written to U.
 template <int P0, int P1, bool TILED, int MODE, bool FUSE>
@@ -1396,7 +1398,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
       float2 nv;
       if constexpr (MODE == 3) {
         nv = gv;
-      } else if constexpr (MODE == 1) {
+      } else if constexpr (MODE == 1 || MODE == 4) {
         nv = make_float2(0.f - alpha * gv.x, 0.f - alpha * gv.y);
         mx = fmaxf(mx, nv.x * nv.x + nv.y * nv.y);
       } else {
@@ -1411,7 +1413,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
           nv.y *= sh;
         }
       }
-      xc[c0 * P1] = nv;
+      if constexpr (MODE != 4) xc[c0 * P1] = nv;
       v[c0] = nv;
     }
     if constexpr (FUSE) {
@@ -1420,7 +1422,7 @@ __global__ __launch_bounds__(NTHR) void k_adj_spatial_t(
       for (int n0 = 0; n0 < W0; ++n0) sm[(k * W0 + n0) * LD + c1] = v[n0];
     }
   }
-  if constexpr (MODE == 1) {
+  if constexpr (MODE == 1 || MODE == 4) {
     __shared__ float red[NTHR / 64];
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
@@ -2268,6 +2270,7 @@ static int adj_spatial_layout(AgdDev const *g, AgdClass const &c, float alpha, f
                   : launch_adj_spatial<P0, P1, TILED, 0, false>(g, c, alpha, lamb);
     case 1: return launch_adj_spatial<P0, P1, TILED, 1, false>(g, c, alpha, lamb);
     case 2: return launch_adj_spatial<P0, P1, TILED, 2, false>(g, c, alpha, lamb);
+    case 4: return launch_adj_spatial<P0, P1, TILED, 4, false>(g, c, alpha, lamb);
     default:
       return fuse ? launch_adj_spatial<P0, P1, TILED, 3, true>(g, c, alpha, lamb)
                   : launch_adj_spatial<P0, P1, TILED, 3, false>(g, c, alpha, lamb);
@@ -2601,8 +2604,10 @@ int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fus
         fprintf(stderr, "ERROR: fused update requested with split-precision transforms\n");
         return 1;
       }
-      if (c.p0 == 20 ? launch_adj_x3<20>(g, c, alpha, lamb, mode)
-                     : launch_adj_x3<40>(g, c, alpha, lamb, mode))
+      // mode 4 as mode 1 here (the model is written, then overwritten)
+      int const m = mode == 4 ? 1 : mode;
+      if (c.p0 == 20 ? launch_adj_x3<20>(g, c, alpha, lamb, m)
+                     : launch_adj_x3<40>(g, c, alpha, lamb, m))
         return 1;
       continue;
     }
@@ -2624,7 +2629,7 @@ int agdg_adj_spatial(AgdDev const *g, float alpha, float lamb, int mode, int fus
           launch("adj_spatial", k_adj_spatial, dim3(n), dim3(NTHR), g->lds_spatial,
                  g->win, g->tasks, first, g->plans, (float2 const *)g->roots,
                  (float2 const *)g->V, (float2 *)g->X, alpha, lamb,
-                 (float const *)g->lamb_dev, mode, g->scalar))
+                 (float const *)g->lamb_dev, mode == 4 ? 1 : mode, g->scalar))
         return 1;
     }
   }
