@@ -1798,35 +1798,42 @@ __global__ __launch_bounds__(time_threads<M1 * M2>()) void k_adj_time_t(
     }
     __syncthreads();
     // half spectrum of the real window, x2 on the interior bins; the pair
-    // (k, M - k) shares its two inputs
+    // (k, M - k) shares its two inputs. A thread takes two neighbouring
+    // traces of a bin pair, so each of its stores is 16 bytes.
     float2 *vo = vt + (int64_t)tw * tw_step;
-    for (int i = tid; i < (M / 2 + 1) * TG; i += NT) {
-      int const k = i / TG;
-      int const j = i - k * TG;
+    for (int i = tid; i < (M / 2 + 1) * (TG / 2); i += NT) {
+      int const k = i / (TG / 2);
+      int const j0p = 2 * (i - k * (TG / 2));
       int const kc = (k == 0) ? 0 : M - k;
-      float2 const za = Z[j * LDL + fft2_pos<M1, M2>(k)];
-      float2 const zb = Z[j * LDL + fft2_pos<M1, M2>(kc)];
-      // X[k] from (za, conj zb), X[M - k] from (zb, conj za)
-      float2 const e1 = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
-      float2 const o1 = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
-      float2 X1 = agdfft::c_add(e1, agdfft::c_mul(o1, RP[k]));
-      float2 X2 = agdfft::c_add(make_float2(e1.x, -e1.y),
-                                agdfft::c_mul(make_float2(o1.x, -o1.y), RP[M - k]));
-      if (k == 0) {
-        // DC, and Nyquist (RP[M] = -1), both real
-        X1.y = 0.f;
-        X2 = make_float2(za.x - za.y, 0.f);
+      float2 X1[2], X2[2];
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) {
+        int const j = j0p + jj;
+        float2 const za = Z[j * LDL + fft2_pos<M1, M2>(k)];
+        float2 const zb = Z[j * LDL + fft2_pos<M1, M2>(kc)];
+        // X[k] from (za, conj zb), X[M - k] from (zb, conj za)
+        float2 const e1 = make_float2(0.5f * (za.x + zb.x), 0.5f * (za.y - zb.y));
+        float2 const o1 = make_float2(0.5f * (za.y + zb.y), -0.5f * (za.x - zb.x));
+        X1[jj] = agdfft::c_add(e1, agdfft::c_mul(o1, RP[k]));
+        X2[jj] = agdfft::c_add(make_float2(e1.x, -e1.y),
+                               agdfft::c_mul(make_float2(o1.x, -o1.y), RP[M - k]));
+        if (k == 0) {
+          // DC, and Nyquist (RP[M] = -1), both real
+          X1[jj].y = 0.f;
+          X2[jj] = make_float2(za.x - za.y, 0.f);
+        }
+        if (k >= 1 && k <= kmax2) {
+          X1[jj].x *= two;
+          X1[jj].y *= two;
+        }
+        if (M - k >= 1 && M - k <= kmax2) {
+          X2[jj].x *= two;
+          X2[jj].y *= two;
+        }
       }
-      if (k >= 1 && k <= kmax2) {
-        X1.x *= two;
-        X1.y *= two;
-      }
-      if (M - k >= 1 && M - k <= kmax2) {
-        X2.x *= two;
-        X2.y *= two;
-      }
-      vo[k * TG + j] = X1;
-      if (k != M - k) vo[(M - k) * TG + j] = X2;
+      *(float4 *)(vo + k * TG + j0p) = make_float4(X1[0].x, X1[0].y, X1[1].x, X1[1].y);
+      if (k != M - k)
+        *(float4 *)(vo + (M - k) * TG + j0p) = make_float4(X2[0].x, X2[0].y, X2[1].x, X2[1].y);
     }
     __syncthreads();
   }
