@@ -1136,19 +1136,22 @@ __global__ void k_flip_weights(uint16_t const *__restrict__ wt, uint16_t *__rest
 // consecutive (n, d, h) rows and a group of taps (NKH = 3: the nine taps of
 // one kd, grid.y = kd; NKH = 1: the three taps of one (kd, kh)). Per row it
 // stages dy transposed (dys[co][v]) and the input rows transposed once per
-// kw shift (xs[r][kw][ci][v]), so both operands are aligned 8-byte LDS
-// reads; waves split the (tap, co tile, ci tile) products and the four
-// 16-voxel chunks of the row. The block's partial sums (fixed-order
+// kw shift (xs[r][kw][ci][v]), so both operands are aligned 16-byte LDS
+// reads (K = 32 voxels per matrix-core step); waves split the (tap, co
+// tile, ci tile) products and the two 32-voxel chunks of the row. The block's partial sums (fixed-order
 // reduction over its waves) go to part[block][27][CO][CI] (taps of its
 // group only) and, for grid.y == 0, the bias partials to bpart[block][CO].
 template <int CI, int CO, int NKH>
 struct WgradShape {
   static constexpr int NTAP = NKH * 3;
   static constexpr int NTILE = NTAP * (CO / 16) * (CI / 16);
-  static constexpr int TG = NTILE <= 32 ? 1 : (NTILE <= 64 ? 2 : 4);  // waves over tiles
-  static constexpr int NCS = 4 / TG;                                    // chunk groups
-  static constexpr int NPW = (NTILE + TG - 1) / TG;                     // tiles per wave
-  static constexpr int LR = 68;  // LDS row: 64 voxels + 4 (spreads the banks)
+  static constexpr int TG = NTILE <= 64 ? 2 : 4;  // waves over tiles
+  static constexpr int NCS = 4 / TG;               // chunk groups (of the two 32-voxel chunks)
+  static constexpr int CPW = 2 / NCS;              // chunks per wave
+  static constexpr int NPW = (NTILE + TG - 1) / TG;  // tiles per wave
+  // LDS row: 64 voxels + 8: 16-byte aligned reads, and the eight rows of a
+  // read cycle in distinct bank groups
+  static constexpr int LR = 72;
   static constexpr int XS = NKH * 3 * CI * LR;                          // bf16 elements
   static constexpr int DS = CO * LR;
   static constexpr int RED = (NCS > 1 ? TG * NPW * 256 : 0) > 256       // floats
@@ -1244,8 +1247,8 @@ __global__ __launch_bounds__(UT) void k_conv3_wgrad(
       }
     }
 #pragma unroll
-    for (int cc = 0; cc < S::TG; ++cc) {
-      int const c = cs + cc * S::NCS;  // 16-voxel chunk
+    for (int cc = 0; cc < S::CPW; ++cc) {
+      int const c = cs + cc * S::NCS;  // 32-voxel chunk
 #pragma unroll
       for (int j = 0; j < S::NPW; ++j) {
         int const t = tg + j * S::TG;
@@ -1253,9 +1256,9 @@ __global__ __launch_bounds__(UT) void k_conv3_wgrad(
         int const ct = t % (CO / 16);
         int const it = (t / (CO / 16)) % (CI / 16);
         int const tap = t / ((CO / 16) * (CI / 16));  // r * 3 + kw
-        bf16x4 const a = *(bf16x4 const *)(dys + (ct * 16 + i) * LR + c * 16 + 4 * kq);
-        bf16x4 const b = *(bf16x4 const *)(xs + (tap * CI + it * 16 + i) * LR + c * 16 + 4 * kq);
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        u32x4 const a = *(u32x4 const *)(dys + (ct * 16 + i) * LR + c * 32 + 8 * kq);
+        u32x4 const b = *(u32x4 const *)(xs + (tap * CI + it * 16 + i) * LR + c * 32 + 8 * kq);
+        acc[j] = mfma32(a, b, acc[j]);
       }
     }
   }
