@@ -121,6 +121,13 @@ struct ConvDepth {
   static constexpr int DB = CO <= 16 ? ((CU && !UPC) ? 2 : 4) : (CO <= 32 ? 2 : 1);
 };
 
+// waves per SIMD the registers are bounded for: two where the LDS allows two
+// blocks per CU, else one (the 64-channel layers, d2a on combined taps)
+template <int CO, bool UPC>
+struct ConvWaves {
+  static constexpr int W = (CO >= 64 || (UPC && CO >= 32)) ? 1 : 2;
+};
+
 // Combined taps of a 3x3x3 convolution over a nearest x2 upsampled input.
 // Along one axis, output index o = 2 c + p reads input o + k - 1 (k = 0, 1,
 // 2), i.e. coarse c + floor((p + k - 1) / 2): for p = 0 the coarse offsets
@@ -142,7 +149,7 @@ __host__ __device__ constexpr int up_tap(int pd, int ph, int a, int b, int kw) {
 // k_up_weights from wt) instead of wt's 27: per 16-channel chunk 12
 // combined taps, 6 matrix-core steps of two, instead of 27 taps in 14.
 template <int CI, int CO, int CU, bool UPC = false>
-__global__ __launch_bounds__(UT, 2) void k_conv3(
+__global__ __launch_bounds__(UT, (ConvWaves<CO, UPC>::W)) void k_conv3(
     uint16_t const *__restrict__ x, uint16_t const *__restrict__ xu,
     uint16_t const *__restrict__ wt, uint16_t const *__restrict__ wcu,
     float const *__restrict__ bias, uint16_t const *__restrict__ gate,
