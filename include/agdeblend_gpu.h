@@ -152,6 +152,11 @@ int agdx_set_time_variant(AgdProblem *p, int c, int var);
 /* Sets time class c to the factorization m1 x m2 of its half length (one
  * of the autotuner's candidates). */
 int agdx_set_time_factors(AgdProblem *p, int c, int m1, int m2);
+/* The time kernels' blocks, 4 ints each into out (or NULL): window, first
+ * trace, and the run [tw0, tw1) of the window's time windows the block
+ * transforms (all of them unless AGD_TIME_TW sets runs, or the window-chunk
+ * order AGD_CHUNK_MB does: runs of 4). Returns their number. */
+int agdx_time_blocks(AgdProblem const *p, int *out);
 /* Device copy of the patch data set by agdx_set_data: all patches
  * concatenated in patch order, agdx_data_len floats (valid until
  * agdx_warm_model, agdx_finish or agdx_get_output overwrite it). */
