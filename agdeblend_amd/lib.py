@@ -129,7 +129,8 @@ EXPORTS = ["agd_deblend", "agd_blend", "agdeblend_deblend", "agdeblend_blend", "
            "agdx_set_data", "agdx_forward", "agdx_adjoint", "agdx_run", "agdx_run_warm",
            "agdx_warm_model", "agdx_set_spatial_precision", "agdx_autotune",
            "agdx_n_spatial_classes", "agdx_n_time_classes", "agdx_time_factors",
-           "agdx_set_time_factors", "agdx_time_variants", "agdx_set_time_variant", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
+           "agdx_set_time_factors", "agdx_time_blocks", "agdx_time_variants",
+           "agdx_set_time_variant", "agdx_data_dev", "agdx_data_len", "agdx_warm_model_dev",
            "agdx_run_warm_dev", "agdx_get_output",
            "agdx_finish",
            "agdx_get_model", "agdx_trace_len", "agdx_operator", "agdx_iterate", "agdx_last_lambda0", "agdx_last_warm_fallback", "agdx_last_warm_ratio",
